@@ -1,0 +1,124 @@
+/*
+ * lem.h -- C-ABI of liblem.so, the gfx950 (MI355X) library behind the LyceanEM
+ * raycast + frequency-domain propagation path.
+ *
+ * The reference binds this path two ways: numba.cuda kernels launched from
+ * Python (kernel1Dv3, freqdomainkernal) and the pybind11 module `lyceanem.em`
+ * (lyceanem/src/frequency_domain_models.cu:314-320).  This header replaces both
+ * with plain `extern "C"` entry points that ctypes (or any FFI) can bind.
+ *
+ * Conventions
+ *   - Every buffer is a caller-owned DEVICE pointer (e.g. torch.Tensor.data_ptr()
+ *     of a contiguous tensor on the current HIP device).  Points are N x 3 float32
+ *     (x, y, z rows, like the reference's `unified_model`), complex weights are
+ *     N x 3 complex64 (interleaved re, im), triangles are T x 9 float32
+ *     (v0, v1, v2 -- the layout of base_types.triangle_t, base_types.py:21-38).
+ *   - `stream` is a hipStream_t (NULL = the default stream); all work is enqueued
+ *     on it and the call returns without synchronising.
+ *   - Return value: 0 on success, a negative LEM_E_* code otherwise, with a
+ *     message in lem_last_error() (thread-local).  Nothing here calls exit() or
+ *     abort(), unlike gpuErrchk (lyceanem/src/gpu_error.cuh:2-10).
+ *   - No entry point allocates device memory; scratch space comes from a
+ *     caller-supplied workspace whose size a *_workspace() query returns.
+ */
+#ifndef LEM_H
+#define LEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LEM_ABI_VERSION 1
+
+#define LEM_OK 0
+#define LEM_E_INVALID (-1)   /* bad argument (sizes, null pointers, flags) */
+#define LEM_E_HIP (-2)       /* HIP runtime or launch error */
+#define LEM_E_WORKSPACE (-3) /* workspace missing or too small */
+
+/* Predicate semantics.
+ * LEM_SEM_NUMBA : numba `hit`, rayfunctions.py:37-104 -- two-sided,
+ *                 EPSILON = 1e-7, float32 edges/cross/dot, float64 F, U, V, t,
+ *                 accept window (2e-7, dist - 2e-7), kernel1Dv3's shrinking
+ *                 closest-hit `dist` (rayfunctions.py:212-244).
+ * LEM_SEM_NATIVE: native `hit`, lyceanem/src/raycasting.cuh:19-68 -- back-face
+ *                 culled, EPSILON = 1e-6f, all float32, any-hit only. */
+#define LEM_SEM_NUMBA 0
+#define LEM_SEM_NATIVE 1
+
+/* Visibility flags. */
+#define LEM_VIS_ANY_HIT 1   /* stop each ray at its first accepted hit (mask only) */
+#define LEM_VIS_SKIP_SELF 2 /* never test pairs with source row == sink row
+                               (native flag = !self_to_self,
+                               raycasting_accelerated.cuh:72) */
+
+/* Field flags. */
+#define LEM_FIELD_SINK_TANGENT 1 /* project onto the sink tangent plane, as
+                                    freqdomainkernal does for line-of-sight rows
+                                    of a 3-column index (empropagation.py:1150-1161) */
+#define LEM_FIELD_ELEMENTS 2     /* write the S x K x 3 network instead of the
+                                    source sum (frequency_domain.py:703-716) */
+
+int lem_abi_version(void);
+const char *lem_last_error(void);
+/* Writes the current device's gcnArchName (e.g. "gfx950:sramecc+:xnack-"). */
+int lem_device_arch(char *buf, int len);
+
+/* Dense S x K brute-force visibility.
+ * Replaces the first raycast wave of workchunkingv2 (rayfunctions.py:2681-2783:
+ * create_model_index -> charge_rays_environment1Dv2 -> kernel1Dv3 -> rayHits1Dv2)
+ * for source -> sink rays, without materialising the S*K index or ray payload.
+ * src: S x 3, snk: K x 3, tris: T x 9.
+ * hit_bits: S x ceil(K/64) uint64 words; bit k%64 of word k/64 of row s is the
+ *   kernel1Dv3 `intersect` flag of ray s -> k (1 = blocked).
+ * dist: S x K float32 kernel1Dv3 `dist` (NULL to skip; needs the closest-hit
+ *   mode with LEM_SEM_NUMBA, i.e. flags without LEM_VIS_ANY_HIT). */
+int lem_visibility_dense(const float *src, int64_t S, const float *snk, int64_t K,
+                         const float *tris, int64_t T, int semantics, int flags,
+                         uint64_t *hit_bits, float *dist, void *stream);
+
+/* Explicit ray list: the kernel1Dv3 launch of launchRaycaster1Dv3 /
+ * chunkingRaycaster1Dv3 (rayfunctions.py:1951-2017, 2102-2369) including the
+ * ray charging of charge_rays_environment1Dv2 (rayfunctions.py:1790-1842).
+ * points: N x 3 unified point table; pairs: R x 2 int32 0-based
+ * (origin row, target row).  intersect: R bytes (0/1, 0xFF for a pair outside
+ * the table); dist: R float32 or NULL. */
+int lem_raycast_pairs(const float *points, int64_t n_points, const int32_t *pairs,
+                      int64_t n_rays, const float *tris, int64_t n_tris, int semantics,
+                      int flags, uint8_t *intersect, float *dist, void *stream);
+
+/* Line-of-sight field, fused over the visibility mask.
+ * Replaces freqdomainkernal (empropagation.py:1083-1265) for 2-column paths
+ * (and the line-of-sight rows of a 3-column index) plus the host source sum
+ * `np.dot(ones(S), scatter_map[:, :, c])` (frequency_domain.py:647-649).
+ * src: S x 3, src_w: S x 3 complex64; snk, snk_n: K x 3; snk_w: K x 3 complex64
+ * or NULL (weight 1, frequency_domain.py:398-400); hit_bits as produced by
+ * lem_visibility_dense, or NULL (all visible).
+ * out: K x 3 complex64 (source sum), or S x K x 3 complex64 with
+ * LEM_FIELD_ELEMENTS.  workspace: lem_field_los_workspace() bytes. */
+int64_t lem_field_los_workspace(int64_t S, int64_t K, int flags);
+int lem_field_los(const float *src, const float *src_w, int64_t S, const float *snk,
+                  const float *snk_n, const float *snk_w, int64_t K, const uint64_t *hit_bits,
+                  double alpha, double beta, int flags, float *out, void *workspace,
+                  int64_t workspace_bytes, void *stream);
+
+/* Generic path list: the network EMGPUFreqDomain returns
+ * (empropagation.py:2197-2410).  pos, nrm: N x 3; w: N x 3 complex64 (the
+ * scattering_t table, base_types.py:89-119); index: V x ncol int32, 1-based,
+ * 0 = absent point (workchunkingv2's full_index).  Paths are grouped by their
+ * (source, sink) slot from targettingindex (empropagation.py:3394-3429):
+ * `order` lists the path rows slot by slot, seg_start (nseg + 1) bounds each
+ * slot's run, seg_slot = s * K + k.  out: S x K x 3 complex128, only the listed
+ * slots are written (zero it first).  err_flag (device int, zeroed by the caller)
+ * is set to 1 if a row points outside the table. */
+int lem_field_paths(const float *pos, const float *nrm, const float *w, int64_t N,
+                    const int32_t *index, int ncol, const int64_t *order,
+                    const int64_t *seg_start, const int64_t *seg_slot, int64_t nseg,
+                    double alpha, double beta, double *out, int *err_flag, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LEM_H */

@@ -1,0 +1,376 @@
+// lem_fields.hip -- fused, visibility-masked polarised Rayleigh-Sommerfeld
+// propagation for gfx950.
+//
+// Replaces freqdomainkernal (empropagation.py:1083-1265) behind
+// EMGPUFreqDomain (empropagation.py:2197-2410) and the native em_wave /
+// ray_launch pair (em.cuh:18-120).  Per (source s, sink k) the contribution is
+//
+//     L_sk * (I - d d^T) w_s          d = (p_k - p_s)/r
+//     L_sk = -(1/2pi) (-(alpha + j beta) - 1/r) e^{-(alpha + j beta) r}/r (d . n_k)
+//
+// then multiplied componentwise by the sink weight and (for the 3-column index
+// of calculate_scattering with scattering > 0) projected onto the sink tangent
+// plane -- both linear and source-independent, so the source sum applies them
+// once per sink.  r and the phase are float64; the rest is float32 with Kahan
+// accumulation over sources; chunk partials merge in a fixed order, so results
+// are deterministic (the reference adds with float64 atomics in arbitrary order).
+#include "lem_device.h"
+#include "lem_internal.h"
+#include "../../include/lem.h"
+
+namespace lem {
+
+constexpr int FB = 256;       // sinks per block
+constexpr int FS_TILE = 256;  // sources per LDS tile
+
+struct SrcLds {
+    double x, y, z;
+    float w[6];
+};
+
+LEM_DEV C3 load_w(const float *__restrict__ w, int64_t i)
+{
+    const float2 *p = reinterpret_cast<const float2 *>(w) + 3 * i;
+    float2 a = p[0], b = p[1], c = p[2];
+    C3 o;
+    o.xr = a.x; o.xi = a.y; o.yr = b.x; o.yi = b.y; o.zr = c.x; o.zi = c.y;
+    return o;
+}
+
+LEM_DEV C3 cmul_comp(const C3 &a, const C3 &b)
+{
+    C3 o;
+    o.xr = a.xr * b.xr - a.xi * b.xi; o.xi = a.xr * b.xi + a.xi * b.xr;
+    o.yr = a.yr * b.yr - a.yi * b.yi; o.yi = a.yr * b.yi + a.yi * b.yr;
+    o.zr = a.zr * b.zr - a.zi * b.zi; o.zi = a.zr * b.zi + a.zi * b.zr;
+    return o;
+}
+
+LEM_DEV C3 scale(const C3 &a, float lr, float li)
+{
+    C3 o;
+    o.xr = a.xr * lr - a.xi * li; o.xi = a.xr * li + a.xi * lr;
+    o.yr = a.yr * lr - a.yi * li; o.yi = a.yr * li + a.yi * lr;
+    o.zr = a.zr * lr - a.zi * li; o.zi = a.zr * li + a.zi * lr;
+    return o;
+}
+
+// Projector onto the plane normal to n (normalised here; the reference builds
+// its u, v basis from normalised cross products, empropagation.py:1150-1156).
+LEM_DEV C3 project_normal(const C3 &e, float nx, float ny, float nz)
+{
+    float inv = 1.0f / sqrtf(nx * nx + ny * ny + nz * nz);
+    return project(e, nx * inv, ny * inv, nz * inv);
+}
+
+// Contribution of one source->sink leg before the sink weight / tangent step.
+LEM_DEV C3 los_term(double xs, double ys, double zs, const C3 &ws, double xk, double yk,
+                    double zk, float nkx, float nky, float nkz, float alpha, double beta)
+{
+    double dx = xk - xs, dy = yk - ys, dz = zk - zs;
+    Phase p = phase_of(dx, dy, dz, beta);
+    float ux = (float)dx * p.inv_r, uy = (float)dy * p.inv_r, uz = (float)dz * p.inv_r;
+    float cosang = ux * nkx + uy * nky + uz * nkz;
+    float lr, li;
+    leg_factor(p, alpha, (float)beta, cosang, lr, li);
+    C3 e = project(ws, ux, uy, uz);
+    return scale(e, lr, li);
+}
+
+LEM_DEV bool blocked(const unsigned long long *__restrict__ hit_bits, int64_t W64, int64_t s,
+                     int64_t k)
+{
+    if (!hit_bits)
+        return false;
+    unsigned long long w = hit_bits[s * W64 + (k >> 6)];
+    return (w >> (k & 63)) & 1ull;
+}
+
+LEM_DEV void stage_sources(SrcLds *s_src, const float *__restrict__ src,
+                           const float *__restrict__ src_w, int64_t base, int64_t s_end)
+{
+    int64_t j = base + threadIdx.x;
+    SrcLds L;
+    if (j < s_end) {
+        L.x = (double)src[3 * j];
+        L.y = (double)src[3 * j + 1];
+        L.z = (double)src[3 * j + 2];
+        C3 w = load_w(src_w, j);
+        L.w[0] = w.xr; L.w[1] = w.xi; L.w[2] = w.yr; L.w[3] = w.yi; L.w[4] = w.zr; L.w[5] = w.zi;
+    } else {
+        L.x = L.y = L.z = 0.0;
+        for (int i = 0; i < 6; ++i)
+            L.w[i] = 0.0f;
+    }
+    s_src[threadIdx.x] = L;
+}
+
+// Source-summed partials: block b covers sinks [(b % nbk)*256, +256) and
+// source chunk c = b / nbk ([c*CH, min(S, (c+1)*CH))).
+__global__ __launch_bounds__(FB) void k_field_los_partial(
+    const float *__restrict__ src, const float *__restrict__ src_w, int64_t S,
+    const float *__restrict__ snk, const float *__restrict__ snk_n, int64_t K,
+    const unsigned long long *__restrict__ hit_bits, int64_t W64, float alpha, double beta,
+    int64_t CH, int64_t nbk, float *__restrict__ ws_sum, float *__restrict__ ws_cmp)
+{
+    __shared__ SrcLds s_src[FS_TILE];
+    const int64_t c = (int64_t)blockIdx.x / nbk;
+    const int64_t k = ((int64_t)blockIdx.x % nbk) * FB + threadIdx.x;
+    const bool active = k < K;
+    const int64_t kk = active ? k : 0;
+    const double xk = snk[3 * kk], yk = snk[3 * kk + 1], zk = snk[3 * kk + 2];
+    const float nkx = snk_n[3 * kk], nky = snk_n[3 * kk + 1], nkz = snk_n[3 * kk + 2];
+    const int64_t s0 = c * CH;
+    const int64_t s1 = (s0 + CH) < S ? (s0 + CH) : S;
+    KahanC3 acc;
+    acc.zero();
+    for (int64_t base = s0; base < s1; base += FS_TILE) {
+        __syncthreads();
+        stage_sources(s_src, src, src_w, base, s1);
+        __syncthreads();
+        int n = (int)((s1 - base) < FS_TILE ? (s1 - base) : FS_TILE);
+        if (active) {
+            for (int j = 0; j < n; ++j) {
+                if (blocked(hit_bits, W64, base + j, k))
+                    continue;
+                const SrcLds &L = s_src[j];
+                C3 w;
+                w.xr = L.w[0]; w.xi = L.w[1]; w.yr = L.w[2]; w.yi = L.w[3]; w.zr = L.w[4];
+                w.zi = L.w[5];
+                C3 t = los_term(L.x, L.y, L.z, w, xk, yk, zk, nkx, nky, nkz, alpha, beta);
+                float v[6] = {t.xr, t.xi, t.yr, t.yi, t.zr, t.zi};
+                acc.add(v);
+            }
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            ws_sum[(c * 6 + i) * K + k] = acc.s[i];
+            ws_cmp[(c * 6 + i) * K + k] = acc.c[i];
+        }
+    }
+}
+
+// Fixed-order merge of the chunk partials, then sink weight and tangent step.
+// out: K x 3 complex64 (interleaved).
+__global__ __launch_bounds__(FB) void k_field_los_finish(
+    const float *__restrict__ ws_sum, const float *__restrict__ ws_cmp, int64_t NC, int64_t K,
+    const float *__restrict__ snk_n, const float *__restrict__ snk_w, int sink_tangent,
+    float *__restrict__ out)
+{
+    const int64_t k = (int64_t)blockIdx.x * FB + threadIdx.x;
+    if (k >= K)
+        return;
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t c = 0; c < NC; ++c) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+            a[i] += (double)ws_sum[(c * 6 + i) * K + k] - (double)ws_cmp[(c * 6 + i) * K + k];
+    }
+    C3 e;
+    e.xr = (float)a[0]; e.xi = (float)a[1]; e.yr = (float)a[2];
+    e.yi = (float)a[3]; e.zr = (float)a[4]; e.zi = (float)a[5];
+    if (snk_w)
+        e = cmul_comp(e, load_w(snk_w, k));
+    if (sink_tangent)
+        e = project_normal(e, snk_n[3 * k], snk_n[3 * k + 1], snk_n[3 * k + 2]);
+    float2 *o = reinterpret_cast<float2 *>(out) + 3 * k;
+    o[0] = make_float2(e.xr, e.xi);
+    o[1] = make_float2(e.yr, e.yi);
+    o[2] = make_float2(e.zr, e.zi);
+}
+
+// Per-element network (calculate_scattering(elements=True)): out is
+// S x K x 3 complex64; block b covers source s = b / nbk, 256 sinks.
+__global__ __launch_bounds__(FB) void k_field_los_elements(
+    const float *__restrict__ src, const float *__restrict__ src_w, const float *__restrict__ snk,
+    const float *__restrict__ snk_n, const float *__restrict__ snk_w, int64_t K,
+    const unsigned long long *__restrict__ hit_bits, int64_t W64, float alpha, double beta,
+    int sink_tangent, int64_t nbk, float *__restrict__ out)
+{
+    const int64_t s = (int64_t)blockIdx.x / nbk;
+    const int64_t k = ((int64_t)blockIdx.x % nbk) * FB + threadIdx.x;
+    if (k >= K)
+        return;
+    C3 e;
+    if (blocked(hit_bits, W64, s, k)) {
+        e.xr = e.xi = e.yr = e.yi = e.zr = e.zi = 0.0f;
+    } else {
+        const float nkx = snk_n[3 * k], nky = snk_n[3 * k + 1], nkz = snk_n[3 * k + 2];
+        e = los_term(src[3 * s], src[3 * s + 1], src[3 * s + 2], load_w(src_w, s), snk[3 * k],
+                     snk[3 * k + 1], snk[3 * k + 2], nkx, nky, nkz, alpha, beta);
+        if (snk_w)
+            e = cmul_comp(e, load_w(snk_w, k));
+        if (sink_tangent)
+            e = project_normal(e, nkx, nky, nkz);
+    }
+    float2 *o = reinterpret_cast<float2 *>(out) + 3 * (s * K + k);
+    o[0] = make_float2(e.xr, e.xi);
+    o[1] = make_float2(e.yr, e.yi);
+    o[2] = make_float2(e.zr, e.zi);
+}
+
+// Generic path list (EMGPUFreqDomain's input): one thread per (source, sink)
+// slot, summing that slot's paths in index order.  index rows are 1-based,
+// 0 = absent point; `order` lists path rows grouped by slot.
+__global__ __launch_bounds__(FB) void k_field_paths(
+    const float *__restrict__ pos, const float *__restrict__ nrm, const float *__restrict__ w,
+    int64_t N, const int32_t *__restrict__ index, int ncol, const int64_t *__restrict__ order,
+    const int64_t *__restrict__ seg_start, const int64_t *__restrict__ seg_slot, int64_t nseg,
+    float alpha, double beta, double *__restrict__ out, int *__restrict__ err)
+{
+    const int64_t g = (int64_t)blockIdx.x * FB + threadIdx.x;
+    if (g >= nseg)
+        return;
+    KahanC3 acc;
+    acc.zero();
+    for (int64_t q = seg_start[g]; q < seg_start[g + 1]; ++q) {
+        const int32_t *row = index + (int64_t)ncol * order[q];
+        bool ok = row[0] >= 1 && row[0] <= N && row[1] >= 1 && row[1] <= N;
+        for (int i = 2; i < ncol; ++i)
+            ok = ok && row[i] >= 0 && row[i] <= N;
+        if (!ok) {
+            *err = 1;
+            continue;
+        }
+        C3 E = load_w(w, row[0] - 1);
+        float lr = 1.0f, li = 0.0f;
+        for (int i = 0; i < ncol - 1; ++i) {
+            int64_t a = row[i] - 1;
+            int32_t b = row[i + 1];
+            if (i >= 1) {
+                if (a < 0)
+                    break;
+                E = project_normal(E, nrm[3 * a], nrm[3 * a + 1], nrm[3 * a + 2]);
+            }
+            if (b != 0) {
+                int64_t bb = b - 1;
+                double dx = (double)pos[3 * bb] - (double)pos[3 * a];
+                double dy = (double)pos[3 * bb + 1] - (double)pos[3 * a + 1];
+                double dz = (double)pos[3 * bb + 2] - (double)pos[3 * a + 2];
+                Phase p = phase_of(dx, dy, dz, beta);
+                float ux = (float)dx * p.inv_r, uy = (float)dy * p.inv_r,
+                      uz = (float)dz * p.inv_r;
+                float cosang = ux * nrm[3 * bb] + uy * nrm[3 * bb + 1] + uz * nrm[3 * bb + 2];
+                float l2r, l2i;
+                leg_factor(p, alpha, (float)beta, cosang, l2r, l2i);
+                float nr = lr * l2r - li * l2i;
+                float ni = lr * l2i + li * l2r;
+                lr = nr;
+                li = ni;
+                E = project(E, ux, uy, uz);
+                E = cmul_comp(E, load_w(w, bb));
+            }
+        }
+        C3 t = scale(E, lr, li);
+        float v[6] = {t.xr, t.xi, t.yr, t.yi, t.zr, t.zi};
+        acc.add(v);
+    }
+    double *o = out + 6 * seg_slot[g];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+        o[i] = (double)acc.s[i] - (double)acc.c[i];
+}
+
+// Zero fill as a kernel (stream-capturable, unlike a memset node).
+__global__ void k_fill_zero(float *__restrict__ p, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * FB + threadIdx.x;
+    if (i < n)
+        p[i] = 0.0f;
+}
+
+// Chunk count for the source-summed mode: enough (sink, chunk) threads to fill
+// 256 CUs several times over while keeping >= 64 sources per thread.
+inline int64_t los_chunks(int64_t S, int64_t K)
+{
+    int64_t want = ceil_div(1 << 19, K > 0 ? K : 1);
+    int64_t cap = ceil_div(S, 64);
+    int64_t nc = want < cap ? want : cap;
+    return nc < 1 ? 1 : nc;
+}
+
+}  // namespace lem
+
+using namespace lem;
+
+extern "C" int64_t lem_field_los_workspace(int64_t S, int64_t K, int flags)
+{
+    if (flags & LEM_FIELD_ELEMENTS)
+        return 0;
+    if (S <= 0 || K <= 0)
+        return 0;
+    int64_t nc = los_chunks(S, K);
+    return 2 * nc * 6 * K * (int64_t)sizeof(float);
+}
+
+extern "C" int lem_field_los(const float *src, const float *src_w, int64_t S, const float *snk,
+                             const float *snk_n, const float *snk_w, int64_t K,
+                             const uint64_t *hit_bits, double alpha, double beta, int flags,
+                             float *out, void *workspace, int64_t workspace_bytes, void *stream)
+{
+    clear_error();
+    LEM_REQUIRE(S >= 0 && K >= 0, "lem_field_los: negative size");
+    if (K == 0)
+        return OK;
+    LEM_REQUIRE(out && snk && snk_n, "lem_field_los: null pointer");
+    LEM_REQUIRE(S == 0 || (src && src_w), "lem_field_los: null source arrays");
+    const int64_t w64 = ceil_div(K, 64);
+    const int tangent = (flags & LEM_FIELD_SINK_TANGENT) ? 1 : 0;
+    hipStream_t st = as_stream(stream);
+    const unsigned long long *hb = reinterpret_cast<const unsigned long long *>(hit_bits);
+    const int64_t nbk = ceil_div(K, FB);
+    if (flags & LEM_FIELD_ELEMENTS) {
+        if (S == 0)
+            return OK;
+        LEM_REQUIRE(S * nbk <= 2147483647LL, "lem_field_los: S*K too large for one launch");
+        hipLaunchKernelGGL(k_field_los_elements, dim3((unsigned)(S * nbk)), dim3(FB), 0, st, src,
+                           src_w, snk, snk_n, snk_w, K, hb, w64, (float)alpha, beta, tangent, nbk,
+                           out);
+        return check_launch("k_field_los_elements");
+    }
+    if (S == 0) {
+        hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)ceil_div(6 * K, FB)), dim3(FB), 0, st, out,
+                           6 * K);
+        return check_launch("k_fill_zero");
+    }
+    const int64_t need = lem_field_los_workspace(S, K, flags);
+    if (!workspace || workspace_bytes < need) {
+        set_error("lem_field_los: workspace smaller than lem_field_los_workspace()");
+        return E_WORKSPACE;
+    }
+    const int64_t NC = los_chunks(S, K);
+    const int64_t CH = ceil_div(S, NC);
+    float *ws_sum = reinterpret_cast<float *>(workspace);
+    float *ws_cmp = ws_sum + NC * 6 * K;
+    LEM_REQUIRE(NC * nbk <= 2147483647LL, "lem_field_los: grid too large");
+    hipLaunchKernelGGL(k_field_los_partial, dim3((unsigned)(NC * nbk)), dim3(FB), 0, st, src, src_w,
+                       S, snk, snk_n, K, hb, w64, (float)alpha, beta, CH, nbk, ws_sum, ws_cmp);
+    int rc = check_launch("k_field_los_partial");
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_field_los_finish, dim3((unsigned)nbk), dim3(FB), 0, st, ws_sum, ws_cmp,
+                       NC, K, snk_n, snk_w, tangent, out);
+    return check_launch("k_field_los_finish");
+}
+
+extern "C" int lem_field_paths(const float *pos, const float *nrm, const float *w, int64_t N,
+                               const int32_t *index, int ncol, const int64_t *order,
+                               const int64_t *seg_start, const int64_t *seg_slot, int64_t nseg,
+                               double alpha, double beta, double *out, int *err_flag,
+                               void *stream)
+{
+    clear_error();
+    LEM_REQUIRE(N >= 0 && nseg >= 0, "lem_field_paths: negative size");
+    LEM_REQUIRE(ncol >= 2 && ncol <= 4, "lem_field_paths: index must have 2 to 4 columns");
+    if (nseg == 0)
+        return OK;
+    LEM_REQUIRE(pos && nrm && w && index && order && seg_start && seg_slot && out && err_flag,
+                "lem_field_paths: null pointer");
+    LEM_REQUIRE(ceil_div(nseg, FB) <= 2147483647LL, "lem_field_paths: too many segments");
+    hipLaunchKernelGGL(k_field_paths, dim3((unsigned)ceil_div(nseg, FB)), dim3(FB), 0,
+                       as_stream(stream), pos, nrm, w, N, index, ncol, order, seg_start, seg_slot,
+                       nseg, (float)alpha, beta, out, err_flag);
+    return check_launch("k_field_paths");
+}

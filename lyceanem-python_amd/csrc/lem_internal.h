@@ -1,0 +1,36 @@
+// lem_internal.h -- host-side helpers shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace lem {
+
+// Thread-local last-error message (lem_last_error()).
+void set_error(const std::string &msg);
+void clear_error();
+
+// Status codes (mirrored in include/lem.h).
+constexpr int OK = 0;
+constexpr int E_INVALID = -1;
+constexpr int E_HIP = -2;
+constexpr int E_WORKSPACE = -3;
+
+// Checks the launch that was just issued; records the HIP error text.
+int check_launch(const char *what);
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace lem
+
+#define LEM_REQUIRE(cond, msg)            \
+    do {                                  \
+        if (!(cond)) {                    \
+            ::lem::set_error(msg);        \
+            return ::lem::E_INVALID;      \
+        }                                 \
+    } while (0)

@@ -1,0 +1,215 @@
+"""ctypes binding of liblem.so (the gfx950 HIP library, declared in include/lem.h).
+
+PyTorch-ROCm provides device memory and the current stream; every compute call
+goes to a hand-written HIP kernel.  There is no CPU fallback: without the built
+library or without a HIP device the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "liblem.so")
+ABI_VERSION = 1
+
+SEM_NUMBA = 0
+SEM_NATIVE = 1
+VIS_ANY_HIT = 1
+VIS_SKIP_SELF = 2
+FIELD_SINK_TANGENT = 1
+FIELD_ELEMENTS = 2
+
+# Entry points and their ctypes signatures (mirrors include/lem.h).
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_INT = ctypes.c_int
+_D = ctypes.c_double
+SIGNATURES = {
+    "lem_abi_version": ([], _INT),
+    "lem_last_error": ([], ctypes.c_char_p),
+    "lem_device_arch": ([ctypes.c_char_p, _INT], _INT),
+    "lem_visibility_dense": ([_P, _I64, _P, _I64, _P, _I64, _INT, _INT, _P, _P, _P], _INT),
+    "lem_raycast_pairs": ([_P, _I64, _P, _I64, _P, _I64, _INT, _INT, _P, _P, _P], _INT),
+    "lem_field_los_workspace": ([_I64, _I64, _INT], _I64),
+    "lem_field_los": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _D, _D, _INT, _P, _P, _I64, _P],
+                      _INT),
+    "lem_field_paths": ([_P, _P, _P, _I64, _P, _INT, _P, _P, _P, _I64, _D, _D, _P, _P, _P],
+                        _INT),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class LemError(RuntimeError):
+    """A liblem.so entry point returned an error code."""
+
+
+def load_library(path: str = LIB_PATH):
+    """Load liblem.so and bind every entry point (no GPU needed for this step)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(path):
+                raise LemError(
+                    f"{path} is missing: build it with "
+                    "`python lyceanem-python_amd/build_lib.py` (there is no CPU fallback)")
+            lib = ctypes.CDLL(path)
+            for name, (args, res) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.argtypes = args
+                fn.restype = res
+            if lib.lem_abi_version() != ABI_VERSION:
+                raise LemError(f"liblem.so ABI {lib.lem_abi_version()} != {ABI_VERSION}")
+            _lib = lib
+    return _lib
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise LemError("no HIP device visible: the LyceanEM MI355X path has no CPU fallback")
+    return torch
+
+
+def device():
+    return _torch().device("cuda", _torch().cuda.current_device())
+
+
+def stream_ptr():
+    return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = load_library().lem_last_error().decode(errors="replace")
+        raise LemError(f"{what} failed ({rc}): {msg}")
+
+
+def ptr(t):
+    """Device pointer of a contiguous tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise LemError("liblem.so needs contiguous tensors")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def to_dev(a, dtype=None):
+    """numpy array or tensor -> contiguous tensor on the current HIP device."""
+    torch = _torch()
+    if isinstance(a, torch.Tensor):
+        t = a
+        if dtype is not None and t.dtype != dtype:
+            t = t.to(dtype)
+        if t.device.type != "cuda":
+            t = t.to(device())
+        return t.contiguous()
+    arr = np.ascontiguousarray(a)
+    t = torch.from_numpy(arr)
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(device()).contiguous()
+
+
+# --------------------------------------------------------------------------
+# typed wrappers (tensors in, tensors out; all on the current stream)
+# --------------------------------------------------------------------------
+
+def visibility_dense(src, snk, tris, semantics=SEM_NUMBA, any_hit=True, skip_self=False,
+                     want_dist=False):
+    """Dense S x K hit bits (uint64 words, 1 = blocked) and optional dist."""
+    torch = _torch()
+    lib = load_library()
+    src = to_dev(src, torch.float32).reshape(-1, 3)
+    snk = to_dev(snk, torch.float32).reshape(-1, 3)
+    tris = to_dev(tris, torch.float32).reshape(-1, 9)
+    S, K, T = src.shape[0], snk.shape[0], tris.shape[0]
+    W = (K + 63) // 64
+    bits = torch.zeros((S, W), dtype=torch.int64, device=src.device)
+    dist = torch.empty((S, K), dtype=torch.float32, device=src.device) if want_dist else None
+    flags = (VIS_ANY_HIT if any_hit else 0) | (VIS_SKIP_SELF if skip_self else 0)
+    check(lib.lem_visibility_dense(ptr(src), S, ptr(snk), K, ptr(tris) if T else None, T,
+                                   semantics, flags, ptr(bits), ptr(dist), stream_ptr()),
+          "lem_visibility_dense")
+    return bits, dist
+
+
+def raycast_pairs(points, pairs, tris, semantics=SEM_NUMBA, any_hit=False, want_dist=True):
+    """kernel1Dv3 over explicit 0-based (origin, target) rows of `points`."""
+    torch = _torch()
+    lib = load_library()
+    pts = to_dev(points, torch.float32).reshape(-1, 3)
+    pr = to_dev(pairs, torch.int32).reshape(-1, 2)
+    tris = to_dev(tris, torch.float32).reshape(-1, 9)
+    R, T = pr.shape[0], tris.shape[0]
+    inter = torch.empty(R, dtype=torch.uint8, device=pts.device)
+    dist = torch.empty(R, dtype=torch.float32, device=pts.device) if want_dist else None
+    flags = VIS_ANY_HIT if any_hit else 0
+    check(lib.lem_raycast_pairs(ptr(pts), pts.shape[0], ptr(pr), R, ptr(tris) if T else None, T,
+                                semantics, flags, ptr(inter), ptr(dist), stream_ptr()),
+          "lem_raycast_pairs")
+    return inter, dist
+
+
+def field_los(src, src_w, snk, snk_n, hit_bits=None, snk_w=None, alpha=0.0, beta=2 * np.pi,
+              sink_tangent=False, elements=False):
+    """Fused line-of-sight field: (K,3) complex64 source sum or (S,K,3) network."""
+    torch = _torch()
+    lib = load_library()
+    src = to_dev(src, torch.float32).reshape(-1, 3)
+    snk = to_dev(snk, torch.float32).reshape(-1, 3)
+    snk_n = to_dev(snk_n, torch.float32).reshape(-1, 3)
+    S, K = src.shape[0], snk.shape[0]
+    src_w = to_dev(src_w, torch.complex64).reshape(S, 3)
+    if snk_w is not None:
+        snk_w = to_dev(snk_w, torch.complex64).reshape(K, 3)
+    flags = (FIELD_SINK_TANGENT if sink_tangent else 0) | (FIELD_ELEMENTS if elements else 0)
+    out = torch.empty((S, K, 3) if elements else (K, 3), dtype=torch.complex64,
+                      device=src.device)
+    wsb = int(lib.lem_field_los_workspace(S, K, flags))
+    ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=src.device)
+    if hit_bits is not None:
+        hit_bits = to_dev(hit_bits, torch.int64)
+        if hit_bits.shape != (S, (K + 63) // 64):
+            raise LemError(f"hit_bits shape {tuple(hit_bits.shape)} != {(S, (K + 63) // 64)}")
+    check(lib.lem_field_los(ptr(src), ptr(src_w), S, ptr(snk), ptr(snk_n), ptr(snk_w), K,
+                            ptr(hit_bits), float(alpha), float(beta), flags, ptr(out), ptr(ws),
+                            wsb, stream_ptr()), "lem_field_los")
+    return out
+
+
+def field_paths(pos, nrm, w, index, order, seg_start, seg_slot, S, K, alpha, beta):
+    """Network of an explicit path list -> (S,K,3) complex128 tensor."""
+    torch = _torch()
+    lib = load_library()
+    pos = to_dev(pos, torch.float32).reshape(-1, 3)
+    nrm = to_dev(nrm, torch.float32).reshape(-1, 3)
+    N = pos.shape[0]
+    w = to_dev(w, torch.complex64).reshape(N, 3)
+    idx = to_dev(index, torch.int32)
+    order = to_dev(order, torch.int64)
+    seg_start = to_dev(seg_start, torch.int64)
+    seg_slot = to_dev(seg_slot, torch.int64)
+    nseg = seg_slot.shape[0]
+    out = torch.zeros((S, K, 3), dtype=torch.complex128, device=pos.device)
+    err = torch.zeros(1, dtype=torch.int32, device=pos.device)
+    check(lib.lem_field_paths(ptr(pos), ptr(nrm), ptr(w), N, ptr(idx), idx.shape[1], ptr(order),
+                              ptr(seg_start), ptr(seg_slot), nseg, float(alpha), float(beta),
+                              ptr(out), ptr(err), stream_ptr()), "lem_field_paths")
+    if int(err.item()) != 0:
+        raise LemError("lem_field_paths: path index points outside the point table")
+    return out
+
+
+def unpack_bits(bits, K):
+    """(S, ceil(K/64)) uint64 words -> (S, K) bool (host numpy)."""
+    b = bits.detach().cpu().numpy().view(np.uint64)
+    S = b.shape[0]
+    by = b.view(np.uint8).reshape(S, -1)
+    return np.unpackbits(by, axis=1, bitorder="little")[:, :K].astype(bool)

@@ -1,0 +1,118 @@
+"""Frequency-domain propagation of the LyceanEM path on the gfx950 kernels.
+
+Same names and return values as the reference functions in
+lyceanem/electromagnetics/empropagation.py; the per-path field runs in
+`lem_field_paths` (include/lem.h) instead of the numba kernel freqdomainkernal.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .. import _native
+
+
+def targettingindex(full_index):
+    """(source, sink) columns of every path and its bounce depth
+    (empropagation.py:3394-3429): line-of-sight rows first, then one-bounce rows,
+    each group in index order."""
+    fi = np.asarray(full_index)
+    scatter_index = np.ones(fi.shape[0], dtype=np.int64)
+    if fi.shape[1] == 2:
+        return fi, scatter_index
+    if fi.shape[1] == 3:
+        los = np.all(fi[:, 2:] == 0, axis=1)
+        sc = np.all(fi[:, 2:] != 0, axis=1)
+        depth = np.append(fi[los][:, [0, 1]], fi[sc][:, [0, 2]], axis=0)
+        scatter_index[sc] = 2
+        return depth, scatter_index
+    if fi.shape[1] == 4:
+        z2 = np.all(fi[:, 2:] == 0, axis=1)
+        depth = fi[z2][:, [0, 1]]
+        rem = fi[~z2]
+        z3 = np.all(rem[:, 3:] == 0, axis=1)
+        depth = np.append(depth, rem[z3][:, [0, 2]], axis=0)
+        rem2 = rem[~z3]
+        depth = np.append(depth, rem2[np.all(rem2[:, 3:] != 0, axis=1)][:, [0, 3]], axis=0)
+        scatter_index[np.all(fi[:, 2:] != 0, axis=1)] = 3
+        scatter_index[(fi[:, 2] != 0) & (fi[:, 3] == 0)] = 2
+        return depth, scatter_index
+    raise ValueError("full_index must have 2 to 4 columns")
+
+
+def path_slots(full_index, source_num, sink_num):
+    """0-based (source, sink) slot of every path row: the source column and the
+    last non-zero column (the pairing targettingindex intends)."""
+    fi = np.asarray(full_index, dtype=np.int64)
+    last = fi[:, 1].copy()
+    for c in range(2, fi.shape[1]):
+        nz = fi[:, c] != 0
+        last[nz] = fi[nz, c]
+    s = fi[:, 0] - 1
+    k = last - 1 - source_num
+    return s, k
+
+
+def EMGPUFreqDomain(source_num, sink_num, full_index, point_information, wavelength, alpha,
+                    beta):
+    """Scattering network (source_num, sink_num, 3) complex128 of every path in
+    full_index (empropagation.py:2197-2410).
+
+    point_information is the scattering_t table (base_types.py:89-119): points,
+    normals and complex weights of sources, sinks and scattering points.  The
+    paths of each (source, sink) slot are summed on the GPU in index order
+    (Kahan-compensated float32), so the result does not depend on launch order;
+    the reference adds them with float64 atomics in arbitrary order."""
+    pi = np.asarray(point_information)
+    if pi.shape[0] and not np.all(pi["Electric"]):
+        raise NotImplementedError("EMGPUFreqDomain: magnetic (Electric=False) sources")
+    fi = np.asarray(full_index)
+    S, K = int(source_num), int(sink_num)
+    if fi.shape[0] == 0:
+        return np.zeros((S, K, 3), dtype=np.complex128)
+    pos = np.stack([pi["px"], pi["py"], pi["pz"]], axis=1).astype(np.float32)
+    nrm = np.stack([pi["nx"], pi["ny"], pi["nz"]], axis=1).astype(np.float32)
+    w = np.stack([pi["ex"], pi["ey"], pi["ez"]], axis=1).astype(np.complex64)
+    s, k = path_slots(fi, S, K)
+    if s.min() < 0 or s.max() >= S or k.min() < 0 or k.max() >= K:
+        raise IndexError("full_index rows do not start on a source and end on a sink")
+    slot = s * K + k
+    order = np.argsort(slot, kind="stable")
+    ss = slot[order]
+    starts = np.flatnonzero(np.r_[True, ss[1:] != ss[:-1]])
+    seg_start = np.append(starts, len(ss)).astype(np.int64)
+    seg_slot = ss[starts].astype(np.int64)
+    net = _native.field_paths(pos, nrm, w, np.ascontiguousarray(fi, np.int32), order, seg_start,
+                              seg_slot, S, K, float(alpha), float(beta))
+    return net.cpu().numpy()
+
+
+def vector_mapping(local_E_vector, point_normal, rotation_matrix):
+    """Local (u, v, n) excitation -> global vector (empropagation.py:3692-3762):
+    n is the rotated point normal, u = z x n / |z x n| (x x n when n is along z),
+    v = n x u / |u x n|, and the result is local_E_vector @ [u; v; n]."""
+    dt = np.asarray(local_E_vector).dtype
+    n = np.matmul(np.asarray(point_normal).astype(dt), np.asarray(rotation_matrix))
+    axes = np.eye(3)
+    uvn = np.zeros((3, 3), dtype=dt)
+    uvn[2, :] = n
+    z_orth = np.linalg.norm(np.cross(axes[:, 2], n))
+    if abs(z_orth) == 0:
+        c = np.cross(axes[0, :], n)
+    else:
+        c = np.cross(axes[2, :], n)
+    uvn[0, :] = c / np.linalg.norm(c)
+    uvn[1, :] = np.cross(n, uvn[0, :]) / np.linalg.norm(np.cross(uvn[0, :], n))
+    return np.matmul(local_E_vector, uvn)
+
+
+def calculate_conformalVectors(desired_E_vector, source_normals, antenna_axes=np.eye(3)):
+    """Per-point conformal excitation vectors (empropagation.py:3807-3849)."""
+    normals = np.asarray(source_normals)
+    out = np.zeros((normals.shape[0], 3), dtype=np.complex64)
+    dv = np.asarray(desired_E_vector)
+    ax = np.asarray(antenna_axes).astype(np.complex64)
+    per_point = dv.ndim == 2 and dv.shape[0] == normals.shape[0]
+    for i in range(normals.shape[0]):
+        e = dv[i, :] if per_point else dv.ravel()
+        out[i, :] = vector_mapping(e, normals[i, :], ax)
+    return out

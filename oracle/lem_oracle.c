@@ -1,0 +1,525 @@
+/*
+ * lem_oracle.c -- CPU oracle for LyceanEM's raycast + frequency-domain hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP library is compared
+ * against (and the CPU baseline that bench.py times beside the GPU).  Nothing in
+ * the product library links, loads or calls it.
+ *
+ * It restates, in plain C, the reference arithmetic cited next to each function
+ * (paths relative to the LyceanEM-Python source tree).  Build: oracle/Makefile
+ * (gcc -O2 -ffp-contract=off -fopenmp), so every FP32 expression is evaluated
+ * as written, in source order, with no fused multiply-add.
+ *
+ * Pinning (see tests/test_oracle_*.py): closed-form hits/misses, exact rational
+ * geometry on tiny scenes, both predicate semantics against each other away from
+ * their documented differences, the projector identity, single-pair closed forms,
+ * linearity in the weights and the unblocked planar-array factor.
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_API __attribute__((visibility("default")))
+
+/* rayfunctions.py:19  EPSILON = 1e-7 (a Python float, i.e. double) */
+static const double NUMBA_EPS = 1e-7;
+/* math_utility.cuh:9  constexpr float EPSILON = 0.000001 */
+static const float NATIVE_EPS = 0.000001f;
+
+/* ------------------------------------------------------------------------ */
+/* Ray charging                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* Ray from origin o to target t: length L = |t-o| and unit direction (t-o)/L.
+ * math_functions.py:148-156 (calc_dv_norm, numba njit on float32 arrays: the
+ * difference, squares, sum and sqrt stay float32 -- numba resolves a float32
+ * array ** int literal to the float32 loop -- then a float32 division), used by
+ * charge_rays_environment1Dv2 (rayfunctions.py:1790-1842).  The native twin
+ * points_to_rays (raycasting.cuh:80-95) evaluates the same expression. */
+ORC_API void orc_charge(const float *o, const float *t, int64_t n, float *d_out,
+                        float *len_out)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        float dx = t[3 * i + 0] - o[3 * i + 0];
+        float dy = t[3 * i + 1] - o[3 * i + 1];
+        float dz = t[3 * i + 2] - o[3 * i + 2];
+        float s = dx * dx + dy * dy;
+        s = s + dz * dz;
+        float L = sqrtf(s);
+        len_out[i] = L;
+        d_out[3 * i + 0] = dx / L;
+        d_out[3 * i + 1] = dy / L;
+        d_out[3 * i + 2] = dz / L;
+    }
+}
+
+static inline void charge_one(const float *o, const float *t, float *d, float *L)
+{
+    float dx = t[0] - o[0];
+    float dy = t[1] - o[1];
+    float dz = t[2] - o[2];
+    float s = dx * dx + dy * dy;
+    s = s + dz * dz;
+    float l = sqrtf(s);
+    *L = l;
+    d[0] = dx / l;
+    d[1] = dy / l;
+    d[2] = dz / l;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Moller-Trumbore predicates                                                */
+/* ------------------------------------------------------------------------ */
+
+/* numba `hit` (rayfunctions.py:37-104), two-sided.
+ * Edges, cross and dot products are float32 (rayfunctions.py:22-33 dot/cross,
+ * evaluated left to right).  F = 1.0/A promotes to float64 (a Python float
+ * literal), so U, V, U+V and t are float64.  The accept window compares the
+ * float64 t against the ray's float32 `dist` promoted to float64.
+ * tri = (v0x,v0y,v0z, v1x,v1y,v1z, v2x,v2y,v2z) like base_types.triangle_t.
+ * Returns 1 on an accepted hit and writes the float64 distance. */
+ORC_API int orc_hit_numba(const float *o, const float *d, float dist, const float *tri,
+                          double *t_out)
+{
+    float e1x = tri[3] - tri[0], e1y = tri[4] - tri[1], e1z = tri[5] - tri[2];
+    float e2x = tri[6] - tri[0], e2y = tri[7] - tri[1], e2z = tri[8] - tri[2];
+    /* pvec = cross(d, e2) */
+    float px = d[1] * e2z - d[2] * e2y;
+    float py = d[2] * e2x - d[0] * e2z;
+    float pz = d[0] * e2y - d[1] * e2x;
+    /* A = dot(pvec, e1) */
+    float A = px * e1x + py * e1y;
+    A = A + pz * e1z;
+    if (-NUMBA_EPS < (double)A && (double)A < NUMBA_EPS)
+        return 0;
+    float tx = o[0] - tri[0], ty = o[1] - tri[1], tz = o[2] - tri[2];
+    double F = 1.0 / (double)A;
+    float du = tx * px + ty * py;
+    du = du + tz * pz;
+    double U = F * (double)du;
+    if (U < 0.0 || U > 1.0)
+        return 0;
+    /* q = cross(tvec, e1) */
+    float qx = ty * e1z - tz * e1y;
+    float qy = tz * e1x - tx * e1z;
+    float qz = tx * e1y - ty * e1x;
+    float dv = d[0] * qx + d[1] * qy;
+    dv = dv + d[2] * qz;
+    double V = F * (double)dv;
+    if (V < 0.0 || (U + V) > 1.0)
+        return 0;
+    float dt = e2x * qx + e2y * qy;
+    dt = dt + e2z * qz;
+    double t = F * (double)dt;
+    if (t > 2.0 * NUMBA_EPS && t < (double)dist - 2.0 * NUMBA_EPS) {
+        *t_out = t;
+        return 1;
+    }
+    return 0;
+}
+
+/* Native CUDA `hit` (raycasting.cuh:19-68): back-face culled (A<0 -> miss,
+ * :31-34), EPSILON = 1e-6f, everything float32; F = 1.0/A is a double division
+ * narrowed to float (:39).  helper_math dot/cross evaluate left to right. */
+ORC_API int orc_hit_native(const float *o, const float *d, float len, const float *tri)
+{
+    float e1x = tri[3] - tri[0], e1y = tri[4] - tri[1], e1z = tri[5] - tri[2];
+    float e2x = tri[6] - tri[0], e2y = tri[7] - tri[1], e2z = tri[8] - tri[2];
+    float px = d[1] * e2z - d[2] * e2y;
+    float py = d[2] * e2x - d[0] * e2z;
+    float pz = d[0] * e2y - d[1] * e2x;
+    /* dot(edge1, pvec) */
+    float A = e1x * px + e1y * py;
+    A = A + e1z * pz;
+    if (fabsf(A) < NATIVE_EPS)
+        return 0;
+    if (A < 0)
+        return 0;
+    float tx = o[0] - tri[0], ty = o[1] - tri[1], tz = o[2] - tri[2];
+    float F = (float)(1.0 / (double)A);
+    float du = tx * px + ty * py;
+    du = du + tz * pz;
+    float U = F * du;
+    if (U < 0.0 || U > 1.0)
+        return 0;
+    float qx = ty * e1z - tz * e1y;
+    float qy = tz * e1x - tx * e1z;
+    float qz = tx * e1y - ty * e1x;
+    float dv = d[0] * qx + d[1] * qy;
+    dv = dv + d[2] * qz;
+    float V = F * dv;
+    float UV = U + V;
+    if (V < 0.0 || UV > 1.0)
+        return 0;
+    float dt = e2x * qx + e2y * qy;
+    dt = dt + e2z * qz;
+    float t = F * dt;
+    float hi = len - 2 * NATIVE_EPS;
+    if (t > 2 * NATIVE_EPS && t < hi)
+        return 1;
+    return 0;
+}
+
+/* kernel1Dv3 (rayfunctions.py:212-244) for one ray: every triangle in order;
+ * an accepted hit with t < dist sets dist = (float32)t and intersect = True,
+ * so later triangles see the shrunken window. */
+static inline void raycast_one_numba(const float *o, const float *d, float dist0,
+                                     const float *tris, int64_t T, uint8_t *hit,
+                                     float *dist_out, int any_hit_only)
+{
+    float dist = dist0;
+    uint8_t inter = 0;
+    for (int64_t j = 0; j < T; ++j) {
+        double t;
+        if (orc_hit_numba(o, d, dist, tris + 9 * j, &t)) {
+            if (t < (double)dist) {
+                dist = (float)t;
+                inter = 1;
+                if (any_hit_only)
+                    break;
+            }
+        }
+    }
+    *hit = inter;
+    if (dist_out)
+        *dist_out = dist;
+}
+
+/* launchRaycaster1Dv3 (rayfunctions.py:1951-2017) on an already charged ray
+ * payload: rays is R x 7 float (ox,oy,oz,dx,dy,dz,dist) like base_types.ray_t. */
+ORC_API void orc_raycast_rays(const float *rays, int64_t R, const float *tris, int64_t T,
+                              uint8_t *intersect, float *dist_out, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0)
+        omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < R; ++i) {
+        const float *r = rays + 7 * i;
+        raycast_one_numba(r, r + 3, r[6], tris, T, intersect + i, dist_out + i, 0);
+    }
+}
+
+/* Dense S x K visibility.  semantics 0: numba charge + kernel1Dv3 over every
+ * triangle (dist is the kernel's final `dist`); semantics 1: native culled
+ * predicate, any hit (raycasting_brute_force.cuh:19-67 with a working copy-back).
+ * skip_self (native flag == false): pairs with source index == sink index are
+ * never tested (raycasting_accelerated.cuh:72). */
+ORC_API void orc_visibility_dense(const float *src, int64_t S, const float *snk, int64_t K,
+                                  const float *tris, int64_t T, int semantics,
+                                  int any_hit_only, int skip_self, uint8_t *hit,
+                                  float *dist, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0)
+        omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t s = 0; s < S; ++s) {
+        for (int64_t k = 0; k < K; ++k) {
+            float d[3], L;
+            charge_one(src + 3 * s, snk + 3 * k, d, &L);
+            int64_t idx = s * K + k;
+            if (skip_self && s == k) {
+                hit[idx] = 0;
+                if (dist)
+                    dist[idx] = L;
+                continue;
+            }
+            if (semantics == 0) {
+                raycast_one_numba(src + 3 * s, d, L, tris, T, hit + idx,
+                                  dist ? dist + idx : NULL, any_hit_only);
+            } else {
+                uint8_t h = 0;
+                for (int64_t j = 0; j < T; ++j) {
+                    if (orc_hit_native(src + 3 * s, d, L, tris + 9 * j)) {
+                        h = 1;
+                        break;
+                    }
+                }
+                hit[idx] = h;
+                if (dist)
+                    dist[idx] = L;
+            }
+        }
+    }
+}
+
+/* Sampled visibility: rays given by explicit (source row, sink row) pairs of
+ * the dense problem, numba semantics.  Used for subsampled checks at sizes the
+ * full oracle cannot finish. */
+ORC_API void orc_visibility_sampled(const float *src, const float *snk, const int64_t *pairs,
+                                    int64_t n, const float *tris, int64_t T,
+                                    int any_hit_only, uint8_t *hit, float *dist,
+                                    int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0)
+        omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i) {
+        const float *o = src + 3 * pairs[2 * i];
+        const float *t = snk + 3 * pairs[2 * i + 1];
+        float d[3], L;
+        charge_one(o, t, d, &L);
+        raycast_one_numba(o, d, L, tris, T, hit + i, dist ? dist + i : NULL, any_hit_only);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Fields (float64)                                                          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    double re, im;
+} cplx;
+
+static inline cplx cmul(cplx a, cplx b)
+{
+    cplx r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    return r;
+}
+
+/* sourcelaunchtransformGPU (empropagation.py:178-249), native twin ray_launch
+ * (em.cuh:18-61): build u = normalize(axis x d) with the axis whose cross norm
+ * is largest (ties as in the reference's if/elif chain), v = normalize(d x u),
+ * and return (E.u)u + (E.v)v.  Float64 throughout. */
+static void launch_transform(cplx E[3], const double dir[3])
+{
+    double axes[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double nrm[3];
+    double cr[3][3];
+    for (int a = 0; a < 3; ++a) {
+        const double *x = axes[a];
+        cr[a][0] = x[1] * dir[2] - x[2] * dir[1];
+        cr[a][1] = x[2] * dir[0] - x[0] * dir[2];
+        cr[a][2] = x[0] * dir[1] - x[1] * dir[0];
+        nrm[a] = sqrt(cr[a][0] * cr[a][0] + cr[a][1] * cr[a][1] + cr[a][2] * cr[a][2]);
+    }
+    int pick = -1;
+    if (fabs(nrm[0]) > fabs(nrm[1]) && fabs(nrm[0]) > fabs(nrm[2]))
+        pick = 0;
+    else if (fabs(nrm[1]) >= fabs(nrm[0]) && fabs(nrm[1]) > fabs(nrm[2]))
+        pick = 1;
+    else if (fabs(nrm[2]) >= fabs(nrm[0]) && fabs(nrm[2]) >= fabs(nrm[1]))
+        pick = 2;
+    double u[3] = {0, 0, 0}, v[3];
+    if (pick >= 0) {
+        for (int i = 0; i < 3; ++i)
+            u[i] = cr[pick][i] / nrm[pick];
+    }
+    /* v = normalize(cross(dir, u)) */
+    v[0] = dir[1] * u[2] - dir[2] * u[1];
+    v[1] = dir[2] * u[0] - dir[0] * u[2];
+    v[2] = dir[0] * u[1] - dir[1] * u[0];
+    double vn = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (int i = 0; i < 3; ++i)
+        v[i] /= vn;
+    cplx eu = {0, 0}, ev = {0, 0};
+    for (int i = 0; i < 3; ++i) {
+        eu.re += E[i].re * u[i];
+        eu.im += E[i].im * u[i];
+        ev.re += E[i].re * v[i];
+        ev.im += E[i].im * v[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        E[i].re = eu.re * u[i] + ev.re * v[i];
+        E[i].im = eu.im * u[i] + ev.im * v[i];
+    }
+}
+
+/* lossy_propagation (empropagation.py:768-801), native twin em_wave
+ * (em.cuh:64-120): r = |p2-p1| (optionally rounded to float32 as the numba
+ * kernel's float32 local does, :771-772), d = (p2-p1)/r,
+ * G = exp(-(alpha+j beta) r)/r, dG = (-(alpha+j beta) - 1/r) G,
+ * loss = -(1/2pi) dG (d . n2) with the end point's raw normal. */
+static cplx lossy(const float *p1, const float *p2, const float *n2, double alpha,
+                  double beta, int r_f32, double dir[3])
+{
+    double dx = (double)p2[0] - (double)p1[0];
+    double dy = (double)p2[1] - (double)p1[1];
+    double dz = (double)p2[2] - (double)p1[2];
+    double r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (r_f32)
+        r = (double)(float)r;
+    dir[0] = dx / r;
+    dir[1] = dy / r;
+    dir[2] = dz / r;
+    double cosang = dir[0] * (double)n2[0] + dir[1] * (double)n2[1] + dir[2] * (double)n2[2];
+    double front = -(1.0 / (2.0 * M_PI));
+    double inv = 1.0 / r;
+    double amp = exp(-alpha * r) * inv;
+    cplx G = {amp * cos(-beta * r), amp * sin(-beta * r)};
+    cplx a = {-alpha - inv, -beta};
+    cplx dG = cmul(a, G);
+    cplx loss = {front * dG.re * cosang, front * dG.im * cosang};
+    return loss;
+}
+
+static inline cplx ldc(const float *w, int64_t i, int c)
+{
+    cplx z = {(double)w[6 * i + 2 * c], (double)w[6 * i + 2 * c + 1]};
+    return z;
+}
+
+/* One line-of-sight path source s -> sink k of freqdomainkernal
+ * (empropagation.py:1083-1265): E = w_s launched along d_sk, times the sink
+ * weight (:1168-1179), optionally projected onto the sink's tangent plane as the
+ * kernel does for LOS rows of a 3-column index (:1150-1161 with i == 1), times
+ * the leg loss. */
+static void los_pair(const float *ps, const float *ws, int64_t s, const float *pk,
+                     const float *nk, const float *wk, int64_t k, double alpha, double beta,
+                     int flags, cplx out[3])
+{
+    double dir[3];
+    cplx loss = lossy(ps + 3 * s, pk + 3 * k, nk + 3 * k, alpha, beta, flags & 4, dir);
+    cplx E[3];
+    for (int c = 0; c < 3; ++c)
+        E[c] = ldc(ws, s, c);
+    launch_transform(E, dir);
+    if (wk) {
+        for (int c = 0; c < 3; ++c)
+            E[c] = cmul(E[c], ldc(wk, k, c));
+    }
+    if (flags & 1) {
+        double n[3] = {nk[3 * k], nk[3 * k + 1], nk[3 * k + 2]};
+        launch_transform(E, n);
+    }
+    for (int c = 0; c < 3; ++c)
+        out[c] = cmul(E[c], loss);
+}
+
+/* Dense line-of-sight network in float64.
+ * src_pos S x 3 float, src_w S x 3 complex64 (interleaved re,im), sink_pos /
+ * sink_nrm K x 3 float, sink_w K x 3 complex64 or NULL (weight 1 as set by
+ * frequency_domain.py:398-400), hit S x K (1 = blocked) or NULL.
+ * flags: 1 = project onto the sink tangent plane (3-column index quirk),
+ *        2 = elements (write the S x K x 3 network instead of the source sum),
+ *        4 = round path lengths to float32 like the numba kernel.
+ * out: K x 3 complex128 (sum over sources, Kahan-compensated long double) or
+ * S x K x 3 complex128 when flags & 2. */
+ORC_API void orc_field_los(const float *src_pos, const float *src_w, int64_t S,
+                           const float *snk_pos, const float *snk_nrm, const float *snk_w,
+                           int64_t K, const uint8_t *hit, double alpha, double beta, int flags,
+                           double *out, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0)
+        omp_set_num_threads(nthreads);
+#endif
+    if (flags & 2) {
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int64_t s = 0; s < S; ++s) {
+            for (int64_t k = 0; k < K; ++k) {
+                double *o = out + 6 * (s * K + k);
+                if (hit && hit[s * K + k]) {
+                    for (int i = 0; i < 6; ++i)
+                        o[i] = 0.0;
+                    continue;
+                }
+                cplx c3[3];
+                los_pair(src_pos, src_w, s, snk_pos, snk_nrm, snk_w, k, alpha, beta, flags, c3);
+                for (int c = 0; c < 3; ++c) {
+                    o[2 * c] = c3[c].re;
+                    o[2 * c + 1] = c3[c].im;
+                }
+            }
+        }
+        return;
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t k = 0; k < K; ++k) {
+        long double acc[6] = {0, 0, 0, 0, 0, 0};
+        long double cmp[6] = {0, 0, 0, 0, 0, 0};
+        for (int64_t s = 0; s < S; ++s) {
+            if (hit && hit[s * K + k])
+                continue;
+            cplx c3[3];
+            los_pair(src_pos, src_w, s, snk_pos, snk_nrm, snk_w, k, alpha, beta, flags, c3);
+            double v[6] = {c3[0].re, c3[0].im, c3[1].re, c3[1].im, c3[2].re, c3[2].im};
+            for (int i = 0; i < 6; ++i) {
+                long double y = (long double)v[i] - cmp[i];
+                long double t = acc[i] + y;
+                cmp[i] = (t - acc[i]) - y;
+                acc[i] = t;
+            }
+        }
+        for (int i = 0; i < 6; ++i)
+            out[6 * k + i] = (double)acc[i];
+    }
+}
+
+/* freqdomainkernal (empropagation.py:1083-1265) over an explicit path list, the
+ * network EMGPUFreqDomain returns (:2197-2410).  Float64, paths summed in index
+ * order (the reference adds with float64 atomics in an unspecified order).
+ * pos / nrm: N x 3 float; w: N x 3 complex64 (interleaved) -- the scattering_t
+ * table (base_types.py:89-119) with Electric = True everywhere.
+ * index: V x ncol int64, 1-based, 0 = no point (a padded LOS row).
+ * The (source, sink) slot of each row comes from targettingindex
+ * (empropagation.py:3394-3429): LOS rows -> (col0, col1), others -> (col0, last).
+ * out: S x K x 3 complex128, zeroed here. */
+ORC_API void orc_field_paths(const float *pos, const float *nrm, const float *w, int64_t N_points,
+                             const int64_t *index, int64_t V, int ncol, int64_t S, int64_t K,
+                             double alpha, double beta, int r_f32, double *out)
+{
+    (void)N_points;
+    memset(out, 0, sizeof(double) * 6 * (size_t)(S * K));
+    for (int64_t p = 0; p < V; ++p) {
+        const int64_t *row = index + (int64_t)ncol * p;
+        cplx E[3];
+        int64_t last = row[1];
+        for (int i = 0; i < ncol - 1; ++i) {
+            int64_t a = row[i] - 1;
+            int64_t b = row[i + 1];
+            if (i == 0) {
+                for (int c = 0; c < 3; ++c)
+                    E[c] = ldc(w, a, c);
+            } else {
+                double n[3] = {nrm[3 * a], nrm[3 * a + 1], nrm[3 * a + 2]};
+                launch_transform(E, n);
+            }
+            if (b != 0) {
+                double dir[3];
+                const float *pa = pos + 3 * a, *pb = pos + 3 * (b - 1);
+                double dx = (double)pb[0] - pa[0], dy = (double)pb[1] - pa[1],
+                       dz = (double)pb[2] - pa[2];
+                double r = sqrt(dx * dx + dy * dy + dz * dz);
+                dir[0] = dx / r;
+                dir[1] = dy / r;
+                dir[2] = dz / r;
+                launch_transform(E, dir);
+                for (int c = 0; c < 3; ++c)
+                    E[c] = cmul(E[c], ldc(w, b - 1, c));
+                last = b;
+            }
+        }
+        double dir[3];
+        cplx loss = lossy(pos + 3 * (row[0] - 1), pos + 3 * (row[1] - 1), nrm + 3 * (row[1] - 1),
+                          alpha, beta, r_f32, dir);
+        for (int i = 1; i < ncol - 1; ++i) {
+            if (row[i + 1] != 0) {
+                cplx l2 = lossy(pos + 3 * (row[i] - 1), pos + 3 * (row[i + 1] - 1),
+                                nrm + 3 * (row[i + 1] - 1), alpha, beta, r_f32, dir);
+                loss = cmul(loss, l2);
+            }
+        }
+        int64_t s = row[0] - 1;
+        int64_t k = last - 1 - S;
+        if (s < 0 || s >= S || k < 0 || k >= K)
+            continue;
+        double *o = out + 6 * (s * K + k);
+        for (int c = 0; c < 3; ++c) {
+            cplx v = cmul(E[c], loss);
+            o[2 * c] += v.re;
+            o[2 * c + 1] += v.im;
+        }
+    }
+}

@@ -1,0 +1,132 @@
+"""GPU tests of the drop-in Python API against the oracle's pipeline
+(workchunkingv2 -> freqdomainkernal restated on the CPU)."""
+import numpy as np
+import pytest
+
+from lyceanem.mesh import Mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(n=6, n_tris=60, seed=3):
+    rng = np.random.default_rng(seed)
+    c = (np.arange(n) - (n - 1) / 2) * 0.05
+    xx, yy = np.meshgrid(c, c)
+    src = np.stack([xx.ravel(), yy.ravel(), np.zeros(n * n)], 1)
+    ctr = rng.uniform(-1, 1, (n_tris, 1, 3)) * [1, 1, 0.3] + [0, 0, 0.8]
+    tv = (ctr + rng.uniform(-0.2, 0.2, (n_tris, 3, 3))).reshape(-1, 3)
+    solid = Mesh(points=tv, cells=[("triangle", np.arange(3 * n_tris).reshape(-1, 3))])
+    aperture = Mesh(points=src, cells=[],
+                    point_data={"Normals": np.tile([0.0, 0, 1], (n * n, 1)),
+                                "Area": np.full(n * n, 0.0025)})
+    return aperture, solid
+
+
+def _sinks(n_el=9, n_az=13, r=50.0):
+    from lyceanem.geometry.targets import spherical_field
+
+    return spherical_field(np.linspace(-180, 180, n_az), np.linspace(-90, 90, n_el),
+                           outward_normals=True, field_radius=r)
+
+
+def test_workchunking_matches_oracle(oracle, monkeypatch):
+    from lyceanem.raycasting import rayfunctions as RF
+    from lyceanem.geometry import geometryfunctions as GF
+    from lyceanem.base_types import triangles_as_array
+
+    aperture, solid = _scene()
+    snk = _sinks().points.astype(np.float32)
+    src = aperture.points.astype(np.float32)
+    tris = GF.mesh_conversion(solid)
+    rng = np.random.default_rng(1)
+    scat = rng.uniform(-1, 1, (11, 3)).astype(np.float32) * [1, 1, 0.2] + [0, 0, 1.4]
+    for ms in (1, 2):
+        got, rays = RF.workchunkingv2(src, snk, scat.astype(np.float32), tris, ms)
+        exp, erays = oracle.workchunking(src, snk, scat, triangles_as_array(tris), ms)
+        assert np.array_equal(got, exp) and rays == erays
+        assert got.shape[0] > 0
+    # chunked == unchunked when free memory is tiny
+    monkeypatch.setattr(RF, "free_memory_bytes", lambda: 40000)
+    got1, _ = RF.workchunkingv2(src, snk, np.empty((0, 3), np.float32), tris, 1)
+    exp1, _ = oracle.workchunking(src, snk, np.empty((0, 3), np.float32),
+                                  triangles_as_array(tris), 1)
+    assert np.array_equal(got1, exp1)
+    got2, _ = RF.workchunkingv2(src, snk, scat.astype(np.float32), tris, 2)
+    exp2, _ = oracle.workchunking(src, snk, scat, triangles_as_array(tris), 2)
+    assert sorted(map(tuple, got2)) == sorted(map(tuple, exp2))
+
+
+@pytest.mark.parametrize("elements", [False, True])
+def test_calculate_scattering_los(oracle, elements):
+    from lyceanem.models import frequency_domain as FD
+    from lyceanem.geometry import geometryfunctions as GF
+    from lyceanem.base_types import triangles_as_array
+
+    aperture, solid = _scene()
+    sinks = _sinks()
+    lam = 0.1
+    desired = np.array([[1.0, 0.5j, 0.0]])
+    Ex, Ey, Ez = FD.calculate_scattering(aperture, sinks, solid, desired, wavelength=lam,
+                                         elements=elements, beta=2 * np.pi / lam)
+    src = aperture.points.astype(np.float32)
+    snk = sinks.points.astype(np.float32)
+    tris = triangles_as_array(GF.mesh_conversion(solid))
+    hit, _ = oracle.visibility_dense(src, snk, tris, any_hit_only=True, want_dist=False)
+    w = (np.repeat(desired.astype(np.complex64), len(src), 0) * 0.0025).astype(np.complex64)
+    ref = oracle.field_los(src, w, snk, sinks.point_data["Normals"], 0.0, 2 * np.pi / lam,
+                           hit=hit, elements=elements)
+    got = np.stack([Ex, Ey, Ez], -1)
+    assert got.dtype == np.complex128 and got.shape == ref.shape
+    assert 0 < hit.mean() < 1
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-4
+
+
+def test_calculate_scattering_one_bounce(oracle):
+    from lyceanem.models import frequency_domain as FD
+    from lyceanem.geometry import geometryfunctions as GF
+    from lyceanem.base_types import triangles_as_array
+
+    aperture, solid = _scene(n=4, n_tris=30)
+    sinks = _sinks(5, 9)
+    rng = np.random.default_rng(9)
+    sp = rng.uniform(-1, 1, (13, 3)) * [1, 1, 0.1] + [0, 0, 1.5]
+    scat = Mesh(points=sp, cells=[], point_data={"Normals": np.tile([0.0, 0, -1], (13, 1))})
+    lam = 0.2
+    Ex, Ey, Ez = FD.calculate_scattering(aperture, sinks, solid, np.array([[0.0, 1.0, 0.0]]),
+                                         scatter_points=scat, wavelength=lam, scattering=1,
+                                         beta=2 * np.pi / lam)
+    src = aperture.points.astype(np.float32)
+    snk = sinks.points.astype(np.float32)
+    tris = triangles_as_array(GF.mesh_conversion(solid))
+    full, _ = oracle.workchunking(src, snk, sp.astype(np.float32), tris, 2)
+    S, K = len(src), len(snk)
+    pos = np.concatenate([src, snk, sp]).astype(np.float32)
+    nrm = np.concatenate([aperture.point_data["Normals"], sinks.point_data["Normals"],
+                          scat.point_data["Normals"]]).astype(np.float32)
+    w = np.ones((len(pos), 3), np.complex64)
+    w[:S] = np.array([0, 1, 0], np.complex64)
+    net = oracle.field_paths(pos, nrm, w, full, S, K, 0.0, 2 * np.pi / lam)
+    ref = net.sum(0)
+    got = np.stack([Ex, Ey, Ez], -1)
+    assert (full[:, 2] != 0).any()
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-4
+
+
+def test_calculate_farfield_unit_sphere_quirk(oracle):
+    from lyceanem.models import frequency_domain as FD
+
+    aperture, solid = _scene(n=4, n_tris=5)
+    az = np.linspace(-180, 180, 13)
+    el = np.linspace(-90, 90, 7)
+    et, ep = FD.calculate_farfield(aperture, solid, np.array([[1.0, 0, 0]]), az_range=az,
+                                   el_range=el, wavelength=0.1, farfield_distance=20,
+                                   beta=2 * np.pi / 0.1)
+    assert et.shape == (7 * 13,) and ep.shape == (7 * 13,)
+    from lyceanem.geometry.targets import spherical_field
+
+    sinks = spherical_field(az, el, True, 1.0)  # radius 1, outward, whatever farfield_distance
+    Ex, Ey, Ez = FD.calculate_scattering(aperture, sinks, solid, np.array([[1.0, 0, 0]]),
+                                         wavelength=0.1, beta=2 * np.pi / 0.1)
+    th, ph = sinks.point_data["theta_(Radians)"], sinks.point_data["phi_(Radians)"]
+    et2, ep2 = oracle.etheta_ephi(Ex, Ey, Ez, th, ph)
+    assert np.allclose(et, et2) and np.allclose(ep, ep2)

@@ -1,0 +1,209 @@
+"""GPU parity: the HIP kernels through the C-ABI against the CPU oracle.
+
+Bar: hit masks and kernel1Dv3 distances bit-exact; fields within 1e-4 relative
+of the float64 oracle (norm-wise over sinks, and per sink against the sum of
+term magnitudes), with the phase error stated.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIELD_RTOL = 1e-4
+
+
+def _bits_to_bool(native, bits, K):
+    return native.unpack_bits(bits, K)
+
+
+def _c1_small(n=16, n_tris=2000, n_el=31, n_az=61, seed=1):
+    from lyceanem import scenes
+
+    return scenes.planar_array_c1(n=n, n_tris=n_tris, n_el=n_el, n_az=n_az, seed=seed)
+
+
+@pytest.mark.parametrize("any_hit", [True, False])
+def test_visibility_dense_exact(native, oracle, any_hit):
+    sc = _c1_small()
+    S, K = len(sc["src"]), len(sc["snk"])
+    print(f"visibility_dense S={S} K={K} T={len(sc['tris'])} any_hit={any_hit}", flush=True)
+    bits, dist = native.visibility_dense(sc["src"], sc["snk"], sc["tris"], any_hit=any_hit,
+                                         want_dist=not any_hit)
+    got = _bits_to_bool(native, bits, K)
+    hit, odist = oracle.visibility_dense(sc["src"], sc["snk"], sc["tris"])
+    assert 0.05 < hit.mean() < 0.95
+    assert np.array_equal(got, hit), f"{(got != hit).sum()} mask mismatches"
+    if not any_hit:
+        d = dist.cpu().numpy()
+        assert np.array_equal(d, odist), f"{(d != odist).sum()} dist mismatches"
+
+
+def test_visibility_full_c1_sampled(native, oracle):
+    from lyceanem import scenes
+
+    sc = scenes.planar_array_c1()
+    S, K = len(sc["src"]), len(sc["snk"])
+    print(f"visibility_dense full C1 S={S} K={K}", flush=True)
+    bits, _ = native.visibility_dense(sc["src"], sc["snk"], sc["tris"], any_hit=True)
+    got = _bits_to_bool(native, bits, K)
+    rng = np.random.default_rng(0)
+    pairs = np.stack([rng.integers(0, S, 40000), rng.integers(0, K, 40000)], 1)
+    hit, _ = oracle.visibility_sampled(sc["src"], sc["snk"], pairs, sc["tris"], any_hit_only=True)
+    assert np.array_equal(got[pairs[:, 0], pairs[:, 1]], hit)
+
+
+def test_visibility_edge_shapes(native, oracle):
+    rng = np.random.default_rng(3)
+    tris = (rng.uniform(0, 2, (1, 3)) + rng.uniform(-0.5, 0.5, (37, 9))).astype(np.float32)
+    for S, K in [(1, 1), (3, 65), (2, 130), (5, 257)]:
+        src = rng.uniform(-1, 0, (S, 3)).astype(np.float32)
+        snk = rng.uniform(2, 3, (K, 3)).astype(np.float32)
+        bits, dist = native.visibility_dense(src, snk, tris, any_hit=False, want_dist=True)
+        hit, odist = oracle.visibility_dense(src, snk, tris)
+        assert np.array_equal(_bits_to_bool(native, bits, K), hit)
+        assert np.array_equal(dist.cpu().numpy(), odist)
+        # padding bits of the last word stay clear
+        words = bits.cpu().numpy().view(np.uint64)
+        if K % 64:
+            assert (words[:, -1] >> np.uint64(K % 64) == 0).all()
+    # no triangles: nothing is blocked
+    bits, _ = native.visibility_dense(src, snk, np.empty((0, 9), np.float32))
+    assert not _bits_to_bool(native, bits, K).any()
+
+
+def test_native_semantics_and_skip_self(native, oracle):
+    sc = _c1_small(n=8, n_tris=500, n_el=9, n_az=15)
+    K = len(sc["snk"])
+    bits, _ = native.visibility_dense(sc["src"], sc["snk"], sc["tris"], semantics=native.SEM_NATIVE)
+    hit, _ = oracle.visibility_dense(sc["src"], sc["snk"], sc["tris"], semantics=1,
+                                     any_hit_only=True, want_dist=False)
+    assert np.array_equal(_bits_to_bool(native, bits, K), hit)
+    bits, _ = native.visibility_dense(sc["src"], sc["snk"], sc["tris"], skip_self=True)
+    hit, _ = oracle.visibility_dense(sc["src"], sc["snk"], sc["tris"], any_hit_only=True,
+                                     skip_self=True, want_dist=False)
+    assert np.array_equal(_bits_to_bool(native, bits, K), hit)
+
+
+def test_raycast_pairs_exact(native, oracle):
+    rng = np.random.default_rng(7)
+    pts = rng.uniform(-2, 2, (300, 3)).astype(np.float32)
+    ctr = rng.uniform(-1.5, 1.5, (700, 1, 3))
+    tris = (ctr + rng.uniform(-0.3, 0.3, (700, 3, 3))).reshape(-1, 9).astype(np.float32)
+    pairs = rng.integers(0, 300, (5000, 2)).astype(np.int32)
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    inter, dist = native.raycast_pairs(pts, pairs, tris, any_hit=False, want_dist=True)
+    d, L = oracle.charge(pts[pairs[:, 0]], pts[pairs[:, 1]])
+    rays = np.concatenate([pts[pairs[:, 0]], d, L[:, None]], 1)
+    oi, od = oracle.raycast_rays(rays, tris)
+    assert np.array_equal(inter.cpu().numpy().astype(bool), oi)
+    assert np.array_equal(dist.cpu().numpy(), od)
+    inter2, _ = native.raycast_pairs(pts, pairs, tris, any_hit=True, want_dist=False)
+    assert np.array_equal(inter2.cpu().numpy().astype(bool), oi)
+    assert 0.05 < oi.mean() < 0.95
+
+
+def _field_errors(got, ref, terms_scale):
+    """(norm-wise relative error, max per-sink error / sum of |terms|)."""
+    nrm = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    per = (np.abs(got - ref).max(axis=-1) / terms_scale).max()
+    return nrm, per
+
+
+def test_field_los_sum_vs_fp64_oracle(native, oracle):
+    sc = _c1_small(n=16, n_tris=400, n_el=31, n_az=61)
+    S, K = len(sc["src"]), len(sc["snk"])
+    hit, _ = oracle.visibility_dense(sc["src"], sc["snk"], sc["tris"], any_hit_only=True,
+                                     want_dist=False)
+    bits, _ = native.visibility_dense(sc["src"], sc["snk"], sc["tris"])
+    assert np.array_equal(_bits_to_bool(native, bits, K), hit)
+    w = (sc["w"] * np.exp(1j * np.linspace(0, 6, S))[:, None]).astype(np.complex64)
+    E = native.field_los(sc["src"], w, sc["snk"], sc["snk_n"], hit_bits=bits, alpha=0.0,
+                         beta=sc["beta"]).cpu().numpy()
+    ref = oracle.field_los(sc["src"], w, sc["snk"], sc["snk_n"], 0.0, sc["beta"], hit=hit)
+    El = oracle.field_los(sc["src"], w, sc["snk"], sc["snk_n"], 0.0, sc["beta"], hit=hit,
+                          elements=True)
+    scale = np.abs(El).sum(axis=0).max(axis=-1) + 1e-30
+    nrm, per = _field_errors(E, ref, scale)
+    print(f"field_los sum: normwise {nrm:.2e}, per-sink {per:.2e}", flush=True)
+    assert nrm < FIELD_RTOL and per < FIELD_RTOL
+
+
+def test_field_los_elements_and_phase(native, oracle):
+    sc = _c1_small(n=8, n_tris=300, n_el=21, n_az=41)
+    S, K = len(sc["src"]), len(sc["snk"])
+    bits, _ = native.visibility_dense(sc["src"], sc["snk"], sc["tris"])
+    hit = _bits_to_bool(native, bits, K)
+    rng = np.random.default_rng(2)
+    w = (rng.normal(size=(S, 3)) + 1j * rng.normal(size=(S, 3))).astype(np.complex64)
+    for tangent in (False, True):
+        E = native.field_los(sc["src"], w, sc["snk"], sc["snk_n"], hit_bits=bits,
+                             beta=sc["beta"], elements=True, sink_tangent=tangent).cpu().numpy()
+        ref = oracle.field_los(sc["src"], w, sc["snk"], sc["snk_n"], 0.0, sc["beta"], hit=hit,
+                               elements=True, sink_tangent=tangent)
+        big = np.abs(ref) > 1e-3 * np.abs(ref).max()
+        rel = np.abs(E[big] - ref[big]) / np.abs(ref[big])
+        dph = np.abs(np.angle(E[big] / ref[big]))
+        print(f"elements tangent={tangent}: max rel {rel.max():.2e}, max phase {dph.max():.2e} rad",
+              flush=True)
+        assert rel.max() < FIELD_RTOL and dph.max() < 1e-4
+        assert (E[hit] == 0).all()
+
+
+def test_field_long_paths_phase(native, oracle):
+    """1-10 km paths at 5.8 GHz: the float64 path length keeps the phase error
+    far below 1e-3 rad (the float32 path length would not)."""
+    rng = np.random.default_rng(4)
+    lam = 299792458.0 / 5.8e9
+    beta = 2 * np.pi / lam
+    src = rng.uniform(-5, 5, (64, 3)).astype(np.float32)
+    u = rng.normal(size=(300, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    snk = (u * rng.uniform(1e3, 1e4, (300, 1))).astype(np.float32)
+    w = np.zeros((64, 3), np.complex64)
+    w[:, 2] = 1.0
+    E = native.field_los(src, w, snk, u.astype(np.float32), beta=beta,
+                         elements=True).cpu().numpy()
+    ref = oracle.field_los(src, w, snk, u.astype(np.float32), 0.0, beta, elements=True)
+    big = np.abs(ref) > 1e-3 * np.abs(ref).max()
+    dph = np.abs(np.angle(E[big] / ref[big]))
+    ref32 = oracle.field_los(src, w, snk, u.astype(np.float32), 0.0, beta, elements=True,
+                             r_f32=True)
+    dph32 = np.abs(np.angle(ref32[big] / ref[big]))
+    print(f"long paths: max phase err {dph.max():.2e} rad (float32-r reference: "
+          f"{dph32.max():.2e})", flush=True)
+    assert dph.max() < 1e-4
+    assert np.abs(E[big] - ref[big]).max() / np.abs(ref[big]).max() < FIELD_RTOL
+
+
+@pytest.mark.parametrize("ncol", [2, 3])
+def test_field_paths_vs_oracle(native, oracle, ncol):
+    rng = np.random.default_rng(10 + ncol)
+    S, K, P = 6, 9, 7
+    pos = np.concatenate([rng.uniform(-1, 1, (S, 3)), rng.uniform(20, 40, (K, 3)),
+                          rng.uniform(3, 8, (P, 3))]).astype(np.float32)
+    nrm = rng.normal(size=(S + K + P, 3)).astype(np.float32)
+    w = (rng.normal(size=(S + K + P, 3)) + 1j * rng.normal(size=(S + K + P, 3))).astype(
+        np.complex64)
+    w[S:S + K] = 1.0
+    los = np.array([[s + 1, S + 1 + k] for s in range(S) for k in range(K)
+                    if rng.random() < 0.7])
+    if ncol == 2:
+        idx = los
+    else:
+        sc = np.array([[s + 1, S + K + 1 + p, S + 1 + k] for s in range(S) for p in range(P)
+                       for k in range(K) if rng.random() < 0.5])
+        idx = np.concatenate([np.concatenate([los, np.zeros((len(los), 1), int)], 1), sc])
+    from lyceanem.electromagnetics import empropagation as EM
+    from lyceanem.base_types import scattering_t
+
+    table = np.zeros(S + K + P, scattering_t)
+    for i, c in enumerate("xyz"):
+        table["p" + c] = pos[:, i]
+        table["n" + c] = nrm[:, i]
+        table["e" + c] = w[:, i]
+    table["Electric"] = True
+    net = EM.EMGPUFreqDomain(S, K, idx.astype(np.int32), table, 1.0, 0.0, 2 * np.pi / 0.5)
+    ref = oracle.field_paths(pos, nrm, w, idx, S, K, 0.0, 2 * np.pi / 0.5)
+    scale = np.abs(ref).max()
+    assert np.abs(net - ref).max() < FIELD_RTOL * scale
+    assert net.dtype == np.complex128 and net.shape == (S, K, 3)

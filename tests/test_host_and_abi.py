@@ -1,0 +1,179 @@
+"""CPU tests: reference golden fixtures, host index logic of the drop-in package,
+and the C-ABI library's exports (no compute calls: there is no GPU here)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+# --------------------------------------------------------------------------
+# reference fixtures
+# --------------------------------------------------------------------------
+
+def test_convert_triangles_golden():
+    """tests/test_ray_functions.py:9-24 of the reference: convertTriangles of
+    receive_horn.ply equals triangle_type_array_ref.npy."""
+    from lyceanem import mesh
+    from lyceanem.raycasting import rayfunctions as RF
+    from lyceanem.base_types import triangle_t
+
+    ref = np.load(os.path.join(GOLDEN, "triangle_type_array_ref.npy"), allow_pickle=False)
+    m = mesh.read_ply(os.path.join(GOLDEN, "receive_horn.ply"))
+    tri = RF.convertTriangles(m)
+    assert tri.dtype == triangle_t and ref.dtype.names == triangle_t.names
+    assert tri.shape == ref.shape == (12,)
+    for name in triangle_t.names:
+        assert np.array_equal(tri[name], ref[name]), name
+
+
+@pytest.mark.parametrize("angle,e_in,e_out", [
+    (0, [1, 0, 0], [0, 1, 0]), (0, [0, 1, 0], [0, 0, 1]), (0, [0, 0, 1], [1, 0, 0]),
+    (-90, [1, 0, 0], [-1, 0, 0]), (-90, [0, 1, 0], [0, 0, 1]), (-90, [0, 0, 1], [0, 1, 0]),
+])
+def test_vector_mapping_known_answers(angle, e_in, e_out):
+    """Known answers of tests/test_empropagation.py:8-95 (normal (1,0,0), rotation
+    about z by 0 / -90 degrees, atol 1e-12)."""
+    from scipy.spatial.transform import Rotation as R
+    from lyceanem.electromagnetics.empropagation import vector_mapping
+
+    rot = R.from_euler("z", angle, degrees=True).as_matrix()
+    out = vector_mapping(np.array(e_in, np.complex64), np.array([1.0, 0, 0]), rot)
+    np.testing.assert_allclose(out, np.array(e_out, np.complex64), atol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# host index semantics: product vs oracle restatement
+# --------------------------------------------------------------------------
+
+def test_create_model_index_matches_oracle(oracle):
+    from lyceanem.raycasting import rayfunctions as RF
+
+    src = np.arange(1, 5).reshape(4, 1)
+    snk = np.arange(5, 8).reshape(3, 1)
+    sct = np.arange(8, 10).reshape(2, 1)
+    for a, b, c in [(src, snk, sct), (src, snk, np.empty((0, 1))),
+                    (np.array([[1, 8], [2, 9], [3, 8]]), snk, np.empty((0, 0)))]:
+        got = RF.create_model_index(a, b, c)
+        exp = oracle.create_model_index(a, b, c)
+        assert got.dtype == np.int32 and np.array_equal(got, exp)
+    # self rows are dropped (scatterer -> itself)
+    got = RF.create_model_index(np.array([[1, 8], [1, 9]]), snk, sct)
+    assert not (got[:, -2] == got[:, -1]).any()
+    assert got.shape[0] == 2 * 5 - 2
+
+
+def test_ray_hits_split(oracle):
+    from lyceanem.raycasting import rayfunctions as RF
+
+    idx = np.array([[1, 5], [1, 8], [2, 6], [2, 9], [3, 7]], np.int32)
+    inter = np.array([False, False, True, False, False])
+    sinks = np.arange(5, 8).reshape(3, 1)
+    f, g = RF.rayHits1Dv2(inter, idx, sinks)
+    f2, g2 = oracle.ray_hits(inter, idx, sinks)
+    assert np.array_equal(f, f2) and np.array_equal(g, g2)
+
+
+def test_targetting_index_and_slots(oracle):
+    from lyceanem.electromagnetics import empropagation as EM
+
+    fi = np.array([[1, 4, 0], [2, 5, 0], [1, 7, 4], [2, 8, 5], [3, 6, 0]], np.int32)
+    d, sc = EM.targettingindex(fi)
+    d2, sc2 = oracle.targetting_index(fi)
+    assert np.array_equal(d, d2) and np.array_equal(sc, sc2)
+    s, k = EM.path_slots(fi, 3, 3)
+    assert np.array_equal(s, [0, 1, 0, 1, 2])
+    assert np.array_equal(k, [0, 1, 0, 1, 2])
+
+
+def test_spherical_field_order_and_angles(oracle):
+    from lyceanem.geometry.targets import spherical_field
+
+    az = np.linspace(-180, 180, 7)
+    el = np.linspace(-90, 90, 5)
+    m = spherical_field(az, el, outward_normals=True, field_radius=2.0)
+    p = m.points.reshape(5, 7, 3)
+    # el-major, az-fastest; z depends on el only
+    assert np.allclose(p[:, :, 2], 2 * np.sin(np.radians(el))[:, None])
+    assert np.allclose(np.linalg.norm(m.points, axis=1), 2.0)
+    assert np.allclose(m.point_data["Normals"], m.points / 2.0)
+    th, ph, r = oracle.theta_phi(m.points)
+    assert np.allclose(m.point_data["theta_(Radians)"], th)
+    assert np.allclose(m.point_data["phi_(Radians)"], ph)
+    inward = spherical_field(az, el, outward_normals=False)
+    assert np.allclose(inward.point_data["Normals"], -inward.points)
+
+
+def test_etheta_ephi_of_pure_polarisations(oracle):
+    """Spherical-unit-vector identities of the calculate_farfield conversion."""
+    th = np.radians(np.array([30.0, 60.0, 120.0]))
+    ph = np.radians(np.array([10.0, 200.0, -45.0]))
+    that = np.stack([np.cos(th) * np.cos(ph), np.cos(th) * np.sin(ph), -np.sin(th)], 1)
+    phat = np.stack([-np.sin(ph), np.cos(ph), np.zeros(3)], 1)
+    E = 2.0 * that + 3.0j * phat
+    et, ep = oracle.etheta_ephi(E[:, 0], E[:, 1], E[:, 2], th, ph)
+    assert np.allclose(et, 2.0) and np.allclose(ep, 3.0j)
+
+
+def test_scene_generators_shapes():
+    from lyceanem import scenes
+
+    c1 = scenes.planar_array_c1(n=4, n_tris=10, n_el=5, n_az=7)
+    assert c1["src"].shape == (16, 3) and c1["tris"].shape == (10, 9)
+    assert c1["snk"].shape == (35, 3)
+    c2 = scenes.terrain_c2(nqx=20, nqy=10, size_x=20.0, size_y=10.0, n_src_side=8, n_snk=16)
+    assert c2["tris"].shape == (400, 9)
+    # sinks sit snk_alt above the triangulated surface
+    h = scenes.terrain_surface_height(c2["heights"], c2["snk"][:, 0], c2["snk"][:, 1])
+    assert np.allclose(c2["snk"][:, 2] - h, 1.5, atol=1e-4)
+
+
+# --------------------------------------------------------------------------
+# C-ABI: the library loads and exports every declared entry point
+# --------------------------------------------------------------------------
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "lem.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lem_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from lyceanem import _native
+
+    lib = _native.load_library()
+    declared = _declared_symbols()
+    assert "lem_visibility_dense" in declared and "lem_field_los" in declared
+    for name in declared:
+        assert hasattr(lib, name), name
+    # the ctypes binding covers exactly the declared entry points
+    assert sorted(_native.SIGNATURES) == declared
+    assert lib.lem_abi_version() == _native.ABI_VERSION
+
+
+def test_library_rejects_bad_arguments_without_gpu():
+    """Argument validation happens before any HIP call."""
+    from lyceanem import _native
+
+    lib = _native.load_library()
+    rc = lib.lem_visibility_dense(None, 4, None, 4, None, 0, 7, 0, None, None, None)
+    assert rc == -1 and b"semantics" in lib.lem_last_error()
+    rc = lib.lem_field_paths(None, None, None, 1, None, 5, None, None, None, 1, 0.0, 1.0, None,
+                             None, None)
+    assert rc == -1 and b"columns" in lib.lem_last_error()
+    assert lib.lem_field_los_workspace(100, 1000, 0) > 0
+    assert lib.lem_field_los_workspace(100, 1000, _native.FIELD_ELEMENTS) == 0
+
+
+def test_product_fails_loudly_without_gpu():
+    import torch
+    from lyceanem import _native
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(_native.LemError):
+        _native.visibility_dense(np.zeros((1, 3)), np.ones((1, 3)), np.zeros((0, 9)))

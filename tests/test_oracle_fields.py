@@ -1,0 +1,176 @@
+"""Pins the oracle's float64 field restatement (freqdomainkernal et al.).
+
+Independent checks: the textbook Huygens far-field kernel j/(lambda r) e^{-jkr}
+cos(theta) that the reference's Rayleigh-Sommerfeld factor tends to for
+r >> lambda, the closed-form array factor of a steered uniform planar array,
+the projector identity of the launch basis, linearity, masking, and the
+two-leg factorisation the GPU one-bounce path relies on.
+"""
+import numpy as np
+import pytest
+
+LAM = 0.1
+BETA = 2 * np.pi / LAM
+
+
+def test_single_pair_huygens_far_field(oracle):
+    rng = np.random.default_rng(1)
+    src = np.zeros((1, 3), np.float32)
+    snk = (rng.normal(size=(20, 3)) * 1000).astype(np.float32)
+    r = np.linalg.norm(snk.astype(np.float64), axis=1)
+    u = snk / r[:, None]
+    nrm = u.astype(np.float32)  # sink normal along the path: cos(theta) = 1
+    w = np.array([[0.3 + 0.1j, -0.2j, 0.7]], np.complex64)
+    E = oracle.field_los(src, w, snk, nrm, 0.0, BETA)
+    w64 = w[0].astype(np.complex128)
+    perp = w64[None, :] - u * (u @ w64)[:, None]
+    huy = (1j / LAM) * np.exp(-1j * BETA * r) / r
+    expect = huy[:, None] * perp
+    # the -(1/r) term of dG is 1/(beta r) ~ 1.6e-5 relative at 1 km
+    assert np.allclose(E, expect, rtol=5e-5, atol=1e-12 * np.abs(expect).max())
+    # the obliquity factor is the raw d . n (negative when the normal faces away)
+    E2 = oracle.field_los(src, w, snk, -nrm, 0.0, BETA)
+    assert np.allclose(E2, -E, rtol=1e-12)
+
+
+def test_attenuation_alpha(oracle):
+    src = np.zeros((1, 3), np.float32)
+    snk = np.array([[0, 0, 50.0], [0, 0, 100.0]], np.float32)
+    nrm = np.array([[0, 0, 1], [0, 0, 1]], np.float32)
+    w = np.array([[1, 0, 0]], np.complex64)
+    E0 = oracle.field_los(src, w, snk, nrm, 0.0, BETA)
+    Ea = oracle.field_los(src, w, snk, nrm, 0.01, BETA)
+    ratio = np.abs(Ea[:, 0]) / np.abs(E0[:, 0])
+    # |(-(a+jb) - 1/r)| / |(-jb - 1/r)| * e^{-a r}
+    corr = np.abs(0.01 + 1j * BETA + 1 / np.array([50, 100])) / np.abs(
+        1j * BETA + 1 / np.array([50, 100]))
+    assert np.allclose(ratio, np.exp(-0.01 * np.array([50.0, 100.0])) * corr, rtol=1e-10)
+
+
+def test_steered_planar_array_factor(oracle):
+    """Sum over an N x N lambda/2 array steered to u0 equals the closed-form
+    array factor times the element term in the far field."""
+    n = 8
+    pitch = LAM / 2
+    c = (np.arange(n) - (n - 1) / 2) * pitch
+    xx, yy = np.meshgrid(c, c)
+    src = np.stack([xx.ravel(), yy.ravel(), np.zeros(n * n)], 1).astype(np.float32)
+    th0, ph0 = np.radians(20.0), np.radians(35.0)
+    u0 = np.array([np.sin(th0) * np.cos(ph0), np.sin(th0) * np.sin(ph0), np.cos(th0)])
+    # e^{-j beta r} propagation: r_s ~ R - p_s.u, so weights e^{-j beta p_s.u0} steer to u0
+    phase = np.exp(-1j * BETA * (src.astype(np.float64) @ u0))
+    w = np.zeros((n * n, 3), np.complex64)
+    w[:, 0] = phase
+    R = 1.0e6
+    th = np.radians(np.array([0.0, 10.0, 20.0, 20.0, 40.0]))
+    ph = np.radians(np.array([0.0, 35.0, 35.0, 100.0, 200.0]))
+    uu = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
+    snk = (uu * R).astype(np.float32)
+    E = oracle.field_los(src, w, snk, uu.astype(np.float32), 0.0, BETA)
+    psx = BETA * pitch * (uu[:, 0] - u0[0])
+    psy = BETA * pitch * (uu[:, 1] - u0[1])
+
+    def af1(p):
+        with np.errstate(invalid="ignore", divide="ignore"):
+            v = np.sin(n * p / 2) / np.sin(p / 2)
+        return np.where(np.abs(np.sin(p / 2)) < 1e-12, n, v)
+
+    AF = af1(psx) * af1(psy)
+    r = np.linalg.norm(snk.astype(np.float64), axis=1)
+    xhat = np.array([1.0, 0, 0])
+    perp = xhat[None, :] - uu * uu[:, [0]]
+    elem = (1j / LAM) * np.exp(-1j * BETA * r) / r
+    expect = (AF * elem)[:, None] * perp
+    scale = np.abs(expect).max()
+    assert np.allclose(E, expect, rtol=1e-3, atol=2e-4 * scale)
+    # main beam at u0 is the largest response of the grid
+    assert np.argmax(np.abs(E[:, 0])) in (2,)
+
+
+def test_projector_identity_and_tangent_quirk(oracle):
+    rng = np.random.default_rng(2)
+    src = rng.uniform(-1, 1, (5, 3)).astype(np.float32)
+    snk = rng.uniform(4, 9, (6, 3)).astype(np.float32)
+    nrm = rng.normal(size=(6, 3)).astype(np.float32)
+    w = (rng.normal(size=(5, 3)) + 1j * rng.normal(size=(5, 3))).astype(np.complex64)
+    E = oracle.field_los(src, w, snk, nrm, 0.0, BETA, elements=True)
+    d = snk[None].astype(np.float64) - src[:, None]
+    d /= np.linalg.norm(d, axis=2, keepdims=True)
+    assert np.abs(np.einsum("skc,skc->sk", E, d)).max() < 1e-12 * np.abs(E).max()
+    # launch basis (E.u)u + (E.v)v equals (I - dd^T)E: E is parallel to the projection
+    proj = w[:, None, :] - d * np.einsum("sc,skc->sk", w.astype(np.complex128), d)[..., None]
+    ratio = E / proj
+    assert np.allclose(ratio, ratio[..., [0]], rtol=1e-9)
+    Et = oracle.field_los(src, w, snk, nrm, 0.0, BETA, elements=True, sink_tangent=True)
+    nh = nrm / np.linalg.norm(nrm.astype(np.float64), axis=1, keepdims=True)
+    assert np.abs(np.einsum("skc,kc->sk", Et, nh)).max() < 1e-12 * np.abs(Et).max()
+    assert np.allclose(Et, E - nh[None] * np.einsum("skc,kc->sk", E, nh)[..., None], atol=1e-14)
+
+
+def test_linearity_and_mask(oracle):
+    rng = np.random.default_rng(4)
+    src = rng.uniform(-1, 1, (9, 3)).astype(np.float32)
+    snk = rng.uniform(3, 9, (7, 3)).astype(np.float32)
+    nrm = rng.normal(size=(7, 3)).astype(np.float32)
+    w1 = (rng.normal(size=(9, 3)) + 1j * rng.normal(size=(9, 3))).astype(np.complex64)
+    w2 = (rng.normal(size=(9, 3)) + 1j * rng.normal(size=(9, 3))).astype(np.complex64)
+    a, b = 0.5 - 0.25j, -1.5 + 2j
+    wc = (a * w1.astype(np.complex128) + b * w2).astype(np.complex64)
+    E1 = oracle.field_los(src, w1, snk, nrm, 0.0, BETA)
+    E2 = oracle.field_los(src, w2, snk, nrm, 0.0, BETA)
+    Ec = oracle.field_los(src, wc, snk, nrm, 0.0, BETA)
+    assert np.allclose(Ec, a * E1 + b * E2, rtol=1e-6, atol=1e-7 * np.abs(Ec).max())
+    hit = rng.random((9, 7)) < 0.4
+    Em = oracle.field_los(src, w1, snk, nrm, 0.0, BETA, hit=hit)
+    El = oracle.field_los(src, w1, snk, nrm, 0.0, BETA, elements=True)
+    assert np.allclose(Em, (El * (~hit)[..., None]).sum(0), rtol=1e-12, atol=1e-18)
+    assert np.allclose(El.sum(0), E1, rtol=1e-12)
+
+
+def test_paths_match_dense_and_two_leg_factorisation(oracle):
+    rng = np.random.default_rng(6)
+    S, K, P = 4, 5, 6
+    src = rng.uniform(-1, 1, (S, 3)).astype(np.float32)
+    snk = rng.uniform(5, 9, (K, 3)).astype(np.float32)
+    sct = rng.uniform(1, 4, (P, 3)).astype(np.float32)
+    src_n = np.tile(np.array([[0, 0, 1]], np.float32), (S, 1))
+    snk_n = rng.normal(size=(K, 3)).astype(np.float32)
+    sct_n = rng.normal(size=(P, 3)).astype(np.float32)
+    ws = (rng.normal(size=(S, 3)) + 1j * rng.normal(size=(S, 3))).astype(np.complex64)
+    wp = (0.5 * np.ones((P, 3))).astype(np.complex64)
+    pos = np.concatenate([src, snk, sct])
+    nrm = np.concatenate([src_n, snk_n, sct_n])
+    wts = np.concatenate([ws, np.ones((K, 3), np.complex64), wp])
+    # 2-column index == dense LOS network
+    idx2 = np.array([[s + 1, S + 1 + k] for s in range(S) for k in range(K)], np.int64)
+    net2 = oracle.field_paths(pos, nrm, wts, idx2, S, K, 0.0, BETA)
+    El = oracle.field_los(src, ws, snk, snk_n, 0.0, BETA, elements=True)
+    assert np.allclose(net2, El, rtol=1e-12, atol=1e-18)
+    # 3-column index: LOS rows get the sink-tangent projection (reference quirk)
+    los = np.concatenate([idx2, np.zeros((len(idx2), 1), np.int64)], 1)
+    sc = np.array([[s + 1, S + K + 1 + p, S + 1 + k] for s in range(S) for p in range(P)
+                   for k in range(K)], np.int64)
+    net3 = oracle.field_paths(pos, nrm, wts, np.concatenate([los, sc]), S, K, 0.0, BETA)
+    Elt = oracle.field_los(src, ws, snk, snk_n, 0.0, BETA, elements=True, sink_tangent=True)
+    # two-leg factorisation of the scattered part, source-summed
+    H = oracle.field_los(src, ws, sct, sct_n, 0.0, BETA, snk_w=wp, sink_tangent=True)
+    Es = oracle.field_los(sct, H.astype(np.complex64), snk, snk_n, 0.0, BETA)
+    total = net3.sum(0)
+    # H is rounded to complex64 between the legs -> 1e-6 relative
+    assert np.allclose(total, Elt.sum(0) + Es, rtol=2e-6, atol=2e-6 * np.abs(total).max())
+
+
+def test_r_f32_error_is_reported_scale(oracle):
+    """Rounding r to float32 as the numba kernel does moves the phase of a 1 km
+    path by up to beta * ulp(1000)/2 ~ 2e-3 rad."""
+    src = np.zeros((1, 3), np.float32)
+    rng = np.random.default_rng(8)
+    u = rng.normal(size=(200, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    snk = (u * 1000).astype(np.float32)
+    w = np.array([[1, 0, 0]], np.complex64)
+    E = oracle.field_los(src, w, snk, u.astype(np.float32), 0.0, BETA)
+    E32 = oracle.field_los(src, w, snk, u.astype(np.float32), 0.0, BETA, r_f32=True)
+    ok = np.abs(E[:, 0]) > 1e-9
+    dph = np.abs(np.angle(E32[ok, 0] / E[ok, 0]))
+    assert 1e-4 < dph.max() < BETA * 6.2e-5 / 2 * 1.01
