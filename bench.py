@@ -1,0 +1,310 @@
+#!/usr/bin/env python
+"""Benchmark of the LyceanEM raycast + frequency-domain hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2] [--no-cpu]
+
+One "step" is one pass of the hot path over the configuration's synthetic scene
+with every input already resident in HBM:
+  c2  (default) 1000 x 500 quad (1,000,000 triangle) terrain, 1024 x 1024
+      sources 30 m up, 16,384 sinks 1.5 m up: uniform-grid DDA visibility of all
+      1.718e10 source -> sink rays, writing the hit mask and the kernel1Dv3
+      closest-hit distance of every ray (grid built once, outside the step).
+  c1  32x32 lambda/2 array (3 GHz), 2,000 random triangles, 91 x 181 far-field
+      sinks: brute-force visibility with kernel1Dv3 semantics (closest hit,
+      every ray tests every triangle) + the fused masked field (source sum).
+
+The dominant kernel's average duration comes from HIP events recorded on the
+stream the kernels run on (torch's current stream).  The CPU baseline is the
+oracle (C + OpenMP restatement of the reference algorithm) on a bounded sample
+of the same workload, on this host's cores.  Rank 0 prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "lyceanem-python_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+MT_FLOPS = 46          # FP32 flops of one Moller-Trumbore test (+1 divide)
+PAIR_FLOPS = 92        # FP32 flops of one Green's-function pair
+PEAK_FP32 = 157.3e12   # MI355X FP32 vector peak (packed)
+PEAK_HBM = 8.0e12
+
+
+def _events(torch, n):
+    return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+
+
+# --------------------------------------------------------------------------
+# config 1: brute-force visibility + line-of-sight field
+# --------------------------------------------------------------------------
+
+class C1:
+    name = "c1"
+
+    def __init__(self, torch):
+        from lyceanem import _native, scenes
+
+        self.torch, self.nat = torch, _native
+        self.lib = _native.load_library()
+        sc = scenes.planar_array_c1()
+        self.sc = sc
+        dev = _native.device()
+        f32 = torch.float32
+        self.src = torch.from_numpy(sc["src"]).to(dev)
+        self.snk = torch.from_numpy(sc["snk"]).to(dev)
+        self.snk_n = torch.from_numpy(sc["snk_n"]).to(dev)
+        self.w = torch.from_numpy(sc["w"]).to(dev)
+        self.tris = torch.from_numpy(sc["tris"]).to(dev)
+        self.S, self.K, self.T = len(sc["src"]), len(sc["snk"]), len(sc["tris"])
+        W = (self.K + 63) // 64
+        self.bits = torch.zeros((self.S, W), dtype=torch.int64, device=dev)
+        self.dist = torch.empty((self.S, self.K), dtype=f32, device=dev)
+        self.out = torch.empty((self.K, 3), dtype=torch.complex64, device=dev)
+        self.wsb = int(self.lib.lem_field_los_workspace(self.S, self.K, 0))
+        self.ws = torch.empty(self.wsb, dtype=torch.uint8, device=dev)
+        self.ev = None
+
+    def rays(self):
+        return self.S * self.K
+
+    def step(self, ev=None):
+        nat, lib, p = self.nat, self.lib, self.nat.ptr
+        st = nat.stream_ptr()
+        if ev is not None:
+            ev[0].record()
+        nat.check(lib.lem_visibility_dense(p(self.src), self.S, p(self.snk), self.K, p(self.tris),
+                                           self.T, nat.SEM_NUMBA, 0, p(self.bits), p(self.dist),
+                                           st), "visibility")
+        if ev is not None:
+            ev[1].record()
+        nat.check(lib.lem_field_los(p(self.src), p(self.w), self.S, p(self.snk), p(self.snk_n),
+                                    None, self.K, p(self.bits), 0.0, float(self.sc["beta"]), 0,
+                                    p(self.out), p(self.ws), self.wsb, st), "field")
+        if ev is not None:
+            ev[2].record()
+
+    def record(self, times_ms):
+        vis = float(np.mean([a for a, _ in times_ms]))
+        fld = float(np.mean([b for _, b in times_ms]))
+        flops = self.rays() * self.T * MT_FLOPS
+        achieved = flops / (vis * 1e-3)
+        return {
+            "kernel": "k_visibility_dense<numba, closest-hit>",
+            "kernel_ms": vis,
+            "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None},
+            "extra": {"field_kernel_ms": fld,
+                      "field_gpairs_per_s": self.rays() / (fld * 1e-3) / 1e9,
+                      "mt_tests": self.rays() * self.T},
+        }
+
+    def cpu_baseline(self, budget_s):
+        from oracle import oracle as orc
+
+        sc = self.sc
+        nth = orc.nthreads_default()
+        rows = 1
+        t0 = time.perf_counter()
+        orc.visibility_dense(sc["src"][:rows], sc["snk"], sc["tris"], want_dist=True,
+                             nthreads=nth)
+        dt = time.perf_counter() - t0
+        rows = int(max(1, min(self.S, budget_s / max(dt, 1e-6))))
+        t0 = time.perf_counter()
+        hit, _ = orc.visibility_dense(sc["src"][:rows], sc["snk"], sc["tris"], want_dist=True,
+                                      nthreads=nth)
+        orc.field_los(sc["src"][:rows], sc["w"][:rows], sc["snk"], sc["snk_n"], 0.0,
+                      sc["beta"], hit=hit, nthreads=nth)
+        dt = time.perf_counter() - t0
+        return {"value": rows * self.K / dt / 1e6, "unit": "Mrays/s", "cores": nth,
+                "kind": "port",
+                "sample": f"first {rows} of {self.S} source rows x {self.K} sinks x {self.T} "
+                          f"triangles (oracle closest-hit visibility + float64 field), "
+                          f"{dt:.1f} s"}
+
+    def config(self):
+        return {"workload": "c1: 32x32 array, 2000 random triangles, 91x181 sinks at 1 km, 3 GHz",
+                "sources": self.S, "sinks": self.K, "triangles": self.T, "rays": self.rays()}
+
+
+# --------------------------------------------------------------------------
+# config 2: 1M-triangle terrain, grid DDA visibility of 1.7e10 rays
+# --------------------------------------------------------------------------
+
+class C2:
+    name = "c2"
+    stats_stride = 8  # MT tests counted on every 8th sink, scaled
+
+    def __init__(self, torch, n_sinks=16384):
+        from lyceanem import _native, scenes
+
+        self.torch, self.nat = torch, _native
+        self.lib = _native.load_library()
+        t0 = time.perf_counter()
+        sc = scenes.terrain_c2(n_snk=n_sinks)
+        self.sc = sc
+        self.gen_s = time.perf_counter() - t0
+        dev = _native.device()
+        self.src = torch.from_numpy(sc["src"]).to(dev)
+        self.snk = torch.from_numpy(sc["snk"]).to(dev)
+        self.tris_t = torch.from_numpy(sc["tris"]).to(dev)
+        self.S, self.K, self.T = len(sc["src"]), len(sc["snk"]), len(sc["tris"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        self.grid = _native.Grid(self.tris_t)
+        torch.cuda.synchronize()
+        self.build_s = time.perf_counter() - t0
+        W = (self.S + 63) // 64
+        self.bits = torch.empty((self.K, W), dtype=torch.int64, device=dev)
+        self.dist = torch.empty((self.K, self.S), dtype=torch.float32, device=dev)
+
+    def rays(self):
+        return self.S * self.K
+
+    def step(self, ev=None):
+        nat, lib, p, g = self.nat, self.lib, self.nat.ptr, self.grid
+        if ev is not None:
+            ev[0].record()
+        nat.check(lib.lem_visibility_grid(p(self.src), self.S, p(self.snk), self.K,
+                                          nat.ctypes.byref(g.info), p(g.cell_start), p(g.recs),
+                                          nat.VIS_SINK_MAJOR, p(self.bits), p(self.dist),
+                                          nat.stream_ptr()), "visibility_grid")
+        if ev is not None:
+            ev[1].record()
+            ev[2].record()
+
+    def record(self, times_ms):
+        vis = float(np.mean([a for a, _ in times_ms]))
+        sub = self.snk[:: self.stats_stride].contiguous()
+        tests, cells, blocked = self.nat.grid_stats(self.grid, self.src, sub)
+        scale = self.K / sub.shape[0]
+        tests_all = tests * scale
+        achieved = tests_all * MT_FLOPS / (vis * 1e-3)
+        bytes_out = self.bits.numel() * 8 + self.dist.numel() * 4
+        return {
+            "kernel": "k_visibility_grid<closest-hit, sink-major>",
+            "kernel_ms": vis,
+            "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None},
+            "extra": {
+                "grid_build_s": self.build_s,
+                "grid_cells": [int(x) for x in self.grid.info.n],
+                "grid_refs": int(self.grid.info.nrefs),
+                "mt_tests_per_ray": tests / (self.S * sub.shape[0]),
+                "cells_per_ray": cells / (self.S * sub.shape[0]),
+                "blocked_fraction": blocked / (self.S * sub.shape[0]),
+                "work_counted_on": f"every {self.stats_stride}th sink ({sub.shape[0]} sinks), "
+                                   f"scaled x{scale:g}",
+                "output_bytes": bytes_out,
+                "output_write_GBps": bytes_out / (vis * 1e-3) / 1e9,
+            },
+        }
+
+    def cpu_baseline(self, budget_s):
+        from oracle import oracle as orc
+
+        sc = self.sc
+        nth = orc.nthreads_default()
+        rng = np.random.default_rng(0)
+        n = 64
+        pairs = np.stack([rng.integers(0, self.S, n), rng.integers(0, self.K, n)], 1)
+        t0 = time.perf_counter()
+        orc.visibility_sampled(sc["src"], sc["snk"], pairs, sc["tris"], nthreads=nth)
+        dt = time.perf_counter() - t0
+        n = int(max(64, min(10 ** 6, n * budget_s / max(dt, 1e-6))))
+        pairs = np.stack([rng.integers(0, self.S, n), rng.integers(0, self.K, n)], 1)
+        t0 = time.perf_counter()
+        orc.visibility_sampled(sc["src"], sc["snk"], pairs, sc["tris"], nthreads=nth)
+        dt = time.perf_counter() - t0
+        return {"value": n / dt / 1e6, "unit": "Mrays/s", "cores": nth, "kind": "port",
+                "sample": f"{n} random source->sink rays of the 1.718e10, brute force over all "
+                          f"{self.T} triangles (kernel1Dv3 closest hit, the reference "
+                          f"algorithm), {dt:.1f} s"}
+
+    def config(self):
+        return {"workload": "c2: 1000x500-quad terrain (1,000,000 triangles, Gaussian heights "
+                            "sigma 5 m, corr 50 m), 1024x1024 sources at +30 m, 16384 sinks at "
+                            "+1.5 m; grid DDA, mask + closest-hit distance per ray",
+                "sources": self.S, "sinks": self.K, "triangles": self.T, "rays": self.rays()}
+
+
+CONFIGS = {"c1": C1, "c2": C2}
+DEFAULT_CONFIG = "c2"
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default=DEFAULT_CONFIG, choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--c2-sinks", type=int, default=16384, help="c2 sink count (probe runs)")
+    a = ap.parse_args(argv)
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the sharding plan is design-only).", file=sys.stderr)
+        return 2
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        return 2
+    torch.cuda.set_device(0)
+    cfg = C2(torch, a.c2_sinks) if a.config == "c2" else CONFIGS[a.config](torch)
+    print(f"[bench] config {a.config}: setup done", file=sys.stderr, flush=True)
+    for _ in range(a.warmup):
+        cfg.step()
+    torch.cuda.synchronize()
+    print(f"[bench] warmup {a.warmup} done", file=sys.stderr, flush=True)
+    evs = [_events(torch, 3) for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        cfg.step(evs[i])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) / a.steps * 1e3
+    times = [(e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])) for e in evs]
+    rec = cfg.record(times)
+    print(f"[bench] timed {a.steps} steps: {ms:.2f} ms/step", file=sys.stderr, flush=True)
+    cpu = None
+    if not a.no_cpu:
+        cpu = cfg.cpu_baseline(a.cpu_seconds)
+        print(f"[bench] cpu baseline: {cpu['value']:.3f} {cpu['unit']}", file=sys.stderr,
+              flush=True)
+    line = {
+        "metric": "Mrays/s (1M-tri DDA); Gpairs/s scattering + % roofline; adjoint/forward time",
+        "value": cfg.rays() / (ms * 1e-3) / 1e6,
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 phase)",
+        "data": "synthetic",
+        "config": cfg.config(),
+        "roofline": rec["roofline"],
+        "cpu_baseline": cpu,
+        "dominant_kernel": {"name": rec["kernel"], "avg_ms": rec["kernel_ms"]},
+        "extra": rec["extra"],
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

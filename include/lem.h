@@ -52,6 +52,8 @@ extern "C" {
 #define LEM_VIS_SKIP_SELF 2 /* never test pairs with source row == sink row
                                (native flag = !self_to_self,
                                raycasting_accelerated.cuh:72) */
+#define LEM_VIS_SINK_MAJOR 4 /* grid traversal only: lanes run over sources, bits
+                                are K x ceil(S/64) words and dist is K x S */
 
 /* Field flags. */
 #define LEM_FIELD_SINK_TANGENT 1 /* project onto the sink tangent plane, as
@@ -116,6 +118,76 @@ int lem_field_paths(const float *pos, const float *nrm, const float *w, int64_t 
                     const int32_t *index, int ncol, const int64_t *order,
                     const int64_t *seg_start, const int64_t *seg_slot, int64_t nseg,
                     double alpha, double beta, double *out, int *err_flag, void *stream);
+
+/* Uniform-grid acceleration structure (host-side descriptor; the arrays it
+ * describes are device buffers the caller allocates from the sizes here). */
+typedef struct {
+    float lo[3];    /* grid origin: scene bounds minus 2*eps */
+    float cell[3];  /* cell edge per axis */
+    int32_t n[3];   /* cells per axis */
+    int32_t pad;
+    int64_t ncells; /* n[0]*n[1]*n[2]; cell (x,y,z) is ((z*n[1] + y)*n[0] + x) */
+    int64_t nrefs;  /* triangle references summed over cells */
+    float eps;      /* inflation of triangle boxes and traversal margin */
+    float pad2;
+} lem_grid_info;
+
+/* Grid build, replacing the tile binning of bin_counts_to_numpy /
+ * bin_triangles_to_numpy (terrain_acceleration_build.cuh:287-431) with a
+ * general 3-D grid.  cells_per_tri sets the cell budget (cells ~ T * value,
+ * 2 is a good default).
+ * 1) lem_grid_plan: bounds, resolution and per-cell counts; synchronises the
+ *    stream and fills *info (ncells, nrefs).
+ * 2) the caller allocates cell_start (ncells + 1 int32), tri_ids (nrefs int32)
+ *    and recs (nrefs x 12 float32: v0, e1, e2 and padding, cell by cell), and
+ * 3) lem_grid_fill writes them; the same workspace must be passed to both. */
+int64_t lem_grid_workspace(int64_t T, float cells_per_tri);
+int lem_grid_plan(const float *tris, int64_t T, float cells_per_tri, lem_grid_info *info,
+                  void *workspace, int64_t workspace_bytes, void *stream);
+int lem_grid_fill(const float *tris, int64_t T, const lem_grid_info *info, int32_t *cell_start,
+                  int32_t *tri_ids, float *recs, void *workspace, int64_t workspace_bytes,
+                  float cells_per_tri, void *stream);
+
+/* Dense S x K visibility through the grid (DDA): the same masks and kernel1Dv3
+ * distances as lem_visibility_dense with LEM_SEM_NUMBA.  Replaces the tile
+ * traversal raycast_tiles / DDA (raycasting_accelerated.cuh:106-268, 458-482)
+ * for source -> sink rays.  Flags: LEM_VIS_ANY_HIT, LEM_VIS_SINK_MAJOR. */
+int lem_visibility_grid(const float *src, int64_t S, const float *snk, int64_t K,
+                        const lem_grid_info *info, const int32_t *cell_start, const float *recs,
+                        int flags, uint64_t *hit_bits, float *dist, void *stream);
+
+/* Line-of-sight field fused with the grid traversal: lem_field_los (source
+ * sum only) where each ray is cast any-hit through the grid instead of read
+ * from a precomputed mask -- the S x K mask is never stored.  Same workspace
+ * size as lem_field_los_workspace(S, K, 0). */
+int lem_field_los_grid(const float *src, const float *src_w, int64_t S, const float *snk,
+                       const float *snk_n, const float *snk_w, int64_t K,
+                       const lem_grid_info *info, const int32_t *cell_start, const float *recs,
+                       double alpha, double beta, int flags, float *out, void *workspace,
+                       int64_t workspace_bytes, void *stream);
+
+/* Hand-written adjoint of lem_field_los (source sum) with the mask frozen.
+ * grad_out: K x 3 complex64, dLoss/dE in PyTorch's convention
+ * (dL/dRe E + j dL/dIm E).  Writes grad_w (S x 3 complex64, dLoss/dw_s) and
+ * grad_p (S x 3 float32, dLoss/d source position); either may be NULL.
+ * flags: the forward's LEM_FIELD_SINK_TANGENT; snk_w as in the forward.
+ * No counterpart in the reference (it has no gradients); it differentiates
+ * lossy_propagation and the launch projector (empropagation.py:178-249,
+ * 768-801) as freqdomainkernal combines them for 2-column paths. */
+int64_t lem_field_los_adjoint_workspace(int64_t S, int64_t K);
+int lem_field_los_adjoint(const float *src, const float *src_w, int64_t S, const float *snk,
+                          const float *snk_n, const float *snk_w, int64_t K,
+                          const uint64_t *hit_bits, double alpha, double beta, int flags,
+                          const float *grad_out, float *grad_w, float *grad_p, void *workspace,
+                          int64_t workspace_bytes, void *stream);
+
+/* Instrumented grid traversal for measurement: adds, over all S x K rays, the
+ * Moller-Trumbore tests performed (stats[0]), cells visited (stats[1]) and
+ * blocked rays (stats[2]) to the device uint64 array `stats` (zero it first).
+ * Writes no masks.  No counterpart in the reference (it reports no work counts). */
+int lem_visibility_grid_stats(const float *src, int64_t S, const float *snk, int64_t K,
+                              const lem_grid_info *info, const int32_t *cell_start,
+                              const float *recs, int flags, uint64_t *stats, void *stream);
 
 #ifdef __cplusplus
 }

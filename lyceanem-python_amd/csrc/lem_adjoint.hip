@@ -1,0 +1,297 @@
+// lem_adjoint.hip -- hand-written adjoint of the line-of-sight field (gfx950).
+//
+// Forward (lem_field_los, source sum):   E_k = sum_s vis(s,k) c_sk,
+//   c_sk = F(r) (d . n_k) (I - d d^T) w_s,  d = (p_k - p_s)/r,
+//   F(r) = -(1/2pi) (-g - 1/r) e^{-g r}/r,  g = alpha + j beta
+// (lossy_propagation, empropagation.py:768-801, and the launch projector,
+// :178-249).  Given gE_k = dLoss/dE_k in PyTorch's convention
+// (dL/dRe + j dL/dIm) and the frozen mask, this computes
+//   grad_w_s = sum_k vis conj(F cos) P(d) gE_k
+//   grad_p_s = Re sum_k vis [ -F' cos (U.Q) d - F (U.Q)(n_k - d cos)/r
+//                             + F cos ((d.w) M U + (U.d) Q)/r ]
+// with U = conj(gE_k), Q = P(d) w_s, M = I - d d^T and
+// F' = dF/dr = -(1/2pi) e^{-g r}/r (g^2 + 2g/r + 2/r^2); bilinear dots (no
+// conjugation).  Sinks are staged through LDS, lanes run over sources, chunk
+// partials are Kahan-summed and merged in a fixed order.  The sink-weight and
+// sink-tangent steps of the forward are applied to gE on entry (their
+// transposes), so the adjoint matches lem_field_los for every flag.
+#include "lem_device.h"
+#include "lem_field_dev.h"
+#include "lem_internal.h"
+#include "../../include/lem.h"
+
+namespace lem {
+
+constexpr int AB = 256;      // sources per block
+constexpr int AK_TILE = 256; // sinks per LDS tile
+
+struct SnkLds {
+    double x, y, z;
+    float nx, ny, nz;
+    float g[6];  // gE (re, im) x 3
+};
+
+// gE <- P(n_k) (conj(w_k) * gE): transposes of the forward's tangent step and
+// sink-weight product, in reverse order.
+__global__ __launch_bounds__(AB) void k_adjoint_prep(const float *__restrict__ gE,
+                                                     const float *__restrict__ snk_n,
+                                                     const float *__restrict__ snk_w, int tangent,
+                                                     int64_t K, float *__restrict__ g_out)
+{
+    int64_t k = (int64_t)blockIdx.x * AB + threadIdx.x;
+    if (k >= K)
+        return;
+    C3 g = load_w(gE, k);
+    if (tangent)
+        g = project_normal(g, snk_n[3 * k], snk_n[3 * k + 1], snk_n[3 * k + 2]);
+    if (snk_w) {
+        C3 w = load_w(snk_w, k);
+        w.xi = -w.xi; w.yi = -w.yi; w.zi = -w.zi;
+        g = cmul_comp(g, w);
+    }
+    float2 *o = reinterpret_cast<float2 *>(g_out) + 3 * k;
+    o[0] = make_float2(g.xr, g.xi);
+    o[1] = make_float2(g.yr, g.yi);
+    o[2] = make_float2(g.zr, g.zi);
+}
+
+LEM_DEV float2 cmulf(float ar, float ai, float br, float bi)
+{
+    return make_float2(ar * br - ai * bi, ar * bi + ai * br);
+}
+
+// Partials: block b covers sources [(b % nbs)*256, +256), sink chunk c = b / nbs.
+__global__ __launch_bounds__(AB) void k_adjoint_partial(
+    const float *__restrict__ src, const float *__restrict__ src_w, int64_t S,
+    const float *__restrict__ snk, const float *__restrict__ snk_n, const float *__restrict__ g,
+    int64_t K, const unsigned long long *__restrict__ hit_bits, int64_t W64, float alpha,
+    double beta, int64_t CH, int64_t nbs, float *__restrict__ ws_sum, float *__restrict__ ws_cmp)
+{
+    __shared__ SnkLds s_k[AK_TILE];
+    const int64_t c = (int64_t)blockIdx.x / nbs;
+    const int64_t s = ((int64_t)blockIdx.x % nbs) * AB + threadIdx.x;
+    const bool active = s < S;
+    const int64_t ss = active ? s : 0;
+    const double xs = src[3 * ss], ys = src[3 * ss + 1], zs = src[3 * ss + 2];
+    const C3 w = load_w(src_w, ss);
+    const int64_t k0 = c * CH;
+    const int64_t k1 = (k0 + CH) < K ? (k0 + CH) : K;
+    const float beta_f = (float)beta;
+    const float FRONT = -0.15915494309189535f;
+    KahanC3 gw;
+    gw.zero();
+    float gp[3] = {0.0f, 0.0f, 0.0f}, gpc[3] = {0.0f, 0.0f, 0.0f};
+    for (int64_t base = k0; base < k1; base += AK_TILE) {
+        __syncthreads();
+        {
+            int64_t j = base + threadIdx.x;
+            SnkLds L;
+            if (j < k1) {
+                L.x = snk[3 * j]; L.y = snk[3 * j + 1]; L.z = snk[3 * j + 2];
+                L.nx = snk_n[3 * j]; L.ny = snk_n[3 * j + 1]; L.nz = snk_n[3 * j + 2];
+                C3 gg = load_w(g, j);
+                L.g[0] = gg.xr; L.g[1] = gg.xi; L.g[2] = gg.yr;
+                L.g[3] = gg.yi; L.g[4] = gg.zr; L.g[5] = gg.zi;
+            } else {
+                L.x = L.y = L.z = 0.0;
+                L.nx = L.ny = L.nz = 0.0f;
+                for (int i = 0; i < 6; ++i)
+                    L.g[i] = 0.0f;
+            }
+            s_k[threadIdx.x] = L;
+        }
+        __syncthreads();
+        const int n = (int)((k1 - base) < AK_TILE ? (k1 - base) : AK_TILE);
+        if (!active)
+            continue;
+        unsigned long long word = 0;
+        for (int j = 0; j < n; ++j) {
+            const int64_t k = base + j;
+            if (hit_bits) {
+                if (j == 0 || (k & 63) == 0)
+                    word = hit_bits[s * W64 + (k >> 6)];
+                if ((word >> (k & 63)) & 1ull)
+                    continue;
+            }
+            const SnkLds &L = s_k[j];
+            double dx = L.x - xs, dy = L.y - ys, dz = L.z - zs;
+            Phase p = phase_of(dx, dy, dz, beta);
+            const float ir = p.inv_r;
+            const float ux = (float)dx * ir, uy = (float)dy * ir, uz = (float)dz * ir;
+            const float cs = ux * L.nx + uy * L.ny + uz * L.nz;
+            // G = e^{-alpha r}/r e^{-j beta r}
+            float amp = ir;
+            if (alpha != 0.0f)
+                amp *= expf(-alpha * (float)p.r);
+            const float Gr = amp * p.c, Gi = amp * p.s;
+            // F = FRONT (-g - 1/r) G ; F' = FRONT (g^2 + 2g/r + 2/r^2) G
+            float2 F = cmulf(-alpha - ir, -beta_f, Gr, Gi);
+            F.x *= FRONT;
+            F.y *= FRONT;
+            const float b_r = alpha * alpha - beta_f * beta_f + 2.0f * alpha * ir + 2.0f * ir * ir;
+            const float b_i = 2.0f * alpha * beta_f + 2.0f * beta_f * ir;
+            float2 dF = cmulf(b_r, b_i, Gr, Gi);
+            dF.x *= FRONT;
+            dF.y *= FRONT;
+            // Q = P(d) w,  uw = d.w
+            const float uwr = ux * w.xr + uy * w.yr + uz * w.zr;
+            const float uwi = ux * w.xi + uy * w.yi + uz * w.zi;
+            const float Qxr = w.xr - ux * uwr, Qxi = w.xi - ux * uwi;
+            const float Qyr = w.yr - uy * uwr, Qyi = w.yi - uy * uwi;
+            const float Qzr = w.zr - uz * uwr, Qzi = w.zi - uz * uwi;
+            // gv = gE_k, U = conj(gv)
+            const float gxr = L.g[0], gxi = L.g[1], gyr = L.g[2], gyi = L.g[3], gzr = L.g[4],
+                        gzi = L.g[5];
+            // weight gradient: conj(F cos) (gv - d (d.gv))
+            {
+                const float ugr = ux * gxr + uy * gyr + uz * gzr;
+                const float ugi = ux * gxi + uy * gyi + uz * gzi;
+                const float Pxr = gxr - ux * ugr, Pxi = gxi - ux * ugi;
+                const float Pyr = gyr - uy * ugr, Pyi = gyi - uy * ugi;
+                const float Pzr = gzr - uz * ugr, Pzi = gzi - uz * ugi;
+                const float Lr = F.x * cs, Li = -F.y * cs;  // conj(F cos)
+                float v[6];
+                v[0] = Lr * Pxr - Li * Pxi; v[1] = Lr * Pxi + Li * Pxr;
+                v[2] = Lr * Pyr - Li * Pyi; v[3] = Lr * Pyi + Li * Pyr;
+                v[4] = Lr * Pzr - Li * Pzi; v[5] = Lr * Pzi + Li * Pzr;
+                gw.add(v);
+            }
+            // position gradient
+            {
+                // U.Q and U.d with U = conj(g)
+                const float UQr = gxr * Qxr + gxi * Qxi + gyr * Qyr + gyi * Qyi + gzr * Qzr +
+                                  gzi * Qzi;
+                const float UQi = gxr * Qxi - gxi * Qxr + gyr * Qyi - gyi * Qyr + gzr * Qzi -
+                                  gzi * Qzr;
+                const float Udr = gxr * ux + gyr * uy + gzr * uz;
+                const float Udi = -(gxi * ux + gyi * uy + gzi * uz);
+                // M U = U - d (U.d)
+                const float MUxr = gxr - ux * Udr, MUxi = -gxi - ux * Udi;
+                const float MUyr = gyr - uy * Udr, MUyi = -gyi - uy * Udi;
+                const float MUzr = gzr - uz * Udr, MUzi = -gzi - uz * Udi;
+                // term1 coefficient on d: -F' cos (U.Q)
+                const float2 t1 = cmulf(dF.x, dF.y, UQr, UQi);
+                const float a1 = -t1.x * cs;
+                // term2: -F (U.Q) (n - d cos)/r  (real part only)
+                const float2 t2 = cmulf(F.x, F.y, UQr, UQi);
+                const float a2 = -t2.x * ir;
+                // term3: F cos ((d.w) M U + (U.d) Q)/r
+                const float2 Fc = make_float2(F.x * cs * ir, F.y * cs * ir);
+                const float2 c1 = cmulf(Fc.x, Fc.y, uwr, uwi);
+                const float2 c2 = cmulf(Fc.x, Fc.y, Udr, Udi);
+                float v[3];
+                v[0] = a1 * ux + a2 * (L.nx - ux * cs) + (c1.x * MUxr - c1.y * MUxi) +
+                       (c2.x * Qxr - c2.y * Qxi);
+                v[1] = a1 * uy + a2 * (L.ny - uy * cs) + (c1.x * MUyr - c1.y * MUyi) +
+                       (c2.x * Qyr - c2.y * Qyi);
+                v[2] = a1 * uz + a2 * (L.nz - uz * cs) + (c1.x * MUzr - c1.y * MUzi) +
+                       (c2.x * Qzr - c2.y * Qzi);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    float y = v[i] - gpc[i];
+                    float t = gp[i] + y;
+                    gpc[i] = (t - gp[i]) - y;
+                    gp[i] = t;
+                }
+            }
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            ws_sum[(c * 9 + i) * S + s] = gw.s[i];
+            ws_cmp[(c * 9 + i) * S + s] = gw.c[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            ws_sum[(c * 9 + 6 + i) * S + s] = gp[i];
+            ws_cmp[(c * 9 + 6 + i) * S + s] = gpc[i];
+        }
+    }
+}
+
+__global__ __launch_bounds__(AB) void k_adjoint_finish(const float *__restrict__ ws_sum,
+                                                       const float *__restrict__ ws_cmp,
+                                                       int64_t NC, int64_t S,
+                                                       float *__restrict__ grad_w,
+                                                       float *__restrict__ grad_p)
+{
+    const int64_t s = (int64_t)blockIdx.x * AB + threadIdx.x;
+    if (s >= S)
+        return;
+    double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t c = 0; c < NC; ++c) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+            a[i] += (double)ws_sum[(c * 9 + i) * S + s] - (double)ws_cmp[(c * 9 + i) * S + s];
+    }
+    if (grad_w) {
+        for (int i = 0; i < 6; ++i)
+            grad_w[6 * s + i] = (float)a[i];
+    }
+    if (grad_p) {
+        for (int i = 0; i < 3; ++i)
+            grad_p[3 * s + i] = (float)a[6 + i];
+    }
+}
+
+// Sink chunks: enough (source, chunk) threads to fill the chip, chunk length a
+// multiple of the LDS tile (keeps 64-bit mask words aligned).
+inline int64_t adj_chunks(int64_t S, int64_t K)
+{
+    int64_t want = ceil_div(1 << 19, S > 0 ? S : 1);
+    int64_t cap = ceil_div(K, AK_TILE);
+    int64_t nc = want < cap ? want : cap;
+    return nc < 1 ? 1 : nc;
+}
+
+}  // namespace lem
+
+using namespace lem;
+
+extern "C" int64_t lem_field_los_adjoint_workspace(int64_t S, int64_t K)
+{
+    if (S <= 0 || K <= 0)
+        return 0;
+    const int64_t NC = adj_chunks(S, K);
+    return 2 * NC * 9 * S * (int64_t)sizeof(float) + 6 * K * (int64_t)sizeof(float) + 256;
+}
+
+extern "C" int lem_field_los_adjoint(const float *src, const float *src_w, int64_t S,
+                                     const float *snk, const float *snk_n, const float *snk_w,
+                                     int64_t K, const uint64_t *hit_bits, double alpha,
+                                     double beta, int flags, const float *grad_out,
+                                     float *grad_w, float *grad_p, void *workspace,
+                                     int64_t workspace_bytes, void *stream)
+{
+    clear_error();
+    LEM_REQUIRE(S > 0 && K > 0, "lem_field_los_adjoint: empty source or sink set");
+    LEM_REQUIRE(src && src_w && snk && snk_n && grad_out && (grad_w || grad_p),
+                "lem_field_los_adjoint: null pointer");
+    LEM_REQUIRE(!(flags & LEM_FIELD_ELEMENTS), "lem_field_los_adjoint: source-sum forward only");
+    const int64_t need = lem_field_los_adjoint_workspace(S, K);
+    if (!workspace || workspace_bytes < need) {
+        set_error("lem_field_los_adjoint: workspace smaller than lem_field_los_adjoint_workspace()");
+        return E_WORKSPACE;
+    }
+    const int64_t NC = adj_chunks(S, K);
+    const int64_t CH = ceil_div(ceil_div(K, NC), AK_TILE) * AK_TILE;
+    const int64_t NCe = ceil_div(K, CH);
+    const int64_t nbs = ceil_div(S, AB);
+    LEM_REQUIRE(NCe * nbs <= 2147483647LL, "lem_field_los_adjoint: grid too large");
+    float *ws_sum = reinterpret_cast<float *>(workspace);
+    float *ws_cmp = ws_sum + NC * 9 * S;
+    float *g = ws_cmp + NC * 9 * S;
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(k_adjoint_prep, dim3((unsigned)ceil_div(K, AB)), dim3(AB), 0, st, grad_out,
+                       snk_n, snk_w, (flags & LEM_FIELD_SINK_TANGENT) ? 1 : 0, K, g);
+    hipLaunchKernelGGL(k_adjoint_partial, dim3((unsigned)(NCe * nbs)), dim3(AB), 0, st, src, src_w,
+                       S, snk, snk_n, g, K, reinterpret_cast<const unsigned long long *>(hit_bits),
+                       ceil_div(K, 64), (float)alpha, beta, CH, nbs, ws_sum, ws_cmp);
+    int rc = check_launch("k_adjoint_partial");
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_adjoint_finish, dim3((unsigned)nbs), dim3(AB), 0, st, ws_sum, ws_cmp, NCe,
+                       S, grad_w, grad_p);
+    return check_launch("k_adjoint_finish");
+}

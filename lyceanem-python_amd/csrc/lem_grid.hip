@@ -1,0 +1,802 @@
+// lem_grid.hip -- uniform-grid acceleration structure and DDA visibility (gfx950).
+//
+// Build (replaces bin_counts_to_numpy / bin_triangles_to_numpy,
+// terrain_acceleration_build.cuh:287-431, whose offsets are an O(B^2) loop in a
+// single 512-thread block):
+//   plan : scene bounds (device reduction), resolution on the host, per-cell
+//          reference counts (each triangle's box inflated by eps, so every cell
+//          it touches holds it), exclusive scan -> total references.
+//   fill : slots by atomic cursor, then each cell's list sorted by triangle id
+//          (deterministic order), then the (v0, e1, e2) records gathered in cell
+//          order so a cell's triangles are one contiguous 48-byte-stride run.
+// Traversal (replaces the tile "DDA" of raycasting_accelerated.cuh:106-268,
+// which scans every tile of the bounding rectangle): Amanatides-Woo stepping
+// over the cells the ray segment crosses, boundary distances recomputed from the
+// ray origin at every step (no accumulated error), and the numba predicate of
+// lem_device.h on every triangle of each visited cell.  The closest-hit mode
+// keeps kernel1Dv3's shrinking window and stops once the best hit lies before
+// the current cell's exit; the any-hit mode stops at the first accepted hit.
+// Masks are identical to brute force: triangle boxes are inflated by eps, which
+// exceeds the float32 error of the cell stepping, so a triangle the predicate
+// accepts always sits in a visited cell.
+#include "lem_device.h"
+#include "lem_field_dev.h"
+#include "lem_internal.h"
+#include "../../include/lem.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstring>
+
+namespace lem {
+
+constexpr int GB = 256;
+
+// Order-preserving float <-> uint key (for atomic min / max of floats).
+LEM_DEV unsigned int fkey(float f)
+{
+    unsigned int u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+inline float fkey_inv(unsigned int k)
+{
+    unsigned int u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+__global__ void k_bounds_init(unsigned int *b)
+{
+    if (threadIdx.x < 3)
+        b[threadIdx.x] = 0xffffffffu;  // min keys
+    else if (threadIdx.x < 6)
+        b[threadIdx.x] = 0u;  // max keys
+}
+
+__global__ __launch_bounds__(GB) void k_bounds(const float *__restrict__ tris, int64_t n_vals,
+                                               unsigned int *__restrict__ b)
+{
+    __shared__ unsigned int smin[3], smax[3];
+    if (threadIdx.x < 3) {
+        smin[threadIdx.x] = 0xffffffffu;
+        smax[threadIdx.x] = 0u;
+    }
+    __syncthreads();
+    unsigned int lmin[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
+    unsigned int lmax[3] = {0u, 0u, 0u};
+    // n_vals = 3 * number of vertices; value i has axis i % 3
+    for (int64_t v = (int64_t)blockIdx.x * GB + threadIdx.x; v * 3 < n_vals;
+         v += (int64_t)gridDim.x * GB) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            unsigned int k = fkey(tris[3 * v + a]);
+            lmin[a] = k < lmin[a] ? k : lmin[a];
+            lmax[a] = k > lmax[a] ? k : lmax[a];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        atomicMin(&smin[a], lmin[a]);
+        atomicMax(&smax[a], lmax[a]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        atomicMin(&b[threadIdx.x], smin[threadIdx.x]);
+        atomicMax(&b[3 + threadIdx.x], smax[threadIdx.x]);
+    }
+}
+
+struct GridDev {
+    float lox, loy, loz;
+    float cx, cy, cz;     // cell size
+    float icx, icy, icz;  // 1 / cell size
+    int nx, ny, nz;
+    float eps;
+};
+
+inline GridDev to_dev(const lem_grid_info &g)
+{
+    GridDev d;
+    d.lox = g.lo[0]; d.loy = g.lo[1]; d.loz = g.lo[2];
+    d.cx = g.cell[0]; d.cy = g.cell[1]; d.cz = g.cell[2];
+    d.icx = 1.0f / g.cell[0]; d.icy = 1.0f / g.cell[1]; d.icz = 1.0f / g.cell[2];
+    d.nx = g.n[0]; d.ny = g.n[1]; d.nz = g.n[2];
+    d.eps = g.eps;
+    return d;
+}
+
+LEM_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+LEM_DEV void tri_cell_range(const GridDev &G, const float *t9, int lo[3], int hi[3])
+{
+    float mn[3], mx[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float v0 = t9[a], v1 = t9[3 + a], v2 = t9[6 + a];
+        mn[a] = fminf(v0, fminf(v1, v2)) - G.eps;
+        mx[a] = fmaxf(v0, fmaxf(v1, v2)) + G.eps;
+    }
+    lo[0] = clampi((int)floorf((mn[0] - G.lox) * G.icx), 0, G.nx - 1);
+    lo[1] = clampi((int)floorf((mn[1] - G.loy) * G.icy), 0, G.ny - 1);
+    lo[2] = clampi((int)floorf((mn[2] - G.loz) * G.icz), 0, G.nz - 1);
+    hi[0] = clampi((int)floorf((mx[0] - G.lox) * G.icx), 0, G.nx - 1);
+    hi[1] = clampi((int)floorf((mx[1] - G.loy) * G.icy), 0, G.ny - 1);
+    hi[2] = clampi((int)floorf((mx[2] - G.loz) * G.icz), 0, G.nz - 1);
+}
+
+__global__ __launch_bounds__(GB) void k_count(const float *__restrict__ tris, int64_t T, GridDev G,
+                                              int *__restrict__ counts)
+{
+    int64_t t = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (t >= T)
+        return;
+    int lo[3], hi[3];
+    tri_cell_range(G, tris + 9 * t, lo, hi);
+    for (int z = lo[2]; z <= hi[2]; ++z)
+        for (int y = lo[1]; y <= hi[1]; ++y)
+            for (int x = lo[0]; x <= hi[0]; ++x)
+                atomicAdd(&counts[((int64_t)z * G.ny + y) * G.nx + x], 1);
+}
+
+// Exclusive scan, three phases: per-block scan + block sums, scan of the block
+// sums in one block, add-back.  counts[n] receives the total.
+__global__ __launch_bounds__(GB) void k_scan_blocks(const int *__restrict__ in, int64_t n,
+                                                    long long *__restrict__ out,
+                                                    long long *__restrict__ bsum)
+{
+    __shared__ long long s[GB];
+    int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
+    long long v = i < n ? (long long)in[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < GB; off <<= 1) {
+        long long a = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += a;
+        __syncthreads();
+    }
+    if (i < n)
+        out[i] = s[threadIdx.x] - v;
+    if (threadIdx.x == GB - 1)
+        bsum[blockIdx.x] = s[GB - 1];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(long long *__restrict__ bsum, int64_t nb,
+                                                    long long *__restrict__ total)
+{
+    __shared__ long long s[1024];
+    // each thread owns a contiguous run of ceil(nb / 1024) sums
+    int64_t per = (nb + 1023) / 1024;
+    int64_t b0 = threadIdx.x * per;
+    int64_t b1 = b0 + per < nb ? b0 + per : nb;
+    long long acc = 0;
+    for (int64_t b = b0; b < b1; ++b)
+        acc += bsum[b];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        long long a = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += a;
+        __syncthreads();
+    }
+    long long run = s[threadIdx.x] - acc;
+    for (int64_t b = b0; b < b1; ++b) {
+        long long v = bsum[b];
+        bsum[b] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023)
+        *total = s[1023];
+}
+
+__global__ __launch_bounds__(GB) void k_scan_add(long long *__restrict__ out, int64_t n,
+                                                 const long long *__restrict__ bsum)
+{
+    int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (i < n)
+        out[i] += bsum[blockIdx.x];
+}
+
+__global__ __launch_bounds__(GB) void k_starts(const long long *__restrict__ scan, int64_t n,
+                                               long long total, int32_t *__restrict__ cell_start,
+                                               int *__restrict__ cursor)
+{
+    int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (i < n) {
+        cell_start[i] = (int32_t)scan[i];
+        cursor[i] = (int)scan[i];
+    } else if (i == n) {
+        cell_start[n] = (int32_t)total;
+    }
+}
+
+__global__ __launch_bounds__(GB) void k_fill(const float *__restrict__ tris, int64_t T, GridDev G,
+                                             int *__restrict__ cursor, int32_t *__restrict__ ids)
+{
+    int64_t t = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (t >= T)
+        return;
+    int lo[3], hi[3];
+    tri_cell_range(G, tris + 9 * t, lo, hi);
+    for (int z = lo[2]; z <= hi[2]; ++z)
+        for (int y = lo[1]; y <= hi[1]; ++y)
+            for (int x = lo[0]; x <= hi[0]; ++x) {
+                int slot = atomicAdd(&cursor[((int64_t)z * G.ny + y) * G.nx + x], 1);
+                ids[slot] = (int32_t)t;
+            }
+}
+
+// Each cell's ids in increasing triangle order (shell sort; lists are short).
+__global__ __launch_bounds__(GB) void k_sort_cells(const int32_t *__restrict__ cell_start,
+                                                   int64_t ncells, int32_t *__restrict__ ids)
+{
+    int64_t c = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (c >= ncells)
+        return;
+    int32_t b = cell_start[c], n = cell_start[c + 1] - b;
+    int32_t *a = ids + b;
+    for (int gap = n / 2; gap > 0; gap /= 2) {
+        for (int i = gap; i < n; ++i) {
+            int32_t v = a[i];
+            int j = i;
+            while (j >= gap && a[j - gap] > v) {
+                a[j] = a[j - gap];
+                j -= gap;
+            }
+            a[j] = v;
+        }
+    }
+}
+
+__global__ __launch_bounds__(GB) void k_gather(const float *__restrict__ tris,
+                                               const int32_t *__restrict__ ids, int64_t nrefs,
+                                               float4 *__restrict__ recs)
+{
+    int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (i >= nrefs)
+        return;
+    Tri t = make_tri(tris + 9 * (int64_t)ids[i]);
+    recs[3 * i + 0] = make_float4(t.v0x, t.v0y, t.v0z, t.e1x);
+    recs[3 * i + 1] = make_float4(t.e1y, t.e1z, t.e2x, t.e2y);
+    recs[3 * i + 2] = make_float4(t.e2z, 0.0f, 0.0f, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// DDA traversal
+// ---------------------------------------------------------------------------
+
+LEM_DEV Tri load_rec(const float4 *__restrict__ recs, int32_t i)
+{
+    float4 a = recs[3 * (int64_t)i], b = recs[3 * (int64_t)i + 1], c = recs[3 * (int64_t)i + 2];
+    Tri t;
+    t.v0x = a.x; t.v0y = a.y; t.v0z = a.z; t.e1x = a.w;
+    t.e1y = b.x; t.e1z = b.y; t.e2x = b.z; t.e2y = b.w;
+    t.e2z = c.x;
+    return t;
+}
+
+// Casts one charged ray through the grid.  Returns the blocked flag and
+// leaves kernel1Dv3's `dist` in `dist` (closest-hit mode).
+template <bool ANY>
+LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
+                      const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
+                      unsigned int *n_tests = nullptr, unsigned int *n_cells = nullptr)
+{
+    const float INF = __builtin_huge_valf();
+    // clip the segment [0, len] to the grid box
+    float hx = G.lox + G.cx * G.nx, hy = G.loy + G.cy * G.ny, hz = G.loz + G.cz * G.nz;
+    float idx = 1.0f / r.dx, idy = 1.0f / r.dy, idz = 1.0f / r.dz;
+    float t0 = 0.0f, t1 = r.len;
+    {
+        float a = (G.lox - r.ox) * idx, b = (hx - r.ox) * idx;
+        if (r.dx == 0.0f) {
+            if (r.ox < G.lox || r.ox > hx) return false;
+        } else {
+            t0 = fmaxf(t0, fminf(a, b));
+            t1 = fminf(t1, fmaxf(a, b));
+        }
+        a = (G.loy - r.oy) * idy; b = (hy - r.oy) * idy;
+        if (r.dy == 0.0f) {
+            if (r.oy < G.loy || r.oy > hy) return false;
+        } else {
+            t0 = fmaxf(t0, fminf(a, b));
+            t1 = fminf(t1, fmaxf(a, b));
+        }
+        a = (G.loz - r.oz) * idz; b = (hz - r.oz) * idz;
+        if (r.dz == 0.0f) {
+            if (r.oz < G.loz || r.oz > hz) return false;
+        } else {
+            t0 = fmaxf(t0, fminf(a, b));
+            t1 = fminf(t1, fmaxf(a, b));
+        }
+    }
+    if (!(t0 <= t1 + G.eps))
+        return false;
+    float pm = t0 > 0.0f ? t0 : 0.0f;
+    int ix = clampi((int)floorf((r.ox + r.dx * pm - G.lox) * G.icx), 0, G.nx - 1);
+    int iy = clampi((int)floorf((r.oy + r.dy * pm - G.loy) * G.icy), 0, G.ny - 1);
+    int iz = clampi((int)floorf((r.oz + r.dz * pm - G.loz) * G.icz), 0, G.nz - 1);
+    const int sx = r.dx > 0.0f ? 1 : (r.dx < 0.0f ? -1 : 0);
+    const int sy = r.dy > 0.0f ? 1 : (r.dy < 0.0f ? -1 : 0);
+    const int sz = r.dz > 0.0f ? 1 : (r.dz < 0.0f ? -1 : 0);
+    auto tnext = [&](int i, int s, float lo, float c, float o, float id) -> float {
+        if (s == 0)
+            return INF;
+        float bnd = lo + c * (float)(i + (s > 0 ? 1 : 0));
+        return (bnd - o) * id;
+    };
+    float tx = tnext(ix, sx, G.lox, G.cx, r.ox, idx);
+    float ty = tnext(iy, sy, G.loy, G.cy, r.oy, idy);
+    float tz = tnext(iz, sz, G.loz, G.cz, r.oz, idz);
+    const float margin = 2.0f * G.eps;
+    bool hit = false;
+    for (;;) {
+        int64_t c = ((int64_t)iz * G.ny + iy) * G.nx + ix;
+        int32_t b = cell_start[c], e = cell_start[c + 1];
+        if (n_cells) {
+            *n_cells += 1;
+            *n_tests += (unsigned int)(e - b);
+        }
+        for (int32_t j = b; j < e; ++j) {
+            Tri t = load_rec(recs, j);
+            double tt;
+            if (hit_numba(r, dist, t, tt) && tt < (double)dist) {
+                dist = (float)tt;
+                hit = true;
+                if (ANY)
+                    return true;
+            }
+        }
+        float texit = fminf(tx, fminf(ty, tz));
+        if (hit && dist + margin <= texit)
+            break;
+        if (texit >= t1 + margin)
+            break;
+        if (tx <= ty && tx <= tz) {
+            ix += sx;
+            if (ix < 0 || ix >= G.nx) break;
+            tx = tnext(ix, sx, G.lox, G.cx, r.ox, idx);
+        } else if (ty <= tz) {
+            iy += sy;
+            if (iy < 0 || iy >= G.ny) break;
+            ty = tnext(iy, sy, G.loy, G.cy, r.oy, idy);
+        } else {
+            iz += sz;
+            if (iz < 0 || iz >= G.nz) break;
+            tz = tnext(iz, sz, G.loz, G.cz, r.oz, idz);
+        }
+    }
+    return hit;
+}
+
+// Dense S x K through the grid.
+// SINK_MAJOR = false: lane = sink, block = (source, 256 sinks); bits S x W64(K).
+// SINK_MAJOR = true : lane = source, block = (sink, 256 sources); bits K x W64(S),
+//                     dist K x S.  Neighbouring sources to one sink give nearly
+//                     parallel rays, so a wavefront walks nearly the same cells.
+template <bool ANY, bool SINK_MAJOR>
+__global__ __launch_bounds__(GB) void k_visibility_grid(
+    const float *__restrict__ src, int64_t S, const float *__restrict__ snk, int64_t K, GridDev G,
+    const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
+    unsigned long long *__restrict__ bits, float *__restrict__ dist_out, int64_t nbl)
+{
+    const int64_t outer = (int64_t)blockIdx.x / nbl;
+    const int64_t inner = ((int64_t)blockIdx.x % nbl) * GB + threadIdx.x;
+    const int64_t NI = SINK_MAJOR ? S : K;
+    const bool active = inner < NI;
+    const int64_t s = SINK_MAJOR ? (active ? inner : 0) : outer;
+    const int64_t k = SINK_MAJOR ? outer : (active ? inner : 0);
+    Ray r = charge(src[3 * s], src[3 * s + 1], src[3 * s + 2], snk[3 * k], snk[3 * k + 1],
+                   snk[3 * k + 2]);
+    float dist = r.len;
+    bool hit = false;
+    if (active)
+        hit = dda_cast<ANY>(r, dist, G, cell_start, recs);
+    unsigned long long bw = __ballot(active && hit);
+    const int64_t W = (NI + 63) / 64;
+    if ((threadIdx.x & 63) == 0) {
+        int64_t word = inner >> 6;
+        if (word < W)
+            bits[outer * W + word] = bw;
+    }
+    if (dist_out && active)
+        dist_out[outer * NI + inner] = dist;
+}
+
+// Instrumented traversal: counts the Moller-Trumbore tests and cell visits a
+// traversal performs (the algorithmic work of the roofline), no outputs.
+template <bool ANY>
+__global__ __launch_bounds__(GB) void k_grid_stats(
+    const float *__restrict__ src, int64_t S, const float *__restrict__ snk, int64_t K, GridDev G,
+    const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
+    unsigned long long *__restrict__ stats, int64_t nbl)
+{
+    const int64_t k = (int64_t)blockIdx.x / nbl;
+    const int64_t s = ((int64_t)blockIdx.x % nbl) * GB + threadIdx.x;
+    unsigned int nt = 0, nc = 0;
+    bool hit = false;
+    if (s < S) {
+        Ray r = charge(src[3 * s], src[3 * s + 1], src[3 * s + 2], snk[3 * k], snk[3 * k + 1],
+                       snk[3 * k + 2]);
+        float dist = r.len;
+        hit = dda_cast<ANY>(r, dist, G, cell_start, recs, &nt, &nc);
+    }
+    // wave sums, one atomic per wave
+    unsigned long long t64 = nt, c64 = nc, h64 = hit ? 1ull : 0ull;
+    for (int off = 32; off > 0; off >>= 1) {
+        t64 += __shfl_down(t64, off);
+        c64 += __shfl_down(c64, off);
+        h64 += __shfl_down(h64, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], t64);
+        atomicAdd(&stats[1], c64);
+        atomicAdd(&stats[2], h64);
+    }
+}
+
+// Field source sum fused with the grid traversal: each (source, sink) ray is
+// cast any-hit through the grid and, when unblocked, its Green's-function term
+// is accumulated -- the mask is never stored.  Same partial layout as
+// k_field_los_partial (lem_fields.hip); merged by the same finish kernel.
+__global__ __launch_bounds__(FB) void k_field_los_grid_partial(
+    const float *__restrict__ src, const float *__restrict__ src_w, int64_t S,
+    const float *__restrict__ snk, const float *__restrict__ snk_n, int64_t K, GridDev G,
+    const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs, float alpha,
+    double beta, int64_t CH, int64_t nbk, float *__restrict__ ws_sum,
+    float *__restrict__ ws_cmp)
+{
+    __shared__ SrcLds s_src[FS_TILE];
+    const int64_t c = (int64_t)blockIdx.x / nbk;
+    const int64_t k = ((int64_t)blockIdx.x % nbk) * FB + threadIdx.x;
+    const bool active = k < K;
+    const int64_t kk = active ? k : 0;
+    const float xkf = snk[3 * kk], ykf = snk[3 * kk + 1], zkf = snk[3 * kk + 2];
+    const double xk = xkf, yk = ykf, zk = zkf;
+    const float nkx = snk_n[3 * kk], nky = snk_n[3 * kk + 1], nkz = snk_n[3 * kk + 2];
+    const int64_t s0 = c * CH;
+    const int64_t s1 = (s0 + CH) < S ? (s0 + CH) : S;
+    KahanC3 acc;
+    acc.zero();
+    for (int64_t base = s0; base < s1; base += FS_TILE) {
+        __syncthreads();
+        stage_sources(s_src, src, src_w, base, s1);
+        __syncthreads();
+        int n = (int)((s1 - base) < FS_TILE ? (s1 - base) : FS_TILE);
+        if (active) {
+            for (int j = 0; j < n; ++j) {
+                const SrcLds &L = s_src[j];
+                Ray r = charge((float)L.x, (float)L.y, (float)L.z, xkf, ykf, zkf);
+                float dist = r.len;
+                if (dda_cast<true>(r, dist, G, cell_start, recs))
+                    continue;
+                C3 w;
+                w.xr = L.w[0]; w.xi = L.w[1]; w.yr = L.w[2]; w.yi = L.w[3]; w.zr = L.w[4];
+                w.zi = L.w[5];
+                C3 t = los_term(L.x, L.y, L.z, w, xk, yk, zk, nkx, nky, nkz, alpha, beta);
+                float v[6] = {t.xr, t.xi, t.yr, t.yi, t.zr, t.zi};
+                acc.add(v);
+            }
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            ws_sum[(c * 6 + i) * K + k] = acc.s[i];
+            ws_cmp[(c * 6 + i) * K + k] = acc.c[i];
+        }
+    }
+}
+
+// Cell size from the scene extent and a target cell count: cubic cells over
+// the axes thicker than a cell, a single layer along thinner axes.
+void choose_resolution(const float ext[3], double target_cells, int64_t cap, int n[3])
+{
+    bool thin[3] = {false, false, false};
+    double h = 0.0;
+    for (int it = 0; it < 3; ++it) {
+        double vol = 1.0;
+        int dims = 0;
+        for (int a = 0; a < 3; ++a)
+            if (!thin[a]) {
+                vol *= std::max((double)ext[a], 1e-30);
+                ++dims;
+            }
+        if (dims == 0) {
+            h = 1.0;
+            break;
+        }
+        h = pow(vol / std::max(target_cells, 1.0), 1.0 / dims);
+        bool changed = false;
+        for (int a = 0; a < 3; ++a)
+            if (!thin[a] && ext[a] < h) {
+                thin[a] = true;
+                changed = true;
+            }
+        if (!changed)
+            break;
+    }
+    for (;;) {
+        int64_t tot = 1;
+        for (int a = 0; a < 3; ++a) {
+            n[a] = thin[a] ? 1 : (int)std::max(1.0, std::min(1e6, ceil(ext[a] / h)));
+            tot *= n[a];
+        }
+        if (tot <= cap)
+            break;
+        h *= 1.1;
+    }
+}
+
+struct PlanWs {
+    unsigned int *bounds;  // 6
+    long long *total;      // 1
+    int *counts;           // cap + 1
+    long long *scan;       // cap + 1
+    long long *bsum;       // ceil((cap + 1) / GB)
+};
+
+inline int64_t plan_cap(int64_t T, float cells_per_tri)
+{
+    double c = (double)T * (double)(cells_per_tri > 0 ? cells_per_tri : 2.0f) + 64.0;
+    if (c > 1.0e9)
+        c = 1.0e9;
+    return (int64_t)c;
+}
+
+inline PlanWs plan_layout(void *ws, int64_t cap)
+{
+    char *p = reinterpret_cast<char *>(ws);
+    PlanWs L;
+    L.bounds = reinterpret_cast<unsigned int *>(p);
+    L.total = reinterpret_cast<long long *>(p + 64);
+    L.counts = reinterpret_cast<int *>(p + 128);
+    int64_t off = 128 + ((4 * (cap + 1) + 255) / 256) * 256;
+    L.scan = reinterpret_cast<long long *>(p + off);
+    off += ((8 * (cap + 1) + 255) / 256) * 256;
+    L.bsum = reinterpret_cast<long long *>(p + off);
+    return L;
+}
+
+inline int64_t plan_bytes(int64_t cap)
+{
+    int64_t off = 128 + ((4 * (cap + 1) + 255) / 256) * 256;
+    off += ((8 * (cap + 1) + 255) / 256) * 256;
+    off += 8 * (ceil_div(cap + 1, GB) + 1);
+    return off;
+}
+
+__global__ void k_zero_int(int *__restrict__ p, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (i < n)
+        p[i] = 0;
+}
+
+}  // namespace lem
+
+using namespace lem;
+
+extern "C" int64_t lem_grid_workspace(int64_t T, float cells_per_tri)
+{
+    if (T < 0)
+        return 0;
+    return plan_bytes(plan_cap(T, cells_per_tri));
+}
+
+extern "C" int lem_grid_plan(const float *tris, int64_t T, float cells_per_tri,
+                             lem_grid_info *info, void *workspace, int64_t workspace_bytes,
+                             void *stream)
+{
+    clear_error();
+    LEM_REQUIRE(T > 0 && tris && info, "lem_grid_plan: needs a non-empty triangle table");
+    LEM_REQUIRE(T <= 2000000000LL, "lem_grid_plan: too many triangles");
+    const int64_t cap = plan_cap(T, cells_per_tri);
+    if (!workspace || workspace_bytes < plan_bytes(cap)) {
+        set_error("lem_grid_plan: workspace smaller than lem_grid_workspace()");
+        return E_WORKSPACE;
+    }
+    hipStream_t st = as_stream(stream);
+    PlanWs L = plan_layout(workspace, cap);
+    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, st, L.bounds);
+    int64_t nvals = 9 * T;
+    int64_t nblk = std::min<int64_t>(ceil_div(3 * T, GB), 4096);
+    hipLaunchKernelGGL(k_bounds, dim3((unsigned)nblk), dim3(GB), 0, st, tris, nvals, L.bounds);
+    int rc = check_launch("k_bounds");
+    if (rc)
+        return rc;
+    unsigned int hb[6];
+    if (hipMemcpyAsync(hb, L.bounds, sizeof(hb), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_error("lem_grid_plan: bounds readback failed");
+        return E_HIP;
+    }
+    float lo[3], hi[3], ext[3];
+    float span = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = fkey_inv(hb[a]);
+        hi[a] = fkey_inv(hb[3 + a]);
+        LEM_REQUIRE(std::isfinite(lo[a]) && std::isfinite(hi[a]),
+                    "lem_grid_plan: non-finite vertex coordinates");
+        span = std::max(span, std::max(fabsf(lo[a]), fabsf(hi[a])));
+        span = std::max(span, hi[a] - lo[a]);
+    }
+    // Inflation: far above the float32 error of cell stepping at this scale.
+    float eps = std::max(1e-4f, 2e-6f * span);
+    for (int a = 0; a < 3; ++a) {
+        lo[a] -= 2 * eps;
+        hi[a] += 2 * eps;
+        ext[a] = hi[a] - lo[a];
+    }
+    int n[3];
+    choose_resolution(ext, (double)T * (cells_per_tri > 0 ? cells_per_tri : 2.0f), cap, n);
+    lem_grid_info g;
+    std::memset(&g, 0, sizeof(g));
+    for (int a = 0; a < 3; ++a) {
+        g.lo[a] = lo[a];
+        g.n[a] = n[a];
+        g.cell[a] = ext[a] / (float)n[a];
+    }
+    g.ncells = (int64_t)n[0] * n[1] * n[2];
+    g.eps = eps;
+    GridDev G = to_dev(g);
+    hipLaunchKernelGGL(k_zero_int, dim3((unsigned)ceil_div(g.ncells + 1, GB)), dim3(GB), 0, st,
+                       L.counts, g.ncells + 1);
+    hipLaunchKernelGGL(k_count, dim3((unsigned)ceil_div(T, GB)), dim3(GB), 0, st, tris, T, G,
+                       L.counts);
+    rc = check_launch("k_count");
+    if (rc)
+        return rc;
+    const int64_t nsb = ceil_div(g.ncells, GB);
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nsb), dim3(GB), 0, st, L.counts, g.ncells,
+                       L.scan, L.bsum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, L.bsum, nsb, L.total);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nsb), dim3(GB), 0, st, L.scan, g.ncells, L.bsum);
+    rc = check_launch("grid scan");
+    if (rc)
+        return rc;
+    long long total = 0;
+    if (hipMemcpyAsync(&total, L.total, sizeof(total), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_error("lem_grid_plan: total readback failed");
+        return E_HIP;
+    }
+    LEM_REQUIRE(total < 2147483647LL, "lem_grid_plan: more than 2^31 triangle references");
+    g.nrefs = total;
+    *info = g;
+    return OK;
+}
+
+extern "C" int lem_grid_fill(const float *tris, int64_t T, const lem_grid_info *info,
+                             int32_t *cell_start, int32_t *tri_ids, float *recs, void *workspace,
+                             int64_t workspace_bytes, float cells_per_tri, void *stream)
+{
+    clear_error();
+    LEM_REQUIRE(T > 0 && tris && info && cell_start && tri_ids && recs,
+                "lem_grid_fill: null argument");
+    const int64_t cap = plan_cap(T, cells_per_tri);
+    LEM_REQUIRE(info->ncells <= cap, "lem_grid_fill: info does not match the workspace");
+    if (!workspace || workspace_bytes < plan_bytes(cap)) {
+        set_error("lem_grid_fill: workspace smaller than lem_grid_workspace()");
+        return E_WORKSPACE;
+    }
+    hipStream_t st = as_stream(stream);
+    PlanWs L = plan_layout(workspace, cap);
+    GridDev G = to_dev(*info);
+    const int64_t nc = info->ncells;
+    // counts is reused as the insertion cursor
+    hipLaunchKernelGGL(k_starts, dim3((unsigned)ceil_div(nc + 1, GB)), dim3(GB), 0, st, L.scan, nc,
+                       (long long)info->nrefs, cell_start, L.counts);
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)ceil_div(T, GB)), dim3(GB), 0, st, tris, T, G,
+                       L.counts, tri_ids);
+    hipLaunchKernelGGL(k_sort_cells, dim3((unsigned)ceil_div(nc, GB)), dim3(GB), 0, st, cell_start,
+                       nc, tri_ids);
+    if (info->nrefs > 0)
+        hipLaunchKernelGGL(k_gather, dim3((unsigned)ceil_div(info->nrefs, GB)), dim3(GB), 0, st,
+                           tris, tri_ids, info->nrefs, reinterpret_cast<float4 *>(recs));
+    return check_launch("lem_grid_fill");
+}
+
+extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk, int64_t K,
+                                   const lem_grid_info *info, const int32_t *cell_start,
+                                   const float *recs, int flags, uint64_t *hit_bits, float *dist,
+                                   void *stream)
+{
+    clear_error();
+    LEM_REQUIRE(S >= 0 && K >= 0, "lem_visibility_grid: negative size");
+    if (S == 0 || K == 0)
+        return OK;
+    LEM_REQUIRE(src && snk && info && cell_start && recs && hit_bits,
+                "lem_visibility_grid: null pointer");
+    LEM_REQUIRE(info->n[0] > 0 && info->n[1] > 0 && info->n[2] > 0, "lem_visibility_grid: bad grid");
+    LEM_REQUIRE(!(flags & LEM_VIS_SKIP_SELF), "lem_visibility_grid: skip-self is brute-force only");
+    const bool any = (flags & LEM_VIS_ANY_HIT) != 0;
+    const bool sink_major = (flags & LEM_VIS_SINK_MAJOR) != 0;
+    LEM_REQUIRE(!(any && dist), "lem_visibility_grid: dist needs the closest-hit mode");
+    const int64_t NI = sink_major ? S : K, NO = sink_major ? K : S;
+    const int64_t nbl = ceil_div(NI, GB);
+    LEM_REQUIRE(NO * nbl <= 2147483647LL, "lem_visibility_grid: S*K too large for one launch");
+    GridDev G = to_dev(*info);
+    hipStream_t st = as_stream(stream);
+    dim3 grid((unsigned)(NO * nbl));
+    unsigned long long *hb = reinterpret_cast<unsigned long long *>(hit_bits);
+    const float4 *rc4 = reinterpret_cast<const float4 *>(recs);
+    if (any) {
+        if (sink_major)
+            hipLaunchKernelGGL((k_visibility_grid<true, true>), grid, dim3(GB), 0, st, src, S, snk,
+                               K, G, cell_start, rc4, hb, dist, nbl);
+        else
+            hipLaunchKernelGGL((k_visibility_grid<true, false>), grid, dim3(GB), 0, st, src, S,
+                               snk, K, G, cell_start, rc4, hb, dist, nbl);
+    } else {
+        if (sink_major)
+            hipLaunchKernelGGL((k_visibility_grid<false, true>), grid, dim3(GB), 0, st, src, S,
+                               snk, K, G, cell_start, rc4, hb, dist, nbl);
+        else
+            hipLaunchKernelGGL((k_visibility_grid<false, false>), grid, dim3(GB), 0, st, src, S,
+                               snk, K, G, cell_start, rc4, hb, dist, nbl);
+    }
+    return check_launch("k_visibility_grid");
+}
+
+extern "C" int lem_visibility_grid_stats(const float *src, int64_t S, const float *snk, int64_t K,
+                                         const lem_grid_info *info, const int32_t *cell_start,
+                                         const float *recs, int flags, uint64_t *stats,
+                                         void *stream)
+{
+    clear_error();
+    LEM_REQUIRE(S > 0 && K > 0 && src && snk && info && cell_start && recs && stats,
+                "lem_visibility_grid_stats: bad argument");
+    const int64_t nbl = ceil_div(S, GB);
+    LEM_REQUIRE(K * nbl <= 2147483647LL, "lem_visibility_grid_stats: S*K too large");
+    GridDev G = to_dev(*info);
+    hipStream_t st = as_stream(stream);
+    unsigned long long *sp = reinterpret_cast<unsigned long long *>(stats);
+    const float4 *rc4 = reinterpret_cast<const float4 *>(recs);
+    if (flags & LEM_VIS_ANY_HIT)
+        hipLaunchKernelGGL((k_grid_stats<true>), dim3((unsigned)(K * nbl)), dim3(GB), 0, st, src,
+                           S, snk, K, G, cell_start, rc4, sp, nbl);
+    else
+        hipLaunchKernelGGL((k_grid_stats<false>), dim3((unsigned)(K * nbl)), dim3(GB), 0, st, src,
+                           S, snk, K, G, cell_start, rc4, sp, nbl);
+    return check_launch("k_grid_stats");
+}
+
+extern "C" int lem_field_los_grid(const float *src, const float *src_w, int64_t S,
+                                  const float *snk, const float *snk_n, const float *snk_w,
+                                  int64_t K, const lem_grid_info *info, const int32_t *cell_start,
+                                  const float *recs, double alpha, double beta, int flags,
+                                  float *out, void *workspace, int64_t workspace_bytes,
+                                  void *stream)
+{
+    clear_error();
+    LEM_REQUIRE(S > 0 && K > 0, "lem_field_los_grid: empty source or sink set");
+    LEM_REQUIRE(src && src_w && snk && snk_n && info && cell_start && recs && out,
+                "lem_field_los_grid: null pointer");
+    LEM_REQUIRE(!(flags & LEM_FIELD_ELEMENTS), "lem_field_los_grid: source sum only");
+    const int64_t NC = los_chunk_count(S, K);
+    const int64_t need = 2 * NC * 6 * K * (int64_t)sizeof(float);
+    if (!workspace || workspace_bytes < need) {
+        set_error("lem_field_los_grid: workspace smaller than lem_field_los_workspace()");
+        return E_WORKSPACE;
+    }
+    const int64_t CH = ceil_div(S, NC);
+    const int64_t nbk = ceil_div(K, FB);
+    LEM_REQUIRE(NC * nbk <= 2147483647LL, "lem_field_los_grid: grid too large");
+    float *ws_sum = reinterpret_cast<float *>(workspace);
+    float *ws_cmp = ws_sum + NC * 6 * K;
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(k_field_los_grid_partial, dim3((unsigned)(NC * nbk)), dim3(FB), 0, st, src,
+                       src_w, S, snk, snk_n, K, to_dev(*info), cell_start,
+                       reinterpret_cast<const float4 *>(recs), (float)alpha, beta, CH, nbk, ws_sum,
+                       ws_cmp);
+    int rc = check_launch("k_field_los_grid_partial");
+    if (rc)
+        return rc;
+    return launch_los_finish(ws_sum, ws_cmp, NC, K, snk_n, snk_w,
+                             (flags & LEM_FIELD_SINK_TANGENT) ? 1 : 0, out, st);
+}

@@ -25,6 +25,12 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Source-sum machinery of lem_fields.hip, shared with the grid-fused kernel.
+int64_t los_chunk_count(int64_t S, int64_t K);
+int launch_los_finish(const float *ws_sum, const float *ws_cmp, int64_t NC, int64_t K,
+                      const float *snk_n, const float *snk_w, int tangent, float *out,
+                      hipStream_t st);
+
 }  // namespace lem
 
 #define LEM_REQUIRE(cond, msg)            \

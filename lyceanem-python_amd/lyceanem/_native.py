@@ -20,6 +20,7 @@ SEM_NUMBA = 0
 SEM_NATIVE = 1
 VIS_ANY_HIT = 1
 VIS_SKIP_SELF = 2
+VIS_SINK_MAJOR = 4
 FIELD_SINK_TANGENT = 1
 FIELD_ELEMENTS = 2
 
@@ -39,7 +40,25 @@ SIGNATURES = {
                       _INT),
     "lem_field_paths": ([_P, _P, _P, _I64, _P, _INT, _P, _P, _P, _I64, _D, _D, _P, _P, _P],
                         _INT),
+    "lem_grid_workspace": ([_I64, ctypes.c_float], _I64),
+    "lem_grid_plan": ([_P, _I64, ctypes.c_float, _P, _P, _I64, _P], _INT),
+    "lem_grid_fill": ([_P, _I64, _P, _P, _P, _P, _P, _I64, ctypes.c_float, _P], _INT),
+    "lem_visibility_grid": ([_P, _I64, _P, _I64, _P, _P, _P, _INT, _P, _P, _P], _INT),
+    "lem_visibility_grid_stats": ([_P, _I64, _P, _I64, _P, _P, _P, _INT, _P, _P], _INT),
+    "lem_field_los_grid": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _P, _P, _D, _D, _INT, _P, _P,
+                            _I64, _P], _INT),
+    "lem_field_los_adjoint_workspace": ([_I64, _I64], _I64),
+    "lem_field_los_adjoint": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _D, _D, _INT, _P, _P, _P, _P,
+                               _I64, _P], _INT),
 }
+
+
+class GridInfo(ctypes.Structure):
+    """lem_grid_info (include/lem.h)."""
+    _fields_ = [("lo", ctypes.c_float * 3), ("cell", ctypes.c_float * 3),
+                ("n", ctypes.c_int32 * 3), ("pad", ctypes.c_int32),
+                ("ncells", ctypes.c_int64), ("nrefs", ctypes.c_int64),
+                ("eps", ctypes.c_float), ("pad2", ctypes.c_float)]
 
 _lock = threading.Lock()
 _lib = None
@@ -184,6 +203,33 @@ def field_los(src, src_w, snk, snk_n, hit_bits=None, snk_w=None, alpha=0.0, beta
     return out
 
 
+def field_los_adjoint(src, src_w, snk, snk_n, grad_out, hit_bits=None, snk_w=None, alpha=0.0,
+                      beta=2 * np.pi, sink_tangent=False, want_w=True, want_p=True):
+    """Adjoint of field_los (source sum): (grad_w (S,3) complex64, grad_p (S,3) float32)."""
+    torch = _torch()
+    lib = load_library()
+    src = to_dev(src, torch.float32).reshape(-1, 3)
+    snk = to_dev(snk, torch.float32).reshape(-1, 3)
+    snk_n = to_dev(snk_n, torch.float32).reshape(-1, 3)
+    S, K = src.shape[0], snk.shape[0]
+    src_w = to_dev(src_w, torch.complex64).reshape(S, 3)
+    g = to_dev(grad_out, torch.complex64).reshape(K, 3)
+    if snk_w is not None:
+        snk_w = to_dev(snk_w, torch.complex64).reshape(K, 3)
+    if hit_bits is not None:
+        hit_bits = to_dev(hit_bits, torch.int64)
+    gw = torch.empty((S, 3), dtype=torch.complex64, device=src.device) if want_w else None
+    gp = torch.empty((S, 3), dtype=torch.float32, device=src.device) if want_p else None
+    wsb = int(lib.lem_field_los_adjoint_workspace(S, K))
+    ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=src.device)
+    flags = FIELD_SINK_TANGENT if sink_tangent else 0
+    check(lib.lem_field_los_adjoint(ptr(src), ptr(src_w), S, ptr(snk), ptr(snk_n), ptr(snk_w), K,
+                                    ptr(hit_bits), float(alpha), float(beta), flags, ptr(g),
+                                    ptr(gw), ptr(gp), ptr(ws), wsb, stream_ptr()),
+          "lem_field_los_adjoint")
+    return gw, gp
+
+
 def field_paths(pos, nrm, w, index, order, seg_start, seg_slot, S, K, alpha, beta):
     """Network of an explicit path list -> (S,K,3) complex128 tensor."""
     torch = _torch()
@@ -205,6 +251,89 @@ def field_paths(pos, nrm, w, index, order, seg_start, seg_slot, S, K, alpha, bet
     if int(err.item()) != 0:
         raise LemError("lem_field_paths: path index points outside the point table")
     return out
+
+
+class Grid:
+    """Device uniform grid over a triangle table (lem_grid_plan + lem_grid_fill)."""
+
+    def __init__(self, tris, cells_per_tri=2.0):
+        torch = _torch()
+        lib = load_library()
+        self.tris = to_dev(tris, torch.float32).reshape(-1, 9)
+        T = self.tris.shape[0]
+        if T == 0:
+            raise LemError("Grid needs at least one triangle")
+        self.cells_per_tri = float(cells_per_tri)
+        wsb = int(lib.lem_grid_workspace(T, self.cells_per_tri))
+        self.ws = torch.empty(wsb, dtype=torch.uint8, device=self.tris.device)
+        self.info = GridInfo()
+        check(lib.lem_grid_plan(ptr(self.tris), T, self.cells_per_tri, ctypes.byref(self.info),
+                                ptr(self.ws), wsb, stream_ptr()), "lem_grid_plan")
+        dev = self.tris.device
+        self.cell_start = torch.empty(self.info.ncells + 1, dtype=torch.int32, device=dev)
+        self.tri_ids = torch.empty(max(self.info.nrefs, 1), dtype=torch.int32, device=dev)
+        self.recs = torch.empty((max(self.info.nrefs, 1), 12), dtype=torch.float32, device=dev)
+        check(lib.lem_grid_fill(ptr(self.tris), T, ctypes.byref(self.info), ptr(self.cell_start),
+                                ptr(self.tri_ids), ptr(self.recs), ptr(self.ws), wsb,
+                                self.cells_per_tri, stream_ptr()), "lem_grid_fill")
+        self.ws = None
+
+    def visibility(self, src, snk, any_hit=True, want_dist=False, sink_major=False):
+        """Hit bits (S x W64(K), or K x W64(S) sink-major) and optional dist."""
+        torch = _torch()
+        lib = load_library()
+        src = to_dev(src, torch.float32).reshape(-1, 3)
+        snk = to_dev(snk, torch.float32).reshape(-1, 3)
+        S, K = src.shape[0], snk.shape[0]
+        rows, cols = (K, S) if sink_major else (S, K)
+        bits = torch.zeros((rows, (cols + 63) // 64), dtype=torch.int64, device=src.device)
+        dist = torch.empty((rows, cols), dtype=torch.float32, device=src.device) \
+            if want_dist else None
+        flags = (VIS_ANY_HIT if any_hit else 0) | (VIS_SINK_MAJOR if sink_major else 0)
+        check(lib.lem_visibility_grid(ptr(src), S, ptr(snk), K, ctypes.byref(self.info),
+                                      ptr(self.cell_start), ptr(self.recs), flags, ptr(bits),
+                                      ptr(dist), stream_ptr()), "lem_visibility_grid")
+        return bits, dist
+
+
+def field_los_grid(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, beta=2 * np.pi,
+                   sink_tangent=False):
+    """Source-summed line-of-sight field with visibility cast through `grid`
+    inside the field kernel: (K,3) complex64."""
+    torch = _torch()
+    lib = load_library()
+    src = to_dev(src, torch.float32).reshape(-1, 3)
+    snk = to_dev(snk, torch.float32).reshape(-1, 3)
+    snk_n = to_dev(snk_n, torch.float32).reshape(-1, 3)
+    S, K = src.shape[0], snk.shape[0]
+    out = torch.zeros((K, 3), dtype=torch.complex64, device=snk.device)
+    if S == 0 or K == 0:
+        return out
+    src_w = to_dev(src_w, torch.complex64).reshape(S, 3)
+    if snk_w is not None:
+        snk_w = to_dev(snk_w, torch.complex64).reshape(K, 3)
+    wsb = int(lib.lem_field_los_workspace(S, K, 0))
+    ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=snk.device)
+    flags = FIELD_SINK_TANGENT if sink_tangent else 0
+    check(lib.lem_field_los_grid(ptr(src), ptr(src_w), S, ptr(snk), ptr(snk_n), ptr(snk_w), K,
+                                 ctypes.byref(grid.info), ptr(grid.cell_start), ptr(grid.recs),
+                                 float(alpha), float(beta), flags, ptr(out), ptr(ws), wsb,
+                                 stream_ptr()), "lem_field_los_grid")
+    return out
+
+
+def grid_stats(grid, src, snk, any_hit=False):
+    """(MT tests, cells visited, blocked rays) of a sink-major grid traversal."""
+    torch = _torch()
+    lib = load_library()
+    src = to_dev(src, torch.float32).reshape(-1, 3)
+    snk = to_dev(snk, torch.float32).reshape(-1, 3)
+    stats = torch.zeros(3, dtype=torch.int64, device=src.device)
+    check(lib.lem_visibility_grid_stats(ptr(src), src.shape[0], ptr(snk), snk.shape[0],
+                                        ctypes.byref(grid.info), ptr(grid.cell_start),
+                                        ptr(grid.recs), VIS_ANY_HIT if any_hit else 0,
+                                        ptr(stats), stream_ptr()), "lem_visibility_grid_stats")
+    return tuple(int(x) for x in stats.cpu().numpy())
 
 
 def unpack_bits(bits, K):

@@ -6,8 +6,11 @@ reference signatures and return values (lyceanem/models/frequency_domain.py).
 Line of sight (scattering=0) runs as two fused device passes: a dense
 source x sink visibility bitmask (`lem_visibility_dense`) and the masked field
 kernel (`lem_field_los`), so the S*K ray index and payload the reference builds
-on the host never exist.  One bounce (scattering=1) follows the reference's
-path list: workchunkingv2 -> EMGPUFreqDomain, both on the device kernels.
+on the host never exist.  One bounce (scattering=1) summed over sources is
+evaluated leg by leg (source -> scatterers, scatterers -> sinks, plus line of
+sight), which equals the reference's per-path sum by linearity; the per-source
+network (elements=True) follows the reference's path list (workchunkingv2 ->
+EMGPUFreqDomain), both on the device kernels.
 """
 from __future__ import annotations
 
@@ -109,15 +112,80 @@ def calculate_scattering(aperture_coords, sink_coords, antenna_solid, desired_E_
     scat_n = np.asarray(scatter_points.point_data["Normals"]).astype(np.float32)
     P = scat.shape[0]
     w = _scatter_branch_weights(aperture_coords, desired_E_axis, project_vectors, S)
+    if not elements:
+        E = one_bounce_fields(src, w, snk, snk_n, scat, scat_n, tris9, alpha=alpha, beta=beta,
+                              los=los)
+        return E[:, 0], E[:, 1], E[:, 2]
+    # per-source network: the reference's path list (S x P x K rows at most)
     src_n = np.asarray(aperture_coords.point_data["Normals"]).astype(np.float32)
     table = _point_table([src, snk, scat], [src_n, snk_n, scat_n],
                          [w, np.ones((K, 3)), np.ones((P, 3))])
     full_index, _ = RF.workchunkingv2(src, snk, scat, tris9, scattering + 1, line_of_sight=los)
     net = EM.EMGPUFreqDomain(S, K, full_index, table, wavelength, alpha, beta)
-    if elements:
-        return net[:, :, 0], net[:, :, 1], net[:, :, 2]
-    ones = np.ones(S)
-    return np.dot(ones, net[:, :, 0]), np.dot(ones, net[:, :, 1]), np.dot(ones, net[:, :, 2])
+    return net[:, :, 0], net[:, :, 1], net[:, :, 2]
+
+
+# Above this many triangles visibility is cast through a uniform grid (DDA)
+# instead of testing every triangle; both give the same masks.
+GRID_MIN_TRIANGLES = 4096
+
+
+def _visibility_bits(src, snk, tris9, grid):
+    if grid is not None:
+        bits, _ = grid.visibility(src, snk, any_hit=True)
+    else:
+        bits, _ = _native.visibility_dense(src, snk, tris9, any_hit=True)
+    return bits
+
+
+def one_bounce_fields(src, src_w, snk, snk_n, scat, scat_n, tris9, scat_w=None, alpha=0.0,
+                      beta=2 * np.pi, los=True, grid=None):
+    """Source-summed fields of every line-of-sight and one-bounce path, as
+    calculate_scattering(scattering=1) gets from workchunkingv2 (max_scatter 2)
+    and EMGPUFreqDomain, evaluated leg by leg:
+
+        H_p = P(n_p) sum_s vis(s,p) L_sp P(d_sp) w_s * w_p      (leg 1, at scatterers)
+        E_k = P(n_k) sum_s vis(s,k) L_sk P(d_sk) w_s            (line of sight)
+            + sum_p vis(p,k) L_pk P(d_pk) H_p                   (leg 2)
+
+    P(n) = I - n n^T; the line-of-sight rows of a 3-column path index are
+    projected onto the sink tangent plane (empropagation.py:1150-1161).
+    scat_w: per-scatterer weight (the reference fixes it to 1,
+    frequency_domain.py:513-515).  As in workchunkingv2's unchunked branch
+    (rayfunctions.py:2835-2861), no visible scatterer means no paths at all.
+    Returns (K, 3) complex128."""
+    tris9 = np.asarray(tris9, np.float32).reshape(-1, 9)
+    T = tris9.shape[0]
+    if grid is None and T >= GRID_MIN_TRIANGLES:
+        grid = _native.Grid(tris9)
+    P = len(scat)
+    K = len(snk)
+    if scat_w is None:
+        scat_w = np.ones((P, 3), np.complex64)
+    if P == 0:
+        return np.zeros((K, 3), np.complex128)
+    if grid is not None:
+        H = _native.field_los_grid(grid, src, src_w, scat, scat_n, snk_w=scat_w, alpha=alpha,
+                                   beta=beta, sink_tangent=True)
+    else:
+        bits = _visibility_bits(src, scat, tris9, None)
+        H = _native.field_los(src, src_w, scat, scat_n, hit_bits=bits, snk_w=scat_w,
+                              alpha=alpha, beta=beta, sink_tangent=True)
+    if not bool(H.abs().sum().item() > 0):
+        return np.zeros((K, 3), np.complex128)
+    if grid is not None:
+        E = _native.field_los_grid(grid, scat, H, snk, snk_n, alpha=alpha, beta=beta)
+        if los:
+            E = E + _native.field_los_grid(grid, src, src_w, snk, snk_n, alpha=alpha, beta=beta,
+                                           sink_tangent=True)
+    else:
+        bits = _visibility_bits(scat, snk, tris9, None)
+        E = _native.field_los(scat, H, snk, snk_n, hit_bits=bits, alpha=alpha, beta=beta)
+        if los:
+            bits = _visibility_bits(src, snk, tris9, None)
+            E = E + _native.field_los(src, src_w, snk, snk_n, hit_bits=bits, alpha=alpha,
+                                      beta=beta, sink_tangent=True)
+    return E.cpu().numpy().astype(np.complex128)
 
 
 def calculate_farfield(aperture_coords, antenna_solid, desired_E_axis, az_range=None,

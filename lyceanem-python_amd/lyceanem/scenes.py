@@ -83,13 +83,14 @@ def terrain_surface_height(h, x, y, dx=1.0):
 
 
 def terrain_c2(nqx=1000, nqy=500, size_x=1000.0, size_y=500.0, n_src_side=1024, n_snk=16384,
-               src_alt=30.0, snk_alt=1.5, seed_h=2, seed_snk=3):
+               src_alt=30.0, snk_alt=1.5, seed_h=2, seed_snk=3, sigma_h=5.0, corr=50.0):
     """Config 2: height-field terrain of nqx x nqy quads (2 triangles each) over
     size_x x size_y metres; sources on an n_src_side^2 grid at src_alt above the
     surface, sinks uniform over the domain at snk_alt above the surface."""
     dx = size_x / nqx
     assert abs(size_y / nqy - dx) < 1e-9, "square quads expected"
-    h = terrain_heights(nqx + 1, nqy + 1, dx=dx, seed=seed_h) if seed_h is not None else \
+    h = terrain_heights(nqx + 1, nqy + 1, dx=dx, sigma_h=sigma_h, corr=corr, seed=seed_h) \
+        if seed_h is not None else \
         np.zeros((nqy + 1, nqx + 1))
     ny, nx = h.shape
     X, Y = np.meshgrid(np.arange(nx) * dx, np.arange(ny) * dx)
@@ -112,3 +113,104 @@ def terrain_c2(nqx=1000, nqy=500, size_x=1000.0, size_y=500.0, n_src_side=1024, 
     snk = np.stack([kx, ky, terrain_surface_height(h, kx, ky, dx) + snk_alt], 1)
     return dict(tris=tris, src=src.astype(np.float32), snk=snk.astype(np.float32), heights=h,
                 dx=dx)
+
+
+def clutter_c3(n_src_side=256, n_tris=50000, n_scat=200000, n_snk_side=64, freq=5.8e9,
+               seed_w=4, seed_mesh=5, seed_scat=6, reflection=0.5, offset=1e-4):
+    """Config 3: one-bounce scattering network at `freq`.
+
+    Sources: n x n array at lambda/2 in z=0 centred at the origin, normals +z,
+    x-polarised weights with seeded uniform random phase.  Mesh: n_tris random
+    triangles (vertices = centre + U[-1.5,1.5]^3 m) with centres uniform in
+    [-90,90] x [-90,90] x [20,200] m.  Scatterers: n_scat area-weighted uniform
+    samples on the triangles, carrying the unit face normal and lifted `offset`
+    metres along it, weight `reflection`.  Sinks: n x n grid in z=0 over
+    x in [20,200], y in [-90,90] m, normals pointing away from the array."""
+    lam = C0 / freq
+    pitch = lam / 2
+    c = (np.arange(n_src_side) - (n_src_side - 1) / 2) * pitch
+    xx, yy = np.meshgrid(c, c)
+    S = n_src_side * n_src_side
+    src = np.stack([xx.ravel(), yy.ravel(), np.zeros(S)], 1).astype(np.float32)
+    src_n = np.tile(np.array([[0, 0, 1]], np.float32), (S, 1))
+    rng = np.random.default_rng(seed_w)
+    w = np.zeros((S, 3), np.complex64)
+    w[:, 0] = np.exp(1j * rng.uniform(0, 2 * np.pi, S))
+    rng = np.random.default_rng(seed_mesh)
+    ctr = np.stack([rng.uniform(-90, 90, n_tris), rng.uniform(-90, 90, n_tris),
+                    rng.uniform(20, 200, n_tris)], 1)
+    v = ctr[:, None, :] + rng.uniform(-1.5, 1.5, (n_tris, 3, 3))
+    tris = v.reshape(-1, 9).astype(np.float32)
+    t64 = tris.astype(np.float64).reshape(-1, 3, 3)
+    cr = np.cross(t64[:, 1] - t64[:, 0], t64[:, 2] - t64[:, 0])
+    area = 0.5 * np.linalg.norm(cr, axis=1)
+    fn = cr / np.linalg.norm(cr, axis=1, keepdims=True)
+    rng = np.random.default_rng(seed_scat)
+    pick = rng.choice(n_tris, n_scat, p=area / area.sum())
+    r1 = np.sqrt(rng.uniform(0, 1, n_scat))
+    r2 = rng.uniform(0, 1, n_scat)
+    a, b, cc = t64[pick, 0], t64[pick, 1], t64[pick, 2]
+    pts = (1 - r1)[:, None] * a + (r1 * (1 - r2))[:, None] * b + (r1 * r2)[:, None] * cc
+    scat_n = fn[pick]
+    scat = (pts + offset * scat_n).astype(np.float32)
+    g = np.linspace(0, 1, n_snk_side)
+    kx, ky = np.meshgrid(20 + 180 * g, -90 + 180 * g)
+    K = n_snk_side * n_snk_side
+    snk = np.stack([kx.ravel(), ky.ravel(), np.zeros(K)], 1).astype(np.float32)
+    snk_n = (snk / np.linalg.norm(snk, axis=1, keepdims=True)).astype(np.float32)
+    return dict(src=src, src_n=src_n, w=w, tris=tris, scat=scat,
+                scat_n=scat_n.astype(np.float32),
+                scat_w=np.full((n_scat, 3), reflection, np.complex64), snk=snk, snk_n=snk_n,
+                wavelength=lam, alpha=0.0, beta=2 * np.pi / lam)
+
+
+def conformal_c4(n_az_el=64, n_z_el=64, radius=0.5, wavelength=0.03, n_el=181, n_az=361,
+                 far=1000.0, seed_w=7, standoff=1e-3, n_side=128, n_ring=64):
+    """Config 4: conformal array on a cylinder for beam synthesis.
+
+    Elements: n_az_el around the full circumference x n_z_el along z at lambda/2
+    pitch, `standoff` outside a cylinder of `radius`, radial normals,
+    z-polarised unit weights with seeded random phase.  Blocking mesh: the
+    cylinder side (n_side facets x n_ring rings, 2 triangles each, plus end caps)
+    and a radial fin at azimuth 90 deg (1 m out, 0.8 m tall); ~20k triangles.
+    Sinks: n_el x n_az (el-major, az-fastest) far-field grid at `far` metres."""
+    phi = 2 * np.pi * np.arange(n_az_el) / n_az_el
+    z = (np.arange(n_z_el) - (n_z_el - 1) / 2) * (wavelength / 2)
+    pg, zg = np.meshgrid(phi, z)
+    pg, zg = pg.ravel(), zg.ravel()
+    n = np.stack([np.cos(pg), np.sin(pg), np.zeros_like(pg)], 1)
+    pos = (n * (radius + standoff) + np.stack([0 * pg, 0 * pg, zg], 1)).astype(np.float32)
+    rng = np.random.default_rng(seed_w)
+    S = len(pg)
+    w = np.zeros((S, 3), np.complex64)
+    w[:, 2] = np.exp(1j * rng.uniform(0, 2 * np.pi, S))
+    h = n_z_el * wavelength / 2 * 1.25
+    a = 2 * np.pi * np.arange(n_side + 1) / n_side
+    zz = np.linspace(-h / 2, h / 2, n_ring + 1)
+    P = np.stack([radius * np.cos(a)[None, :].repeat(n_ring + 1, 0),
+                  radius * np.sin(a)[None, :].repeat(n_ring + 1, 0),
+                  zz[:, None].repeat(n_side + 1, 1)], -1)
+    q00, q10, q01, q11 = P[:-1, :-1], P[:-1, 1:], P[1:, :-1], P[1:, 1:]
+    side = np.concatenate([np.concatenate([q00, q10, q11], -1).reshape(-1, 9),
+                           np.concatenate([q00, q11, q01], -1).reshape(-1, 9)])
+    caps = []
+    for zc in (-h / 2, h / 2):
+        c = np.array([0.0, 0.0, zc])
+        for i in range(n_side):
+            p0 = np.array([radius * np.cos(a[i]), radius * np.sin(a[i]), zc])
+            p1 = np.array([radius * np.cos(a[i + 1]), radius * np.sin(a[i + 1]), zc])
+            caps.append(np.concatenate([c, p0, p1]))
+    fr = np.linspace(radius, radius + 1.0, 61)
+    fz = np.linspace(-0.4, 0.4, 29)
+    F = np.stack([0 * fr[None, :].repeat(29, 0), fr[None, :].repeat(29, 0),
+                  fz[:, None].repeat(61, 1)], -1)
+    f00, f10, f01, f11 = F[:-1, :-1], F[:-1, 1:], F[1:, :-1], F[1:, 1:]
+    fin = np.concatenate([np.concatenate([f00, f10, f11], -1).reshape(-1, 9),
+                          np.concatenate([f00, f11, f01], -1).reshape(-1, 9)])
+    tris = np.concatenate([side, np.asarray(caps), fin]).astype(np.float32)
+    snk, snk_n = sphere_sinks(n_el, n_az, far)
+    el = np.repeat(np.linspace(-90.0, 90.0, n_el), n_az)
+    az = np.tile(np.linspace(-180.0, 180.0, n_az), n_el)
+    in_beam = (np.abs(az - 30.0) <= 5.0) & (np.abs(el) <= 5.0)
+    return dict(src=pos, src_n=n.astype(np.float32), w=w, tris=tris, snk=snk, snk_n=snk_n,
+                in_beam=in_beam, wavelength=wavelength, alpha=0.0, beta=2 * np.pi / wavelength)

@@ -334,3 +334,28 @@ def exact_segment_triangle(o, t, tri9):
     v = dot(D, q) / A
     s = dot(e2, q) / A
     return min(u, v, 1 - u - v), s
+
+
+# --------------------------------------------------------------------------
+# float64 torch restatement of the line-of-sight field (dense reference for
+# gradients: autograd of this closed form, and finite differences of it)
+# --------------------------------------------------------------------------
+
+def field_los_torch64(w, pos, snk, snk_n, visible, alpha, beta):
+    """E_k = sum_s visible[s,k] F(r) (d.n_k) (I - d d^T) w_s in float64 / complex128
+    (lossy_propagation, empropagation.py:768-801, with the launch projector
+    of :178-249 in closed form).  w: (S,3) complex128, pos: (S,3) float64,
+    snk / snk_n: (K,3) float64, visible: (S,K) bool.  Differentiable in w, pos."""
+    import torch
+
+    D = snk[None, :, :] - pos[:, None, :]
+    r = torch.sqrt((D * D).sum(-1))
+    d = D / r[..., None]
+    cos = (d * snk_n[None]).sum(-1)
+    g = complex(alpha, beta)
+    G = torch.exp(-g * r) / r
+    F = -(1.0 / (2.0 * np.pi)) * (-g - 1.0 / r) * G
+    L = F * cos * visible.to(torch.float64)
+    dw = (d.to(torch.complex128) * w[:, None, :]).sum(-1)
+    Q = w[:, None, :] - d.to(torch.complex128) * dw[..., None]
+    return (L[..., None] * Q).sum(0)

@@ -1,0 +1,140 @@
+"""GPU tests of the hand-written adjoint (lem_field_los_adjoint) behind
+lyceanem.differentiable.FarField: against autograd of the float64 closed form
+(dense reference gradient) and float64 central finite differences (h = 1e-6)."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+GRAD_RTOL = 1e-3
+
+
+def _small_c4():
+    from lyceanem import scenes
+
+    return scenes.conformal_c4(n_az_el=16, n_z_el=8, n_el=19, n_az=37, n_side=32, n_ring=8)
+
+
+def _mask(native, sc):
+    bits, _ = native.visibility_dense(sc["src"], sc["snk"], sc["tris"], any_hit=True)
+    return bits, ~native.unpack_bits(bits, len(sc["snk"]))
+
+
+def _loss64(oracle, w, pos, sc, visible, in_beam):
+    from lyceanem.differentiable import db_mask_loss
+
+    E = oracle.field_los_torch64(w, pos, torch.from_numpy(sc["snk"].astype(np.float64)),
+                                 torch.from_numpy(sc["snk_n"].astype(np.float64)),
+                                 torch.from_numpy(visible), sc["alpha"], sc["beta"])
+    return db_mask_loss(E, torch.from_numpy(in_beam))
+
+
+def test_adjoint_vs_dense_reference_gradient(native, oracle):
+    from lyceanem.differentiable import far_field, db_mask_loss
+
+    sc = _small_c4()
+    bits, visible = _mask(native, sc)
+    assert 0.1 < visible.mean() < 0.9
+    dev = native.device()
+    w = torch.from_numpy(sc["w"]).to(dev).requires_grad_(True)
+    pos = torch.from_numpy(sc["src"]).to(dev).requires_grad_(True)
+    snk = torch.from_numpy(sc["snk"]).to(dev)
+    snk_n = torch.from_numpy(sc["snk_n"]).to(dev)
+    in_beam = torch.from_numpy(sc["in_beam"]).to(dev)
+    E = far_field(w, pos, snk, snk_n, hit_bits=bits, beta=sc["beta"])
+    loss = db_mask_loss(E, in_beam)
+    loss.backward()
+    w64 = torch.from_numpy(sc["w"].astype(np.complex128)).requires_grad_(True)
+    p64 = torch.from_numpy(sc["src"].astype(np.float64)).requires_grad_(True)
+    l64 = _loss64(oracle, w64, p64, sc, visible, sc["in_beam"])
+    l64.backward()
+    gw, gp = w.grad.cpu().numpy(), pos.grad.cpu().numpy()
+    rw = np.linalg.norm(gw - w64.grad.numpy()) / np.linalg.norm(w64.grad.numpy())
+    rp = np.linalg.norm(gp - p64.grad.numpy()) / np.linalg.norm(p64.grad.numpy())
+    print(f"adjoint vs dense float64 autograd: weights {rw:.2e}, positions {rp:.2e}", flush=True)
+    assert abs(loss.item() - l64.item()) < 1e-4 * abs(l64.item())
+    assert rw < GRAD_RTOL and rp < GRAD_RTOL
+
+
+def test_adjoint_vs_central_finite_differences(native, oracle):
+    """64 random parameters (real / imaginary weight parts and radial offsets),
+    float64 central differences with h = 1e-6."""
+    from lyceanem.differentiable import far_field, db_mask_loss
+
+    sc = _small_c4()
+    bits, visible = _mask(native, sc)
+    S = len(sc["src"])
+    nhat = torch.from_numpy(sc["src_n"].astype(np.float64))
+    p0 = torch.from_numpy(sc["src"].astype(np.float64))
+    w0 = torch.from_numpy(sc["w"].astype(np.complex128))
+    dev = native.device()
+    a = torch.from_numpy(sc["w"][:, 2].copy()).to(dev).requires_grad_(True)
+    delta = torch.zeros(S, dtype=torch.float32, device=dev, requires_grad=True)
+    zhat = torch.tensor([0, 0, 1], dtype=torch.complex64, device=dev)
+    pos = torch.from_numpy(sc["src"]).to(dev) + delta[:, None] * torch.from_numpy(
+        sc["src_n"]).to(dev)
+    E = far_field(a[:, None] * zhat, pos, torch.from_numpy(sc["snk"]).to(dev),
+                  torch.from_numpy(sc["snk_n"]).to(dev), hit_bits=bits, beta=sc["beta"])
+    loss = db_mask_loss(E, torch.from_numpy(sc["in_beam"]).to(dev))
+    loss.backward()
+    ga = a.grad.cpu().numpy()
+    gd = delta.grad.cpu().numpy()
+    rng = np.random.default_rng(9)
+    kinds = rng.integers(0, 3, 64)
+    idx = rng.integers(0, S, 64)
+    h = 1e-6
+
+    def L(da=None, dd=None):
+        w = w0.clone()
+        p = p0.clone()
+        if da is not None:
+            w[da[0], 2] += da[1]
+        if dd is not None:
+            p[dd[0]] += dd[1] * nhat[dd[0]]
+        return _loss64(oracle, w, p, sc, visible, sc["in_beam"]).item()
+
+    fd, ad = [], []
+    for kd, i in zip(kinds, idx):
+        if kd == 0:
+            fd.append((L(da=(i, h)) - L(da=(i, -h))) / (2 * h))
+            ad.append(ga[i].real)
+        elif kd == 1:
+            fd.append((L(da=(i, 1j * h)) - L(da=(i, -1j * h))) / (2 * h))
+            ad.append(ga[i].imag)
+        else:
+            fd.append((L(dd=(i, h)) - L(dd=(i, -h))) / (2 * h))
+            ad.append(gd[i])
+    fd, ad = np.array(fd), np.array(ad)
+    scale = np.abs(fd).max()
+    err = np.abs(fd - ad) / scale
+    print(f"finite differences: max |fd - adjoint| / max|fd| = {err.max():.2e}", flush=True)
+    assert scale > 0 and err.max() < GRAD_RTOL
+
+
+def test_adjoint_flags_match_autograd(native, oracle):
+    """Sink weights and the sink-tangent step are transposed correctly."""
+    rng = np.random.default_rng(3)
+    S, K = 40, 300
+    src = rng.uniform(-0.5, 0.5, (S, 3)).astype(np.float32)
+    snk = (rng.normal(size=(K, 3)) * 50).astype(np.float32)
+    nrm = rng.normal(size=(K, 3)).astype(np.float32)
+    w = (rng.normal(size=(S, 3)) + 1j * rng.normal(size=(S, 3))).astype(np.complex64)
+    wk = (rng.normal(size=(K, 3)) + 1j * rng.normal(size=(K, 3))).astype(np.complex64)
+    g = (rng.normal(size=(K, 3)) + 1j * rng.normal(size=(K, 3))).astype(np.complex64)
+    beta = 2 * np.pi / 0.2
+    gw, gp = native.field_los_adjoint(src, w, snk, nrm, g, snk_w=wk, beta=beta,
+                                      sink_tangent=True)
+    w64 = torch.from_numpy(w.astype(np.complex128)).requires_grad_(True)
+    p64 = torch.from_numpy(src.astype(np.float64)).requires_grad_(True)
+    E = oracle.field_los_torch64(w64, p64, torch.from_numpy(snk.astype(np.float64)),
+                                 torch.from_numpy(nrm.astype(np.float64)),
+                                 torch.ones((S, K), dtype=torch.bool), 0.0, beta)
+    E = E * torch.from_numpy(wk.astype(np.complex128))
+    nh = torch.from_numpy(nrm.astype(np.float64))
+    nh = nh / nh.norm(dim=1, keepdim=True)
+    E = E - nh * (E * nh).sum(-1, keepdim=True)
+    E.backward(torch.from_numpy(g.astype(np.complex128)))
+    rw = np.linalg.norm(gw.cpu().numpy() - w64.grad.numpy()) / np.linalg.norm(w64.grad.numpy())
+    rp = np.linalg.norm(gp.cpu().numpy() - p64.grad.numpy()) / np.linalg.norm(p64.grad.numpy())
+    assert rw < GRAD_RTOL and rp < GRAD_RTOL

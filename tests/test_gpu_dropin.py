@@ -130,3 +130,26 @@ def test_calculate_farfield_unit_sphere_quirk(oracle):
     th, ph = sinks.point_data["theta_(Radians)"], sinks.point_data["phi_(Radians)"]
     et2, ep2 = oracle.etheta_ephi(Ex, Ey, Ez, th, ph)
     assert np.allclose(et, et2) and np.allclose(ep, ep2)
+
+
+def test_one_bounce_grid_equals_brute_force_masks(oracle):
+    """one_bounce_fields with visibility through the grid (fused kernel) equals
+    the masked brute-force legs."""
+    from lyceanem import _native
+    from lyceanem.models.frequency_domain import one_bounce_fields
+
+    rng = np.random.default_rng(21)
+    ctr = rng.uniform(-3, 3, (900, 1, 3)) * [1, 1, 0.3] + [0, 0, 4]
+    tris = (ctr + rng.uniform(-0.4, 0.4, (900, 3, 3))).reshape(-1, 9).astype(np.float32)
+    src = rng.uniform(-0.5, 0.5, (60, 3)).astype(np.float32) * [1, 1, 0]
+    w = (rng.normal(size=(60, 3)) + 1j * rng.normal(size=(60, 3))).astype(np.complex64)
+    scat = (rng.uniform(-3, 3, (150, 3)) * [1, 1, 0.3] + [0, 0, 4]).astype(np.float32)
+    scat_n = rng.normal(size=(150, 3)).astype(np.float32)
+    snk = (rng.uniform(-6, 6, (90, 3)) * [1, 1, 0] + [0, 0, 9]).astype(np.float32)
+    snk_n = np.tile(np.array([[0, 0, 1]], np.float32), (90, 1))
+    beta = 2 * np.pi / 0.3
+    a = one_bounce_fields(src, w, snk, snk_n, scat, scat_n, tris, beta=beta)
+    b = one_bounce_fields(src, w, snk, snk_n, scat, scat_n, tris, beta=beta,
+                          grid=_native.Grid(tris))
+    assert np.abs(a).max() > 0
+    assert np.allclose(a, b, rtol=1e-6, atol=1e-6 * np.abs(a).max())

@@ -54,7 +54,8 @@ def test_visibility_full_c1_sampled(native, oracle):
 
 def test_visibility_edge_shapes(native, oracle):
     rng = np.random.default_rng(3)
-    tris = (rng.uniform(0, 2, (1, 3)) + rng.uniform(-0.5, 0.5, (37, 9))).astype(np.float32)
+    tris = (rng.uniform(0, 2, (37, 1, 3)) + rng.uniform(-0.5, 0.5, (37, 3, 3))).reshape(
+        37, 9).astype(np.float32)
     for S, K in [(1, 1), (3, 65), (2, 130), (5, 257)]:
         src = rng.uniform(-1, 0, (S, 3)).astype(np.float32)
         snk = rng.uniform(2, 3, (K, 3)).astype(np.float32)

@@ -174,3 +174,25 @@ def test_r_f32_error_is_reported_scale(oracle):
     ok = np.abs(E[:, 0]) > 1e-9
     dph = np.abs(np.angle(E32[ok, 0] / E[ok, 0]))
     assert 1e-4 < dph.max() < BETA * 6.2e-5 / 2 * 1.01
+
+
+def test_torch64_restatement_matches_c_oracle(oracle):
+    """The closed-form float64 torch field (used for reference gradients) equals
+    the C oracle's u/v-basis restatement."""
+    import torch
+
+    rng = np.random.default_rng(12)
+    S, K = 11, 17
+    src = rng.uniform(-1, 1, (S, 3)).astype(np.float32)
+    snk = rng.uniform(3, 9, (K, 3)).astype(np.float32)
+    nrm = rng.normal(size=(K, 3)).astype(np.float32)
+    w = (rng.normal(size=(S, 3)) + 1j * rng.normal(size=(S, 3))).astype(np.complex64)
+    hit = rng.random((S, K)) < 0.3
+    for alpha in (0.0, 0.02):
+        ref = oracle.field_los(src, w, snk, nrm, alpha, BETA, hit=hit)
+        t = oracle.field_los_torch64(torch.from_numpy(w.astype(np.complex128)),
+                                     torch.from_numpy(src.astype(np.float64)),
+                                     torch.from_numpy(snk.astype(np.float64)),
+                                     torch.from_numpy(nrm.astype(np.float64)),
+                                     torch.from_numpy(~hit), alpha, BETA).numpy()
+        assert np.allclose(t, ref, rtol=1e-10, atol=1e-14 * np.abs(ref).max())
