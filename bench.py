@@ -144,7 +144,7 @@ class C2:
     name = "c2"
     stats_stride = 8  # MT tests counted on every 8th sink, scaled
 
-    def __init__(self, torch, n_sinks=16384):
+    def __init__(self, torch, n_sinks=16384, cells_per_tri=2.0):
         from lyceanem import _native, scenes
 
         self.torch, self.nat = torch, _native
@@ -160,7 +160,7 @@ class C2:
         self.S, self.K, self.T = len(sc["src"]), len(sc["snk"]), len(sc["tris"])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        self.grid = _native.Grid(self.tris_t)
+        self.grid = _native.Grid(self.tris_t, cells_per_tri=cells_per_tri)
         torch.cuda.synchronize()
         self.build_s = time.perf_counter() - t0
         W = (self.S + 63) // 64
@@ -176,8 +176,9 @@ class C2:
             ev[0].record()
         nat.check(lib.lem_visibility_grid(p(self.src), self.S, p(self.snk), self.K,
                                           nat.ctypes.byref(g.info), p(g.cell_start), p(g.recs),
-                                          nat.VIS_SINK_MAJOR, p(self.bits), p(self.dist),
-                                          nat.stream_ptr()), "visibility_grid")
+                                          p(g.cell_box), nat.SEM_NUMBA, nat.VIS_SINK_MAJOR,
+                                          p(self.bits),
+                                          p(self.dist), nat.stream_ptr()), "visibility_grid")
         if ev is not None:
             ev[1].record()
             ev[2].record()
@@ -237,7 +238,261 @@ class C2:
                 "sources": self.S, "sinks": self.K, "triangles": self.T, "rays": self.rays()}
 
 
-CONFIGS = {"c1": C1, "c2": C2}
+# --------------------------------------------------------------------------
+# config 3: one-bounce scattering network (fused grid legs)
+# --------------------------------------------------------------------------
+
+class C3:
+    name = "c3"
+
+    def __init__(self, torch):
+        from lyceanem import _native, scenes
+
+        self.torch, self.nat = torch, _native
+        sc = scenes.clutter_c3()
+        self.sc = sc
+        dev = _native.device()
+        t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev)
+             for k, v in sc.items() if isinstance(v, np.ndarray)}
+        self.t = t
+        self.grid = _native.Grid(t["tris"])
+        self.S, self.P, self.K = len(sc["src"]), len(sc["scat"]), len(sc["snk"])
+        self.T = len(sc["tris"])
+
+    def rays(self):
+        return self.S * self.P + self.P * self.K + self.S * self.K
+
+    def step(self, ev=None):
+        nat, t, g, b = self.nat, self.t, self.grid, self.sc["beta"]
+        if ev is not None:
+            ev[0].record()
+        H = nat.field_los_grid(g, t["src"], t["w"], t["scat"], t["scat_n"], snk_w=t["scat_w"],
+                               beta=b, sink_tangent=True)
+        if ev is not None:
+            ev[1].record()
+        E = nat.field_los_grid(g, t["scat"], H, t["snk"], t["snk_n"], beta=b)
+        E += nat.field_los_grid(g, t["src"], t["w"], t["snk"], t["snk_n"], beta=b,
+                                sink_tangent=True)
+        if ev is not None:
+            ev[2].record()
+        self.E = E
+
+    def record(self, times_ms):
+        leg1 = float(np.mean([a for a, _ in times_ms]))
+        rest = float(np.mean([b for _, b in times_ms]))
+        pairs1 = self.S * self.P
+        return {
+            "kernel": "k_field_los_grid_partial (leg 1: sources -> scatterers)",
+            "kernel_ms": leg1,
+            "roofline": {"bound": "valu", "achieved": pairs1 * PAIR_FLOPS / (leg1 * 1e-3) / 1e12,
+                         "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                         "frac": pairs1 * PAIR_FLOPS / (leg1 * 1e-3) / PEAK_FP32,
+                         "traffic": None, "note": "pair flops only; the DDA cast of every "
+                                                  "pair runs inside the same kernel"},
+            "extra": {"gpairs_per_s": self.rays() / ((leg1 + rest) * 1e-3) / 1e9,
+                      "leg1_gpairs_per_s": pairs1 / (leg1 * 1e-3) / 1e9,
+                      "legs2_los_ms": rest, "grid_cells": [int(x) for x in self.grid.info.n]},
+        }
+
+    def cpu_baseline(self, budget_s):
+        from oracle import oracle as orc
+
+        sc = self.sc
+        nth = orc.nthreads_default()
+        rng = np.random.default_rng(0)
+        n = 32
+        t0 = time.perf_counter()
+        pairs = np.stack([rng.integers(0, self.S, n), rng.integers(0, self.P, n)], 1)
+        orc.visibility_sampled(sc["src"], sc["scat"], pairs, sc["tris"], nthreads=nth)
+        dt = time.perf_counter() - t0
+        n = int(max(32, min(10 ** 6, n * budget_s / max(dt, 1e-6))))
+        pairs = np.stack([rng.integers(0, self.S, n), rng.integers(0, self.P, n)], 1)
+        t0 = time.perf_counter()
+        hit, _ = orc.visibility_sampled(sc["src"], sc["scat"], pairs, sc["tris"], nthreads=nth)
+        dt = time.perf_counter() - t0
+        return {"value": n / dt / 1e9, "unit": "Gpairs/s", "cores": nth, "kind": "port",
+                "sample": f"{n} random source->scatterer pairs, brute-force visibility over "
+                          f"{self.T} triangles (the field term is negligible beside it), "
+                          f"{dt:.1f} s"}
+
+    def config(self):
+        return {"workload": "c3: 256x256 array at 5.8 GHz, 50k random triangles, 200k "
+                            "scatterers, 64x64 sinks; one bounce + line of sight, visibility on "
+                            "both legs", "sources": self.S, "scatterers": self.P,
+                "sinks": self.K, "triangles": self.T, "pairs": self.rays()}
+
+
+# --------------------------------------------------------------------------
+# config 4: beam synthesis, adjoint vs forward
+# --------------------------------------------------------------------------
+
+class C4:
+    name = "c4"
+
+    def __init__(self, torch):
+        from lyceanem import _native, scenes
+
+        self.torch, self.nat = torch, _native
+        sc = scenes.conformal_c4()
+        self.sc = sc
+        dev = _native.device()
+        self.grid = _native.Grid(sc["tris"])
+        self.bits, _ = self.grid.visibility(sc["src"], sc["snk"], any_hit=True)
+        self.pos = torch.from_numpy(sc["src"]).to(dev)
+        self.w = torch.from_numpy(sc["w"]).to(dev)
+        self.snk = torch.from_numpy(sc["snk"]).to(dev)
+        self.snk_n = torch.from_numpy(sc["snk_n"]).to(dev)
+        self.in_beam = torch.from_numpy(sc["in_beam"]).to(dev)
+        self.S, self.K = len(sc["src"]), len(sc["snk"])
+
+    def rays(self):
+        return self.S * self.K
+
+    def step(self, ev=None):
+        from lyceanem.differentiable import far_field, db_mask_loss
+
+        w = self.w.clone().requires_grad_(True)
+        pos = self.pos.clone().requires_grad_(True)
+        if ev is not None:
+            ev[0].record()
+        E = far_field(w, pos, self.snk, self.snk_n, hit_bits=self.bits, beta=self.sc["beta"])
+        if ev is not None:
+            ev[1].record()
+        loss = db_mask_loss(E, self.in_beam)
+        loss.backward()
+        if ev is not None:
+            ev[2].record()
+        self.loss = loss
+
+    def record(self, times_ms):
+        fwd = float(np.mean([a for a, _ in times_ms]))
+        bwd = float(np.mean([b for _, b in times_ms]))
+        pairs = self.rays()
+        return {
+            "kernel": "k_field_los_partial (forward) / k_adjoint_partial (backward)",
+            "kernel_ms": fwd,
+            "roofline": {"bound": "valu", "achieved": pairs * PAIR_FLOPS / (fwd * 1e-3) / 1e12,
+                         "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                         "frac": pairs * PAIR_FLOPS / (fwd * 1e-3) / PEAK_FP32, "traffic": None},
+            "extra": {"forward_ms": fwd, "loss_and_adjoint_ms": bwd,
+                      "adjoint_over_forward": bwd / fwd,
+                      "forward_gpairs_per_s": pairs / (fwd * 1e-3) / 1e9},
+        }
+
+    def cpu_baseline(self, budget_s):
+        from oracle import oracle as orc
+
+        sc = self.sc
+        nth = orc.nthreads_default()
+        rows = 8
+        t0 = time.perf_counter()
+        orc.field_los(sc["src"][:rows], sc["w"][:rows], sc["snk"], sc["snk_n"], 0.0, sc["beta"],
+                      nthreads=nth)
+        dt = time.perf_counter() - t0
+        rows = int(max(8, min(self.S, rows * budget_s / max(dt, 1e-6))))
+        t0 = time.perf_counter()
+        orc.field_los(sc["src"][:rows], sc["w"][:rows], sc["snk"], sc["snk_n"], 0.0, sc["beta"],
+                      nthreads=nth)
+        dt = time.perf_counter() - t0
+        return {"value": rows * self.K / dt / 1e9, "unit": "Gpairs/s", "cores": nth,
+                "kind": "port", "sample": f"float64 field of the first {rows} of {self.S} "
+                                          f"elements x {self.K} sinks, {dt:.1f} s"}
+
+    def config(self):
+        return {"workload": "c4: 64x64 conformal array on a 0.5 m cylinder, 10 GHz, 181x361 "
+                            "far field, cylinder+fin mask (20k triangles) frozen",
+                "sources": self.S, "sinks": self.K, "pairs": self.rays()}
+
+
+# --------------------------------------------------------------------------
+# config 5: batched-excitation contraction on MFMA
+# --------------------------------------------------------------------------
+
+class C5:
+    name = "c5"
+
+    def __init__(self, torch):
+        from lyceanem import _native, scenes
+
+        self.torch, self.nat = torch, _native
+        self.lib = _native.load_library()
+        sc = scenes.batched_c5()
+        self.sc = sc
+        dev = _native.device()
+        self.src = torch.from_numpy(sc["src"]).to(dev)
+        self.snk = torch.from_numpy(sc["snk"]).to(dev)
+        self.snk_n = torch.from_numpy(sc["snk_n"]).to(dev)
+        self.kf = torch.from_numpy(sc["wavenumbers"]).to(dev)
+        self.W = torch.from_numpy(sc["W"]).to(dev)
+        self.S, self.K, self.F = len(sc["src"]), len(sc["snk"]), len(sc["wavenumbers"])
+        self.out = torch.empty((self.F, 3, self.K, 64), dtype=torch.complex64, device=dev)
+
+    def rays(self):
+        return self.S * self.K * self.F * 64
+
+    def step(self, ev=None):
+        nat, p = self.nat, self.nat.ptr
+        pol = self.sc["pol"]
+        if ev is not None:
+            ev[0].record()
+        nat.check(self.lib.lem_gw_mfma(p(self.src), self.S, float(pol[0]), float(pol[1]),
+                                       float(pol[2]), p(self.snk), p(self.snk_n), self.K,
+                                       p(self.kf), self.F, 0.0, p(self.W), 64, p(self.out),
+                                       nat.stream_ptr()), "gw")
+        if ev is not None:
+            ev[1].record()
+            ev[2].record()
+
+    def record(self, times_ms):
+        t = float(np.mean([a for a, _ in times_ms]))
+        flops = 8.0 * self.S * self.K * 64 * self.F * 3
+        torch = self.torch
+        # the plain per-excitation kernel on one (f, b): scaled to all F x 64
+        w = (self.W[0, :, 0][:, None] * torch.tensor([0, 0, 1], dtype=torch.complex64,
+                                                     device=self.W.device)).contiguous()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        self.nat.field_los(self.src, w, self.snk, self.snk_n, beta=float(self.sc["wavenumbers"][0]))
+        e0.record()
+        for _ in range(4):
+            self.nat.field_los(self.src, w, self.snk, self.snk_n,
+                               beta=float(self.sc["wavenumbers"][0]))
+        e1.record()
+        torch.cuda.synchronize()
+        per = e0.elapsed_time(e1) / 4
+        return {
+            "kernel": "k_gw_mfma",
+            "kernel_ms": t,
+            "roofline": {"bound": "mfma", "achieved": flops / (t * 1e-3) / 1e12,
+                         "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                         "frac": flops / (t * 1e-3) / PEAK_FP32, "traffic": None},
+            "extra": {"gemm_flops": flops, "g_evals": self.S * self.K * self.F,
+                      "per_excitation_kernel_ms": per,
+                      "per_excitation_total_ms": per * self.F * 64,
+                      "speedup_vs_per_excitation": per * self.F * 64 / t},
+        }
+
+    def cpu_baseline(self, budget_s):
+        from oracle import oracle as orc
+
+        sc = self.sc
+        nth = orc.nthreads_default()
+        rows = 16
+        t0 = time.perf_counter()
+        G = orc.field_los(sc["src"][:rows], sc["pol"].astype(np.complex64), sc["snk"], sc["snk_n"],
+                          0.0, sc["wavenumbers"][0], elements=True, nthreads=nth)
+        np.einsum("skc,sb->ckb", G, sc["W"][0, :rows].astype(np.complex128))
+        dt = time.perf_counter() - t0
+        return {"value": rows * self.K * 64 / dt / 1e9, "unit": "G (pair x set)/s", "cores": nth,
+                "kind": "port", "sample": f"float64 G of {rows} sources x {self.K} sinks at one "
+                                          f"frequency, contracted with 64 sets, {dt:.1f} s"}
+
+    def config(self):
+        return {"workload": "c5: 128x128 sources x 128x128 far-field sinks x 32 frequencies "
+                            "(2-4 GHz) x 64 steering sets, G generated on the fly, FP32 MFMA",
+                "sources": self.S, "sinks": self.K, "frequencies": self.F, "sets": 64}
+
+
+CONFIGS = {"c1": C1, "c2": C2, "c3": C3, "c4": C4, "c5": C5}
 DEFAULT_CONFIG = "c2"
 
 
@@ -250,6 +505,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--c2-sinks", type=int, default=16384, help="c2 sink count (probe runs)")
+    ap.add_argument("--cells-per-tri", type=float, default=2.0, help="grid cell budget")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -261,7 +517,8 @@ def main(argv=None) -> int:
         print("bench.py needs a HIP device", file=sys.stderr)
         return 2
     torch.cuda.set_device(0)
-    cfg = C2(torch, a.c2_sinks) if a.config == "c2" else CONFIGS[a.config](torch)
+    cfg = C2(torch, a.c2_sinks, a.cells_per_tri) if a.config == "c2" else \
+        CONFIGS[a.config](torch)
     print(f"[bench] config {a.config}: setup done", file=sys.stderr, flush=True)
     for _ in range(a.warmup):
         cfg.step()
@@ -283,10 +540,12 @@ def main(argv=None) -> int:
         cpu = cfg.cpu_baseline(a.cpu_seconds)
         print(f"[bench] cpu baseline: {cpu['value']:.3f} {cpu['unit']}", file=sys.stderr,
               flush=True)
+    unit = {"c3": "Gpairs/s", "c4": "Gpairs/s", "c5": "G (pair x set)/s"}.get(a.config, "Mrays/s")
+    scale = 1e6 if unit == "Mrays/s" else 1e9
     line = {
         "metric": "Mrays/s (1M-tri DDA); Gpairs/s scattering + % roofline; adjoint/forward time",
-        "value": cfg.rays() / (ms * 1e-3) / 1e6,
-        "unit": "Mrays/s",
+        "value": cfg.rays() / (ms * 1e-3) / scale,
+        "unit": unit,
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,

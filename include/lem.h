@@ -138,23 +138,28 @@ typedef struct {
  * 2 is a good default).
  * 1) lem_grid_plan: bounds, resolution and per-cell counts; synchronises the
  *    stream and fills *info (ncells, nrefs).
- * 2) the caller allocates cell_start (ncells + 1 int32), tri_ids (nrefs int32)
- *    and recs (nrefs x 12 float32: v0, e1, e2 and padding, cell by cell), and
+ * 2) the caller allocates cell_start (ncells + 1 int32), tri_ids (nrefs int32),
+ *    recs (nrefs x 12 float32: v0, e1, e2 and padding, cell by cell) and
+ *    cell_box (ncells x 8 float32: the tight box of each cell's contents,
+ *    lo.xyz, 0, hi.xyz, 0), and
  * 3) lem_grid_fill writes them; the same workspace must be passed to both. */
 int64_t lem_grid_workspace(int64_t T, float cells_per_tri);
 int lem_grid_plan(const float *tris, int64_t T, float cells_per_tri, lem_grid_info *info,
                   void *workspace, int64_t workspace_bytes, void *stream);
 int lem_grid_fill(const float *tris, int64_t T, const lem_grid_info *info, int32_t *cell_start,
-                  int32_t *tri_ids, float *recs, void *workspace, int64_t workspace_bytes,
-                  float cells_per_tri, void *stream);
+                  int32_t *tri_ids, float *recs, float *cell_box, void *workspace,
+                  int64_t workspace_bytes, float cells_per_tri, void *stream);
 
-/* Dense S x K visibility through the grid (DDA): the same masks and kernel1Dv3
- * distances as lem_visibility_dense with LEM_SEM_NUMBA.  Replaces the tile
- * traversal raycast_tiles / DDA (raycasting_accelerated.cuh:106-268, 458-482)
- * for source -> sink rays.  Flags: LEM_VIS_ANY_HIT, LEM_VIS_SINK_MAJOR. */
+/* Dense S x K visibility through the grid (DDA): the same masks (and, with
+ * LEM_SEM_NUMBA, kernel1Dv3 distances) as lem_visibility_dense with the same
+ * semantics.  Replaces the tile traversal raycast_tiles / DDA
+ * (raycasting_accelerated.cuh:106-268, 458-482) for source -> sink rays.
+ * Flags: LEM_VIS_ANY_HIT, LEM_VIS_SINK_MAJOR, LEM_VIS_SKIP_SELF.
+ * LEM_SEM_NATIVE is any-hit (no distance). */
 int lem_visibility_grid(const float *src, int64_t S, const float *snk, int64_t K,
                         const lem_grid_info *info, const int32_t *cell_start, const float *recs,
-                        int flags, uint64_t *hit_bits, float *dist, void *stream);
+                        const float *cell_box, int semantics, int flags, uint64_t *hit_bits,
+                        float *dist, void *stream);
 
 /* Line-of-sight field fused with the grid traversal: lem_field_los (source
  * sum only) where each ray is cast any-hit through the grid instead of read
@@ -163,8 +168,8 @@ int lem_visibility_grid(const float *src, int64_t S, const float *snk, int64_t K
 int lem_field_los_grid(const float *src, const float *src_w, int64_t S, const float *snk,
                        const float *snk_n, const float *snk_w, int64_t K,
                        const lem_grid_info *info, const int32_t *cell_start, const float *recs,
-                       double alpha, double beta, int flags, float *out, void *workspace,
-                       int64_t workspace_bytes, void *stream);
+                       const float *cell_box, double alpha, double beta, int flags, float *out,
+                       void *workspace, int64_t workspace_bytes, void *stream);
 
 /* Hand-written adjoint of lem_field_los (source sum) with the mask frozen.
  * grad_out: K x 3 complex64, dLoss/dE in PyTorch's convention
@@ -181,13 +186,29 @@ int lem_field_los_adjoint(const float *src, const float *src_w, int64_t S, const
                           const float *grad_out, float *grad_w, float *grad_p, void *workspace,
                           int64_t workspace_bytes, void *stream);
 
+/* Batched-excitation contraction on FP32 MFMA: for F wavenumbers k_f and B = 64
+ * excitation sets W at once,
+ *   E[f,c,k,b] = sum_s G_c[k,s,f] W[f,s,b],
+ *   G[k,s,f]   = F_f(r) (d . n_k) (e - d (d . e)),  F_f the lossy_propagation
+ *                factor at wavenumber k_f (empropagation.py:768-801), e = (pol_x,
+ *                pol_y, pol_z) the source polarisation, every path visible.
+ * Equals B x F calls of lem_field_los with weights W[f,:,b] e (the reference
+ * runs one calculate_scattering per excitation; its multiE branch cannot run,
+ * frequency_domain.py:612-617).  src: S x 3 (S a multiple of 32), snk, snk_n:
+ * K x 3 (K a multiple of 32), wavenumbers: F float64, W: F x S x 64 complex64,
+ * out: F x 3 x K x 64 complex64. */
+int lem_gw_mfma(const float *src, int64_t S, float pol_x, float pol_y, float pol_z,
+                const float *snk, const float *snk_n, int64_t K, const double *wavenumbers,
+                int64_t F, double alpha, const float *W, int64_t B, float *out, void *stream);
+
 /* Instrumented grid traversal for measurement: adds, over all S x K rays, the
  * Moller-Trumbore tests performed (stats[0]), cells visited (stats[1]) and
  * blocked rays (stats[2]) to the device uint64 array `stats` (zero it first).
  * Writes no masks.  No counterpart in the reference (it reports no work counts). */
 int lem_visibility_grid_stats(const float *src, int64_t S, const float *snk, int64_t K,
                               const lem_grid_info *info, const int32_t *cell_start,
-                              const float *recs, int flags, uint64_t *stats, void *stream);
+                              const float *recs, const float *cell_box, int flags,
+                              uint64_t *stats, void *stream);
 
 #ifdef __cplusplus
 }

@@ -265,6 +265,43 @@ __global__ __launch_bounds__(GB) void k_gather(const float *__restrict__ tris,
     recs[3 * i + 2] = make_float4(t.e2z, 0.0f, 0.0f, 0.0f);
 }
 
+// Tight box of each cell's contents: the union of its triangles' eps-inflated
+// boxes, clipped to the eps-inflated cell.  Empty cells get an inverted box.
+// Stored as two float4 per cell: (lo.xyz, 0), (hi.xyz, 0).
+__global__ __launch_bounds__(GB) void k_cell_boxes(const float *__restrict__ tris,
+                                                   const int32_t *__restrict__ cell_start,
+                                                   const int32_t *__restrict__ ids, GridDev G,
+                                                   int64_t ncells, float4 *__restrict__ box)
+{
+    int64_t c = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (c >= ncells)
+        return;
+    const float INF = __builtin_huge_valf();
+    float lo[3] = {INF, INF, INF}, hi[3] = {-INF, -INF, -INF};
+    for (int32_t j = cell_start[c]; j < cell_start[c + 1]; ++j) {
+        const float *t = tris + 9 * (int64_t)ids[j];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], fminf(t[a], fminf(t[3 + a], t[6 + a])) - G.eps);
+            hi[a] = fmaxf(hi[a], fmaxf(t[a], fmaxf(t[3 + a], t[6 + a])) + G.eps);
+        }
+    }
+    const int ix = (int)(c % G.nx), iy = (int)((c / G.nx) % G.ny), iz = (int)(c / ((int64_t)G.nx * G.ny));
+    const float cl[3] = {G.lox + G.cx * ix - G.eps, G.loy + G.cy * iy - G.eps,
+                         G.loz + G.cz * iz - G.eps};
+    const float ch[3] = {G.lox + G.cx * (ix + 1) + G.eps, G.loy + G.cy * (iy + 1) + G.eps,
+                         G.loz + G.cz * (iz + 1) + G.eps};
+    if (cell_start[c + 1] > cell_start[c]) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fmaxf(lo[a], cl[a]);
+            hi[a] = fminf(hi[a], ch[a]);
+        }
+    }
+    box[2 * c] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+    box[2 * c + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+}
+
 // ---------------------------------------------------------------------------
 // DDA traversal
 // ---------------------------------------------------------------------------
@@ -281,10 +318,41 @@ LEM_DEV Tri load_rec(const float4 *__restrict__ recs, int32_t i)
 
 // Casts one charged ray through the grid.  Returns the blocked flag and
 // leaves kernel1Dv3's `dist` in `dist` (closest-hit mode).
-template <bool ANY>
+// True when the segment t in [t_lo, t_hi] of the ray meets the box (slab test
+// with a margin of eps in t).
+LEM_DEV bool seg_box(const Ray &r, float idx, float idy, float idz, float4 lo, float4 hi,
+                     float t_lo, float t_hi, float m)
+{
+    float tn = t_lo, tf = t_hi;
+    if (r.dx != 0.0f) {
+        float a = (lo.x - r.ox) * idx, b = (hi.x - r.ox) * idx;
+        tn = fmaxf(tn, fminf(a, b));
+        tf = fminf(tf, fmaxf(a, b));
+    } else if (r.ox < lo.x || r.ox > hi.x) {
+        return false;
+    }
+    if (r.dy != 0.0f) {
+        float a = (lo.y - r.oy) * idy, b = (hi.y - r.oy) * idy;
+        tn = fmaxf(tn, fminf(a, b));
+        tf = fminf(tf, fmaxf(a, b));
+    } else if (r.oy < lo.y || r.oy > hi.y) {
+        return false;
+    }
+    if (r.dz != 0.0f) {
+        float a = (lo.z - r.oz) * idz, b = (hi.z - r.oz) * idz;
+        tn = fmaxf(tn, fminf(a, b));
+        tf = fminf(tf, fmaxf(a, b));
+    } else if (r.oz < lo.z || r.oz > hi.z) {
+        return false;
+    }
+    return tn <= tf + m;
+}
+
+template <bool ANY, int SEM = 0>
 LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
                       const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
-                      unsigned int *n_tests = nullptr, unsigned int *n_cells = nullptr)
+                      const float4 *__restrict__ box, unsigned int *n_tests = nullptr,
+                      unsigned int *n_cells = nullptr)
 {
     const float INF = __builtin_huge_valf();
     // clip the segment [0, len] to the grid box
@@ -337,12 +405,20 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
     for (;;) {
         int64_t c = ((int64_t)iz * G.ny + iy) * G.nx + ix;
         int32_t b = cell_start[c], e = cell_start[c + 1];
-        if (n_cells) {
+        if (n_cells)
             *n_cells += 1;
+        if (e > b && !seg_box(r, idx, idy, idz, box[2 * c], box[2 * c + 1], -margin,
+                              dist + margin, margin))
+            e = b;
+        if (n_tests)
             *n_tests += (unsigned int)(e - b);
-        }
         for (int32_t j = b; j < e; ++j) {
             Tri t = load_rec(recs, j);
+            if (SEM == 1) {
+                if (hit_native(r, t))
+                    return true;
+                continue;
+            }
             double tt;
             if (hit_numba(r, dist, t, tt) && tt < (double)dist) {
                 dist = (float)tt;
@@ -378,11 +454,12 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
 // SINK_MAJOR = true : lane = source, block = (sink, 256 sources); bits K x W64(S),
 //                     dist K x S.  Neighbouring sources to one sink give nearly
 //                     parallel rays, so a wavefront walks nearly the same cells.
-template <bool ANY, bool SINK_MAJOR>
+template <bool ANY, bool SINK_MAJOR, int SEM>
 __global__ __launch_bounds__(GB) void k_visibility_grid(
     const float *__restrict__ src, int64_t S, const float *__restrict__ snk, int64_t K, GridDev G,
     const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
-    unsigned long long *__restrict__ bits, float *__restrict__ dist_out, int64_t nbl)
+    const float4 *__restrict__ box, unsigned long long *__restrict__ bits,
+    float *__restrict__ dist_out, int64_t nbl, int skip_self)
 {
     const int64_t outer = (int64_t)blockIdx.x / nbl;
     const int64_t inner = ((int64_t)blockIdx.x % nbl) * GB + threadIdx.x;
@@ -394,8 +471,8 @@ __global__ __launch_bounds__(GB) void k_visibility_grid(
                    snk[3 * k + 2]);
     float dist = r.len;
     bool hit = false;
-    if (active)
-        hit = dda_cast<ANY>(r, dist, G, cell_start, recs);
+    if (active && !(skip_self && s == k))
+        hit = dda_cast<ANY, SEM>(r, dist, G, cell_start, recs, box);
     unsigned long long bw = __ballot(active && hit);
     const int64_t W = (NI + 63) / 64;
     if ((threadIdx.x & 63) == 0) {
@@ -413,7 +490,7 @@ template <bool ANY>
 __global__ __launch_bounds__(GB) void k_grid_stats(
     const float *__restrict__ src, int64_t S, const float *__restrict__ snk, int64_t K, GridDev G,
     const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
-    unsigned long long *__restrict__ stats, int64_t nbl)
+    const float4 *__restrict__ box, unsigned long long *__restrict__ stats, int64_t nbl)
 {
     const int64_t k = (int64_t)blockIdx.x / nbl;
     const int64_t s = ((int64_t)blockIdx.x % nbl) * GB + threadIdx.x;
@@ -423,7 +500,7 @@ __global__ __launch_bounds__(GB) void k_grid_stats(
         Ray r = charge(src[3 * s], src[3 * s + 1], src[3 * s + 2], snk[3 * k], snk[3 * k + 1],
                        snk[3 * k + 2]);
         float dist = r.len;
-        hit = dda_cast<ANY>(r, dist, G, cell_start, recs, &nt, &nc);
+        hit = dda_cast<ANY>(r, dist, G, cell_start, recs, box, &nt, &nc);
     }
     // wave sums, one atomic per wave
     unsigned long long t64 = nt, c64 = nc, h64 = hit ? 1ull : 0ull;
@@ -446,8 +523,9 @@ __global__ __launch_bounds__(GB) void k_grid_stats(
 __global__ __launch_bounds__(FB) void k_field_los_grid_partial(
     const float *__restrict__ src, const float *__restrict__ src_w, int64_t S,
     const float *__restrict__ snk, const float *__restrict__ snk_n, int64_t K, GridDev G,
-    const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs, float alpha,
-    double beta, int64_t CH, int64_t nbk, float *__restrict__ ws_sum,
+    const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
+    const float4 *__restrict__ box, float alpha, double beta, int64_t CH, int64_t nbk,
+    float *__restrict__ ws_sum,
     float *__restrict__ ws_cmp)
 {
     __shared__ SrcLds s_src[FS_TILE];
@@ -472,7 +550,7 @@ __global__ __launch_bounds__(FB) void k_field_los_grid_partial(
                 const SrcLds &L = s_src[j];
                 Ray r = charge((float)L.x, (float)L.y, (float)L.z, xkf, ykf, zkf);
                 float dist = r.len;
-                if (dda_cast<true>(r, dist, G, cell_start, recs))
+                if (dda_cast<true>(r, dist, G, cell_start, recs, box))
                     continue;
                 C3 w;
                 w.xr = L.w[0]; w.xi = L.w[1]; w.yr = L.w[2]; w.yi = L.w[3]; w.zr = L.w[4];
@@ -672,11 +750,12 @@ extern "C" int lem_grid_plan(const float *tris, int64_t T, float cells_per_tri,
 }
 
 extern "C" int lem_grid_fill(const float *tris, int64_t T, const lem_grid_info *info,
-                             int32_t *cell_start, int32_t *tri_ids, float *recs, void *workspace,
-                             int64_t workspace_bytes, float cells_per_tri, void *stream)
+                             int32_t *cell_start, int32_t *tri_ids, float *recs, float *cell_box,
+                             void *workspace, int64_t workspace_bytes, float cells_per_tri,
+                             void *stream)
 {
     clear_error();
-    LEM_REQUIRE(T > 0 && tris && info && cell_start && tri_ids && recs,
+    LEM_REQUIRE(T > 0 && tris && info && cell_start && tri_ids && recs && cell_box,
                 "lem_grid_fill: null argument");
     const int64_t cap = plan_cap(T, cells_per_tri);
     LEM_REQUIRE(info->ncells <= cap, "lem_grid_fill: info does not match the workspace");
@@ -698,23 +777,27 @@ extern "C" int lem_grid_fill(const float *tris, int64_t T, const lem_grid_info *
     if (info->nrefs > 0)
         hipLaunchKernelGGL(k_gather, dim3((unsigned)ceil_div(info->nrefs, GB)), dim3(GB), 0, st,
                            tris, tri_ids, info->nrefs, reinterpret_cast<float4 *>(recs));
+    hipLaunchKernelGGL(k_cell_boxes, dim3((unsigned)ceil_div(nc, GB)), dim3(GB), 0, st, tris,
+                       cell_start, tri_ids, G, nc, reinterpret_cast<float4 *>(cell_box));
     return check_launch("lem_grid_fill");
 }
 
 extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk, int64_t K,
                                    const lem_grid_info *info, const int32_t *cell_start,
-                                   const float *recs, int flags, uint64_t *hit_bits, float *dist,
-                                   void *stream)
+                                   const float *recs, const float *cell_box, int semantics,
+                                   int flags, uint64_t *hit_bits, float *dist, void *stream)
 {
     clear_error();
     LEM_REQUIRE(S >= 0 && K >= 0, "lem_visibility_grid: negative size");
     if (S == 0 || K == 0)
         return OK;
-    LEM_REQUIRE(src && snk && info && cell_start && recs && hit_bits,
+    LEM_REQUIRE(src && snk && info && cell_start && recs && cell_box && hit_bits,
                 "lem_visibility_grid: null pointer");
     LEM_REQUIRE(info->n[0] > 0 && info->n[1] > 0 && info->n[2] > 0, "lem_visibility_grid: bad grid");
-    LEM_REQUIRE(!(flags & LEM_VIS_SKIP_SELF), "lem_visibility_grid: skip-self is brute-force only");
-    const bool any = (flags & LEM_VIS_ANY_HIT) != 0;
+    LEM_REQUIRE(semantics == LEM_SEM_NUMBA || semantics == LEM_SEM_NATIVE,
+                "lem_visibility_grid: unknown semantics");
+    const int skip_self = (flags & LEM_VIS_SKIP_SELF) ? 1 : 0;
+    const bool any = (flags & LEM_VIS_ANY_HIT) != 0 || semantics == LEM_SEM_NATIVE;
     const bool sink_major = (flags & LEM_VIS_SINK_MAJOR) != 0;
     LEM_REQUIRE(!(any && dist), "lem_visibility_grid: dist needs the closest-hit mode");
     const int64_t NI = sink_major ? S : K, NO = sink_major ? K : S;
@@ -725,31 +808,37 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
     dim3 grid((unsigned)(NO * nbl));
     unsigned long long *hb = reinterpret_cast<unsigned long long *>(hit_bits);
     const float4 *rc4 = reinterpret_cast<const float4 *>(recs);
-    if (any) {
+    const float4 *bx4 = reinterpret_cast<const float4 *>(cell_box);
+#define LEM_VG(A, M, SM)                                                                      \
+    hipLaunchKernelGGL((k_visibility_grid<A, M, SM>), grid, dim3(GB), 0, st, src, S, snk, K, G,   \
+                       cell_start, rc4, bx4, hb, dist, nbl, skip_self)
+    if (semantics == LEM_SEM_NATIVE) {
         if (sink_major)
-            hipLaunchKernelGGL((k_visibility_grid<true, true>), grid, dim3(GB), 0, st, src, S, snk,
-                               K, G, cell_start, rc4, hb, dist, nbl);
+            LEM_VG(true, true, 1);
         else
-            hipLaunchKernelGGL((k_visibility_grid<true, false>), grid, dim3(GB), 0, st, src, S,
-                               snk, K, G, cell_start, rc4, hb, dist, nbl);
+            LEM_VG(true, false, 1);
+    } else if (any) {
+        if (sink_major)
+            LEM_VG(true, true, 0);
+        else
+            LEM_VG(true, false, 0);
     } else {
         if (sink_major)
-            hipLaunchKernelGGL((k_visibility_grid<false, true>), grid, dim3(GB), 0, st, src, S,
-                               snk, K, G, cell_start, rc4, hb, dist, nbl);
+            LEM_VG(false, true, 0);
         else
-            hipLaunchKernelGGL((k_visibility_grid<false, false>), grid, dim3(GB), 0, st, src, S,
-                               snk, K, G, cell_start, rc4, hb, dist, nbl);
+            LEM_VG(false, false, 0);
     }
+#undef LEM_VG
     return check_launch("k_visibility_grid");
 }
 
 extern "C" int lem_visibility_grid_stats(const float *src, int64_t S, const float *snk, int64_t K,
                                          const lem_grid_info *info, const int32_t *cell_start,
-                                         const float *recs, int flags, uint64_t *stats,
-                                         void *stream)
+                                         const float *recs, const float *cell_box, int flags,
+                                         uint64_t *stats, void *stream)
 {
     clear_error();
-    LEM_REQUIRE(S > 0 && K > 0 && src && snk && info && cell_start && recs && stats,
+    LEM_REQUIRE(S > 0 && K > 0 && src && snk && info && cell_start && recs && cell_box && stats,
                 "lem_visibility_grid_stats: bad argument");
     const int64_t nbl = ceil_div(S, GB);
     LEM_REQUIRE(K * nbl <= 2147483647LL, "lem_visibility_grid_stats: S*K too large");
@@ -757,25 +846,27 @@ extern "C" int lem_visibility_grid_stats(const float *src, int64_t S, const floa
     hipStream_t st = as_stream(stream);
     unsigned long long *sp = reinterpret_cast<unsigned long long *>(stats);
     const float4 *rc4 = reinterpret_cast<const float4 *>(recs);
+    const float4 *bx4 = reinterpret_cast<const float4 *>(cell_box);
     if (flags & LEM_VIS_ANY_HIT)
         hipLaunchKernelGGL((k_grid_stats<true>), dim3((unsigned)(K * nbl)), dim3(GB), 0, st, src,
-                           S, snk, K, G, cell_start, rc4, sp, nbl);
+                           S, snk, K, G, cell_start, rc4, bx4, sp, nbl);
     else
         hipLaunchKernelGGL((k_grid_stats<false>), dim3((unsigned)(K * nbl)), dim3(GB), 0, st, src,
-                           S, snk, K, G, cell_start, rc4, sp, nbl);
+                           S, snk, K, G, cell_start, rc4, bx4, sp, nbl);
     return check_launch("k_grid_stats");
 }
 
 extern "C" int lem_field_los_grid(const float *src, const float *src_w, int64_t S,
                                   const float *snk, const float *snk_n, const float *snk_w,
                                   int64_t K, const lem_grid_info *info, const int32_t *cell_start,
-                                  const float *recs, double alpha, double beta, int flags,
+                                  const float *recs, const float *cell_box, double alpha,
+                                  double beta, int flags,
                                   float *out, void *workspace, int64_t workspace_bytes,
                                   void *stream)
 {
     clear_error();
     LEM_REQUIRE(S > 0 && K > 0, "lem_field_los_grid: empty source or sink set");
-    LEM_REQUIRE(src && src_w && snk && snk_n && info && cell_start && recs && out,
+    LEM_REQUIRE(src && src_w && snk && snk_n && info && cell_start && recs && cell_box && out,
                 "lem_field_los_grid: null pointer");
     LEM_REQUIRE(!(flags & LEM_FIELD_ELEMENTS), "lem_field_los_grid: source sum only");
     const int64_t NC = los_chunk_count(S, K);
@@ -792,8 +883,9 @@ extern "C" int lem_field_los_grid(const float *src, const float *src_w, int64_t 
     hipStream_t st = as_stream(stream);
     hipLaunchKernelGGL(k_field_los_grid_partial, dim3((unsigned)(NC * nbk)), dim3(FB), 0, st, src,
                        src_w, S, snk, snk_n, K, to_dev(*info), cell_start,
-                       reinterpret_cast<const float4 *>(recs), (float)alpha, beta, CH, nbk, ws_sum,
-                       ws_cmp);
+                       reinterpret_cast<const float4 *>(recs),
+                       reinterpret_cast<const float4 *>(cell_box), (float)alpha, beta, CH, nbk,
+                       ws_sum, ws_cmp);
     int rc = check_launch("k_field_los_grid_partial");
     if (rc)
         return rc;

@@ -42,11 +42,13 @@ SIGNATURES = {
                         _INT),
     "lem_grid_workspace": ([_I64, ctypes.c_float], _I64),
     "lem_grid_plan": ([_P, _I64, ctypes.c_float, _P, _P, _I64, _P], _INT),
-    "lem_grid_fill": ([_P, _I64, _P, _P, _P, _P, _P, _I64, ctypes.c_float, _P], _INT),
-    "lem_visibility_grid": ([_P, _I64, _P, _I64, _P, _P, _P, _INT, _P, _P, _P], _INT),
-    "lem_visibility_grid_stats": ([_P, _I64, _P, _I64, _P, _P, _P, _INT, _P, _P], _INT),
-    "lem_field_los_grid": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _P, _P, _D, _D, _INT, _P, _P,
-                            _I64, _P], _INT),
+    "lem_grid_fill": ([_P, _I64, _P, _P, _P, _P, _P, _P, _I64, ctypes.c_float, _P], _INT),
+    "lem_visibility_grid": ([_P, _I64, _P, _I64, _P, _P, _P, _P, _INT, _INT, _P, _P, _P], _INT),
+    "lem_visibility_grid_stats": ([_P, _I64, _P, _I64, _P, _P, _P, _P, _INT, _P, _P], _INT),
+    "lem_field_los_grid": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _P, _P, _P, _D, _D, _INT, _P,
+                            _P, _I64, _P], _INT),
+    "lem_gw_mfma": ([_P, _I64, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P, _I64, _P,
+                     _I64, _D, _P, _I64, _P, _P], _INT),
     "lem_field_los_adjoint_workspace": ([_I64, _I64], _I64),
     "lem_field_los_adjoint": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _D, _D, _INT, _P, _P, _P, _P,
                                _I64, _P], _INT),
@@ -273,12 +275,15 @@ class Grid:
         self.cell_start = torch.empty(self.info.ncells + 1, dtype=torch.int32, device=dev)
         self.tri_ids = torch.empty(max(self.info.nrefs, 1), dtype=torch.int32, device=dev)
         self.recs = torch.empty((max(self.info.nrefs, 1), 12), dtype=torch.float32, device=dev)
+        self.cell_box = torch.empty((self.info.ncells, 8), dtype=torch.float32, device=dev)
         check(lib.lem_grid_fill(ptr(self.tris), T, ctypes.byref(self.info), ptr(self.cell_start),
-                                ptr(self.tri_ids), ptr(self.recs), ptr(self.ws), wsb,
-                                self.cells_per_tri, stream_ptr()), "lem_grid_fill")
+                                ptr(self.tri_ids), ptr(self.recs), ptr(self.cell_box),
+                                ptr(self.ws), wsb, self.cells_per_tri, stream_ptr()),
+              "lem_grid_fill")
         self.ws = None
 
-    def visibility(self, src, snk, any_hit=True, want_dist=False, sink_major=False):
+    def visibility(self, src, snk, any_hit=True, want_dist=False, sink_major=False,
+                   semantics=SEM_NUMBA, skip_self=False):
         """Hit bits (S x W64(K), or K x W64(S) sink-major) and optional dist."""
         torch = _torch()
         lib = load_library()
@@ -289,10 +294,12 @@ class Grid:
         bits = torch.zeros((rows, (cols + 63) // 64), dtype=torch.int64, device=src.device)
         dist = torch.empty((rows, cols), dtype=torch.float32, device=src.device) \
             if want_dist else None
-        flags = (VIS_ANY_HIT if any_hit else 0) | (VIS_SINK_MAJOR if sink_major else 0)
+        flags = (VIS_ANY_HIT if any_hit else 0) | (VIS_SINK_MAJOR if sink_major else 0) | \
+            (VIS_SKIP_SELF if skip_self else 0)
         check(lib.lem_visibility_grid(ptr(src), S, ptr(snk), K, ctypes.byref(self.info),
-                                      ptr(self.cell_start), ptr(self.recs), flags, ptr(bits),
-                                      ptr(dist), stream_ptr()), "lem_visibility_grid")
+                                      ptr(self.cell_start), ptr(self.recs), ptr(self.cell_box),
+                                      semantics, flags, ptr(bits), ptr(dist), stream_ptr()),
+              "lem_visibility_grid")
         return bits, dist
 
 
@@ -317,8 +324,8 @@ def field_los_grid(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, beta=2 *
     flags = FIELD_SINK_TANGENT if sink_tangent else 0
     check(lib.lem_field_los_grid(ptr(src), ptr(src_w), S, ptr(snk), ptr(snk_n), ptr(snk_w), K,
                                  ctypes.byref(grid.info), ptr(grid.cell_start), ptr(grid.recs),
-                                 float(alpha), float(beta), flags, ptr(out), ptr(ws), wsb,
-                                 stream_ptr()), "lem_field_los_grid")
+                                 ptr(grid.cell_box), float(alpha), float(beta), flags, ptr(out),
+                                 ptr(ws), wsb, stream_ptr()), "lem_field_los_grid")
     return out
 
 
@@ -331,9 +338,30 @@ def grid_stats(grid, src, snk, any_hit=False):
     stats = torch.zeros(3, dtype=torch.int64, device=src.device)
     check(lib.lem_visibility_grid_stats(ptr(src), src.shape[0], ptr(snk), snk.shape[0],
                                         ctypes.byref(grid.info), ptr(grid.cell_start),
-                                        ptr(grid.recs), VIS_ANY_HIT if any_hit else 0,
-                                        ptr(stats), stream_ptr()), "lem_visibility_grid_stats")
+                                        ptr(grid.recs), ptr(grid.cell_box),
+                                        VIS_ANY_HIT if any_hit else 0, ptr(stats), stream_ptr()),
+          "lem_visibility_grid_stats")
     return tuple(int(x) for x in stats.cpu().numpy())
+
+
+def gw_mfma(src, pol, snk, snk_n, wavenumbers, W, alpha=0.0):
+    """E[f, c, k, b] = sum_s G_c[k, s, f] W[f, s, b] on MFMA; W: (F, S, 64) complex64.
+    Returns (F, 3, K, 64) complex64."""
+    torch = _torch()
+    lib = load_library()
+    src = to_dev(src, torch.float32).reshape(-1, 3)
+    snk = to_dev(snk, torch.float32).reshape(-1, 3)
+    snk_n = to_dev(snk_n, torch.float32).reshape(-1, 3)
+    kf = to_dev(np.asarray(wavenumbers, np.float64), torch.float64).reshape(-1)
+    W = to_dev(W, torch.complex64)
+    S, K, F = src.shape[0], snk.shape[0], kf.shape[0]
+    if tuple(W.shape) != (F, S, 64):
+        raise LemError(f"W shape {tuple(W.shape)} != {(F, S, 64)}")
+    out = torch.empty((F, 3, K, 64), dtype=torch.complex64, device=src.device)
+    pol = [float(x) for x in np.asarray(pol, np.float32).ravel()]
+    check(lib.lem_gw_mfma(ptr(src), S, pol[0], pol[1], pol[2], ptr(snk), ptr(snk_n), K, ptr(kf),
+                          F, float(alpha), ptr(W), 64, ptr(out), stream_ptr()), "lem_gw_mfma")
+    return out
 
 
 def unpack_bits(bits, K):

@@ -40,12 +40,12 @@ def far_field(w, pos, sinks, sink_normals, hit_bits=None, alpha=0.0, beta=2 * np
     return FarField.apply(w, pos, sinks, sink_normals, hit_bits, float(alpha), float(beta))
 
 
-def db_mask_loss(E, in_beam, sidelobe_db=-25.0, eps=1e-30):
+def db_mask_loss(E, in_beam, sidelobe_db=-25.0, ref_power=None, eps=1e-30):
     """Squared dB mismatch to a mask: 0 dB inside the main beam, at most
-    `sidelobe_db` outside (excess only), relative to the peak power (treated as
-    a constant).  Mean over sinks."""
+    `sidelobe_db` outside (excess only), relative to `ref_power` (default: the
+    current peak power, treated as a constant).  Mean over sinks."""
     P = (E.real ** 2 + E.imag ** 2).sum(dim=-1)
-    peak = P.detach().max()
+    peak = P.detach().max() if ref_power is None else ref_power
     db = 10.0 * torch.log10(P / peak + eps)
     err = torch.where(in_beam, db, torch.clamp(db - sidelobe_db, min=0.0))
     return (err ** 2).mean()

@@ -214,3 +214,30 @@ def conformal_c4(n_az_el=64, n_z_el=64, radius=0.5, wavelength=0.03, n_el=181, n
     in_beam = (np.abs(az - 30.0) <= 5.0) & (np.abs(el) <= 5.0)
     return dict(src=pos, src_n=n.astype(np.float32), w=w, tris=tris, snk=snk, snk_n=snk_n,
                 in_beam=in_beam, wavelength=wavelength, alpha=0.0, beta=2 * np.pi / wavelength)
+
+
+def batched_c5(n_src_side=128, n_el=128, n_az=128, n_freq=32, f_lo=2e9, f_hi=4e9, n_sets=64,
+               far=1000.0, seed=10):
+    """Config 5: batched excitations for the G . W contraction.
+
+    Sources: n x n grid at lambda_min/2 (f_hi) in z=0, z-polarised.  Sinks:
+    n_el x n_az far-field grid (el-major, az-fastest) at `far` metres.  n_freq
+    wavenumbers evenly spread over [f_lo, f_hi].  W[f, s, b] = exp(-j k_f p_s . u_b)
+    steers set b towards one of n_sets seeded random upper-hemisphere directions."""
+    lam_min = C0 / f_hi
+    pitch = lam_min / 2
+    c = (np.arange(n_src_side) - (n_src_side - 1) / 2) * pitch
+    xx, yy = np.meshgrid(c, c)
+    S = n_src_side * n_src_side
+    src = np.stack([xx.ravel(), yy.ravel(), np.zeros(S)], 1).astype(np.float32)
+    snk, snk_n = sphere_sinks(n_el, n_az, far)
+    freqs = np.linspace(f_lo, f_hi, n_freq)
+    kf = 2 * np.pi * freqs / C0
+    rng = np.random.default_rng(seed)
+    th = np.arccos(rng.uniform(0.0, 1.0, n_sets))
+    ph = rng.uniform(-np.pi, np.pi, n_sets)
+    u = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
+    proj = src.astype(np.float64) @ u.T  # (S, B)
+    W = np.exp(-1j * kf[:, None, None] * proj[None]).astype(np.complex64)
+    return dict(src=src, snk=snk, snk_n=snk_n, pol=np.array([0, 0, 1], np.float32),
+                wavenumbers=kf, W=W, directions=u)

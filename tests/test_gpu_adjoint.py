@@ -21,13 +21,13 @@ def _mask(native, sc):
     return bits, ~native.unpack_bits(bits, len(sc["snk"]))
 
 
-def _loss64(oracle, w, pos, sc, visible, in_beam):
+def _loss64(oracle, w, pos, sc, visible, in_beam, ref_power=None):
     from lyceanem.differentiable import db_mask_loss
 
     E = oracle.field_los_torch64(w, pos, torch.from_numpy(sc["snk"].astype(np.float64)),
                                  torch.from_numpy(sc["snk_n"].astype(np.float64)),
                                  torch.from_numpy(visible), sc["alpha"], sc["beta"])
-    return db_mask_loss(E, torch.from_numpy(in_beam))
+    return db_mask_loss(E, torch.from_numpy(in_beam), ref_power=ref_power)
 
 
 def test_adjoint_vs_dense_reference_gradient(native, oracle):
@@ -76,7 +76,9 @@ def test_adjoint_vs_central_finite_differences(native, oracle):
         sc["src_n"]).to(dev)
     E = far_field(a[:, None] * zhat, pos, torch.from_numpy(sc["snk"]).to(dev),
                   torch.from_numpy(sc["snk_n"]).to(dev), hit_bits=bits, beta=sc["beta"])
-    loss = db_mask_loss(E, torch.from_numpy(sc["in_beam"]).to(dev))
+    # a fixed reference power: finite differences must see the same loss function
+    ref = float((E.real ** 2 + E.imag ** 2).sum(-1).max().item())
+    loss = db_mask_loss(E, torch.from_numpy(sc["in_beam"]).to(dev), ref_power=ref)
     loss.backward()
     ga = a.grad.cpu().numpy()
     gd = delta.grad.cpu().numpy()
@@ -92,7 +94,7 @@ def test_adjoint_vs_central_finite_differences(native, oracle):
             w[da[0], 2] += da[1]
         if dd is not None:
             p[dd[0]] += dd[1] * nhat[dd[0]]
-        return _loss64(oracle, w, p, sc, visible, sc["in_beam"]).item()
+        return _loss64(oracle, w, p, sc, visible, sc["in_beam"], ref_power=ref).item()
 
     fd, ad = [], []
     for kd, i in zip(kinds, idx):
