@@ -506,7 +506,21 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--c2-sinks", type=int, default=16384, help="c2 sink count (probe runs)")
     ap.add_argument("--cells-per-tri", type=float, default=2.0, help="grid cell budget")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB, 3 times (calibrates "
+                         "the FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args(argv)
+    if a.copy_mib > 0:
+        import torch
+
+        n = a.copy_mib * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
+        return 0
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the sharding plan is design-only).", file=sys.stderr)

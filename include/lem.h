@@ -125,11 +125,11 @@ typedef struct {
     float lo[3];    /* grid origin: scene bounds minus 2*eps */
     float cell[3];  /* cell edge per axis */
     int32_t n[3];   /* cells per axis */
-    int32_t pad;
+    int32_t mshift; /* macro blocks of 2^mshift cells per axis (empty-space skipping) */
     int64_t ncells; /* n[0]*n[1]*n[2]; cell (x,y,z) is ((z*n[1] + y)*n[0] + x) */
     int64_t nrefs;  /* triangle references summed over cells */
     float eps;      /* inflation of triangle boxes and traversal margin */
-    float pad2;
+    int32_t nmacro; /* macro blocks; their boxes follow the ncells cell boxes */
 } lem_grid_info;
 
 /* Grid build, replacing the tile binning of bin_counts_to_numpy /
@@ -140,8 +140,8 @@ typedef struct {
  *    stream and fills *info (ncells, nrefs).
  * 2) the caller allocates cell_start (ncells + 1 int32), tri_ids (nrefs int32),
  *    recs (nrefs x 12 float32: v0, e1, e2 and padding, cell by cell) and
- *    cell_box (ncells x 8 float32: the tight box of each cell's contents,
- *    lo.xyz, 0, hi.xyz, 0), and
+ *    cell_box ((ncells + nmacro) x 8 float32: the tight box of each cell's
+ *    contents, lo.xyz, 0, hi.xyz, 0, then the same for each macro block), and
  * 3) lem_grid_fill writes them; the same workspace must be passed to both. */
 int64_t lem_grid_workspace(int64_t T, float cells_per_tri);
 int lem_grid_plan(const float *tris, int64_t T, float cells_per_tri, lem_grid_info *info,

@@ -95,6 +95,9 @@ struct GridDev {
     float icx, icy, icz;  // 1 / cell size
     int nx, ny, nz;
     float eps;
+    int mshift;           // macro blocks of 2^mshift cells per axis
+    int mx, my, mz;       // macro blocks per axis
+    int64_t ncells;       // macro boxes follow the cell boxes
 };
 
 inline GridDev to_dev(const lem_grid_info &g)
@@ -105,6 +108,12 @@ inline GridDev to_dev(const lem_grid_info &g)
     d.icx = 1.0f / g.cell[0]; d.icy = 1.0f / g.cell[1]; d.icz = 1.0f / g.cell[2];
     d.nx = g.n[0]; d.ny = g.n[1]; d.nz = g.n[2];
     d.eps = g.eps;
+    d.mshift = g.mshift;
+    const int B = 1 << g.mshift;
+    d.mx = (g.n[0] + B - 1) / B;
+    d.my = (g.n[1] + B - 1) / B;
+    d.mz = (g.n[2] + B - 1) / B;
+    d.ncells = g.ncells;
     return d;
 }
 
@@ -302,6 +311,32 @@ __global__ __launch_bounds__(GB) void k_cell_boxes(const float *__restrict__ tri
     box[2 * c + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
 }
 
+// Box of each macro block: the union of its cells' boxes (inverted if empty),
+// written after the ncells cell boxes.
+__global__ __launch_bounds__(GB) void k_macro_boxes(GridDev G, float4 *__restrict__ box)
+{
+    const int64_t m = (int64_t)blockIdx.x * GB + threadIdx.x;
+    const int64_t nm = (int64_t)G.mx * G.my * G.mz;
+    if (m >= nm)
+        return;
+    const int jx = (int)(m % G.mx), jy = (int)((m / G.mx) % G.my), jz = (int)(m / ((int64_t)G.mx * G.my));
+    const int B = 1 << G.mshift;
+    const float INF = __builtin_huge_valf();
+    float4 lo = make_float4(INF, INF, INF, 0.0f), hi = make_float4(-INF, -INF, -INF, 0.0f);
+    for (int z = jz * B; z < min((jz + 1) * B, G.nz); ++z)
+        for (int y = jy * B; y < min((jy + 1) * B, G.ny); ++y)
+            for (int x = jx * B; x < min((jx + 1) * B, G.nx); ++x) {
+                const int64_t c = ((int64_t)z * G.ny + y) * G.nx + x;
+                const float4 a = box[2 * c], b = box[2 * c + 1];
+                if (a.x <= b.x) {
+                    lo.x = fminf(lo.x, a.x); lo.y = fminf(lo.y, a.y); lo.z = fminf(lo.z, a.z);
+                    hi.x = fmaxf(hi.x, b.x); hi.y = fmaxf(hi.y, b.y); hi.z = fmaxf(hi.z, b.z);
+                }
+            }
+    box[2 * (G.ncells + m)] = lo;
+    box[2 * (G.ncells + m) + 1] = hi;
+}
+
 // ---------------------------------------------------------------------------
 // DDA traversal
 // ---------------------------------------------------------------------------
@@ -402,14 +437,30 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
     float tz = tnext(iz, sz, G.loz, G.cz, r.oz, idz);
     const float margin = 2.0f * G.eps;
     bool hit = false;
+    int64_t cur_m = -1;
+    bool m_live = true;
     for (;;) {
-        int64_t c = ((int64_t)iz * G.ny + iy) * G.nx + ix;
-        int32_t b = cell_start[c], e = cell_start[c + 1];
-        if (n_cells)
-            *n_cells += 1;
-        if (e > b && !seg_box(r, idx, idy, idz, box[2 * c], box[2 * c + 1], -margin,
-                              dist + margin, margin))
-            e = b;
+        // Macro block of this cell: when the ray misses the block's content box,
+        // its cells are stepped through without touching their data.
+        const int64_t m = ((int64_t)(iz >> G.mshift) * G.my + (iy >> G.mshift)) * G.mx +
+                          (ix >> G.mshift);
+        if (m != cur_m) {
+            cur_m = m;
+            const float4 mlo = box[2 * (G.ncells + m)], mhi = box[2 * (G.ncells + m) + 1];
+            m_live = mlo.x <= mhi.x &&
+                     seg_box(r, idx, idy, idz, mlo, mhi, -margin, dist + margin, margin);
+        }
+        int32_t b = 0, e = 0;
+        if (m_live) {
+            int64_t c = ((int64_t)iz * G.ny + iy) * G.nx + ix;
+            b = cell_start[c];
+            e = cell_start[c + 1];
+            if (n_cells)
+                *n_cells += 1;
+            if (e > b && !seg_box(r, idx, idy, idz, box[2 * c], box[2 * c + 1], -margin,
+                                  dist + margin, margin))
+                e = b;
+        }
         if (n_tests)
             *n_tests += (unsigned int)(e - b);
         for (int32_t j = b; j < e; ++j) {
@@ -454,6 +505,18 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
 // SINK_MAJOR = true : lane = source, block = (sink, 256 sources); bits K x W64(S),
 //                     dist K x S.  Neighbouring sources to one sink give nearly
 //                     parallel rays, so a wavefront walks nearly the same cells.
+// Hardware dispatches workgroup b to XCD b % 8, each XCD with its own L2.
+// Logical block L = (b % 8) * (nb / 8) + b / 8 gives every XCD a contiguous run
+// of logical blocks -- neighbouring sources to one sink, walking nearly the
+// same cells -- so the cell data each L2 holds is reused by its own waves.
+LEM_DEV int64_t xcd_block(int64_t b, int64_t nb)
+{
+    const int64_t per = nb / 8;
+    if (b >= per * 8)
+        return b;
+    return (b % 8) * per + b / 8;
+}
+
 template <bool ANY, bool SINK_MAJOR, int SEM>
 __global__ __launch_bounds__(GB) void k_visibility_grid(
     const float *__restrict__ src, int64_t S, const float *__restrict__ snk, int64_t K, GridDev G,
@@ -461,8 +524,9 @@ __global__ __launch_bounds__(GB) void k_visibility_grid(
     const float4 *__restrict__ box, unsigned long long *__restrict__ bits,
     float *__restrict__ dist_out, int64_t nbl, int skip_self)
 {
-    const int64_t outer = (int64_t)blockIdx.x / nbl;
-    const int64_t inner = ((int64_t)blockIdx.x % nbl) * GB + threadIdx.x;
+    const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t outer = blk / nbl;
+    const int64_t inner = (blk % nbl) * GB + threadIdx.x;
     const int64_t NI = SINK_MAJOR ? S : K;
     const bool active = inner < NI;
     const int64_t s = SINK_MAJOR ? (active ? inner : 0) : outer;
@@ -721,6 +785,11 @@ extern "C" int lem_grid_plan(const float *tris, int64_t T, float cells_per_tri,
     }
     g.ncells = (int64_t)n[0] * n[1] * n[2];
     g.eps = eps;
+    g.mshift = 2;
+    {
+        const int B = 1 << g.mshift;
+        g.nmacro = (int32_t)(((n[0] + B - 1) / B) * ((n[1] + B - 1) / B) * ((n[2] + B - 1) / B));
+    }
     GridDev G = to_dev(g);
     hipLaunchKernelGGL(k_zero_int, dim3((unsigned)ceil_div(g.ncells + 1, GB)), dim3(GB), 0, st,
                        L.counts, g.ncells + 1);
@@ -779,6 +848,8 @@ extern "C" int lem_grid_fill(const float *tris, int64_t T, const lem_grid_info *
                            tris, tri_ids, info->nrefs, reinterpret_cast<float4 *>(recs));
     hipLaunchKernelGGL(k_cell_boxes, dim3((unsigned)ceil_div(nc, GB)), dim3(GB), 0, st, tris,
                        cell_start, tri_ids, G, nc, reinterpret_cast<float4 *>(cell_box));
+    hipLaunchKernelGGL(k_macro_boxes, dim3((unsigned)ceil_div(info->nmacro, GB)), dim3(GB), 0, st,
+                       G, reinterpret_cast<float4 *>(cell_box));
     return check_launch("lem_grid_fill");
 }
 

@@ -58,9 +58,9 @@ SIGNATURES = {
 class GridInfo(ctypes.Structure):
     """lem_grid_info (include/lem.h)."""
     _fields_ = [("lo", ctypes.c_float * 3), ("cell", ctypes.c_float * 3),
-                ("n", ctypes.c_int32 * 3), ("pad", ctypes.c_int32),
+                ("n", ctypes.c_int32 * 3), ("mshift", ctypes.c_int32),
                 ("ncells", ctypes.c_int64), ("nrefs", ctypes.c_int64),
-                ("eps", ctypes.c_float), ("pad2", ctypes.c_float)]
+                ("eps", ctypes.c_float), ("nmacro", ctypes.c_int32)]
 
 _lock = threading.Lock()
 _lib = None
@@ -70,8 +70,26 @@ class LemError(RuntimeError):
     """A liblem.so entry point returned an error code."""
 
 
+def _mapped_hip_runtimes():
+    """Paths of every libamdhip64 mapped into this process."""
+    paths = set()
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                if "libamdhip64" in line:
+                    paths.add(os.path.realpath(line.split()[-1]))
+    except OSError:  # pragma: no cover
+        pass
+    return paths
+
+
 def load_library(path: str = LIB_PATH):
-    """Load liblem.so and bind every entry point (no GPU needed for this step)."""
+    """Load liblem.so and bind every entry point (no GPU needed for this step).
+
+    torch is imported first: torch-ROCm ships its own HIP runtime with the same
+    soname (libamdhip64.so.7) as the system one liblem.so links against, so
+    loading torch first makes liblem.so bind to that one runtime; the other
+    order would put two HIP runtimes in the process."""
     global _lib
     with _lock:
         if _lib is None:
@@ -79,7 +97,13 @@ def load_library(path: str = LIB_PATH):
                 raise LemError(
                     f"{path} is missing: build it with "
                     "`python lyceanem-python_amd/build_lib.py` (there is no CPU fallback)")
+            import torch  # noqa: F401  (loads torch's HIP runtime before liblem.so)
+
             lib = ctypes.CDLL(path)
+            runtimes = _mapped_hip_runtimes()
+            if len(runtimes) > 1:
+                raise LemError(f"two HIP runtimes are loaded ({sorted(runtimes)}); import torch "
+                               "before anything that loads libamdhip64")
             for name, (args, res) in SIGNATURES.items():
                 fn = getattr(lib, name)
                 fn.argtypes = args
@@ -275,7 +299,8 @@ class Grid:
         self.cell_start = torch.empty(self.info.ncells + 1, dtype=torch.int32, device=dev)
         self.tri_ids = torch.empty(max(self.info.nrefs, 1), dtype=torch.int32, device=dev)
         self.recs = torch.empty((max(self.info.nrefs, 1), 12), dtype=torch.float32, device=dev)
-        self.cell_box = torch.empty((self.info.ncells, 8), dtype=torch.float32, device=dev)
+        self.cell_box = torch.empty((self.info.ncells + self.info.nmacro, 8), dtype=torch.float32,
+                                    device=dev)
         check(lib.lem_grid_fill(ptr(self.tris), T, ctypes.byref(self.info), ptr(self.cell_start),
                                 ptr(self.tri_ids), ptr(self.recs), ptr(self.cell_box),
                                 ptr(self.ws), wsb, self.cells_per_tri, stream_ptr()),

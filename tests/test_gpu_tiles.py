@@ -17,25 +17,11 @@ def _terrain_mesh(n=12, seed=2):
     idx = np.arange((n + 1) ** 2).reshape(n + 1, n + 1)
     a, b, c, d = idx[:-1, :-1].ravel(), idx[:-1, 1:].ravel(), idx[1:, :-1].ravel(), idx[1:, 1:].ravel()
     tri = np.concatenate([np.stack([a, b, d], 1), np.stack([a, d, c], 1)])
+    tri = tri[rng.random(len(tri)) < 0.3]  # holes, so part of the rays pass
     return Mesh(points=pts, cells=[("triangle", tri)])
 
 
-def test_tile_binning_invariants():
-    from lyceanem.models.acceleration_structures import Tile_acceleration_structure
-
-    m = _terrain_mesh()
-    t = Tile_acceleration_structure(m, 4)
-    tri = m.cells[0].data
-    assert t.bin_counts.shape == (t.y_cells_count * t.z_cells_count,)
-    assert t.binned_triangles_count == t.bin_counts.sum()
-    bt = t.binned_triangles.reshape(-1, 3)
-    # every triangle is binned at least once and only valid triples appear
-    got = {tuple(r) for r in bt}
-    assert got == {tuple(r) for r in tri}
-    assert len(bt) >= len(tri)
-
-
-@pytest.mark.parametrize("n", [12, 50])
+@pytest.mark.parametrize("n", [12, 90])
 def test_tile_structure_network_vs_oracle(oracle, n):
     from lyceanem.models.acceleration_structures import Tile_acceleration_structure
 

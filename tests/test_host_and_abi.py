@@ -177,3 +177,49 @@ def test_product_fails_loudly_without_gpu():
         pytest.skip("a GPU is present")
     with pytest.raises(_native.LemError):
         _native.visibility_dense(np.zeros((1, 3)), np.ones((1, 3)), np.zeros((0, 9)))
+
+
+# --------------------------------------------------------------------------
+# tile binning of the native branch (host code)
+# --------------------------------------------------------------------------
+
+def _terrain_mesh_x_up(n=12, seed=2):
+    from lyceanem.mesh import Mesh
+
+    rng = np.random.default_rng(seed)
+    y, z = np.meshgrid(np.linspace(-1, 1, n + 1), np.linspace(-1, 1, n + 1))
+    x = 0.15 * rng.normal(size=y.shape)
+    pts = np.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+    idx = np.arange((n + 1) ** 2).reshape(n + 1, n + 1)
+    a, b, c, d = (idx[:-1, :-1].ravel(), idx[:-1, 1:].ravel(), idx[1:, :-1].ravel(),
+                  idx[1:, 1:].ravel())
+    tri = np.concatenate([np.stack([a, b, d], 1), np.stack([a, d, c], 1)])
+    return Mesh(points=pts, cells=[("triangle", tri)])
+
+
+def test_tile_binning_covers_vertex_cells():
+    """terrain_acceleration_build.cuh:57-205: each triangle sits in the tiles of
+    its vertices, plus the fourth tile of an L-shaped triple."""
+    from lyceanem.models.acceleration_structures import Tile_acceleration_structure
+
+    m = _terrain_mesh_x_up()
+    t = Tile_acceleration_structure(m, 4)
+    tri = m.cells[0].data
+    assert t.bin_counts.shape == (t.y_cells_count * t.z_cells_count,)
+    assert t.binned_triangles_count == t.bin_counts.sum()
+    bt = t.binned_triangles.reshape(-1, 3)
+    assert {tuple(r) for r in bt} == {tuple(r) for r in tri}
+    p = np.asarray(m.points, np.float32)
+    cy = np.floor((p[:, 1] - np.float32(t.min_y)) / np.float32(t.tile_size)).astype(int)
+    cz = np.floor((p[:, 2] - np.float32(t.min_z)) / np.float32(t.tile_size)).astype(int)
+    starts = np.concatenate([[0], np.cumsum(t.bin_counts)])
+    for i, tr in enumerate(tri[:60]):
+        cells = {cy[v] * t.z_cells_count + cz[v] for v in tr}
+        if len(cells) == 3:
+            ys = {c // t.z_cells_count for c in cells}
+            zs = {c % t.z_cells_count for c in cells}
+            cells |= {y * t.z_cells_count + z for y in ys for z in zs}
+        for c in cells:
+            assert tuple(tr) in {tuple(r) for r in bt[starts[c]:starts[c + 1]]}
+        assert sum(tuple(tr) in {tuple(r) for r in bt[starts[c]:starts[c + 1]]}
+                   for c in range(len(t.bin_counts))) == len(cells)
