@@ -186,10 +186,13 @@ class C2:
     def record(self, times_ms):
         vis = float(np.mean([a for a, _ in times_ms]))
         sub = self.snk[:: self.stats_stride].contiguous()
-        tests, cells, blocked = self.nat.grid_stats(self.grid, self.src, sub)
+        tests, cells, blocked, macros, conservative = self.nat.grid_stats(self.grid, self.src,
+                                                                         sub)
         scale = self.K / sub.shape[0]
-        tests_all = tests * scale
-        achieved = tests_all * MT_FLOPS / (vis * 1e-3)
+        # algorithmic work: the MT tests of a conservative front-to-back DDA up to
+        # the first hit (every reference of every crossed cell), 46 flops each
+        work_all = conservative * scale
+        achieved = work_all * MT_FLOPS / (vis * 1e-3)
         bytes_out = self.bits.numel() * 8 + self.dist.numel() * 4
         return {
             "kernel": "k_visibility_grid<closest-hit, sink-major>",
@@ -200,8 +203,10 @@ class C2:
                 "grid_build_s": self.build_s,
                 "grid_cells": [int(x) for x in self.grid.info.n],
                 "grid_refs": int(self.grid.info.nrefs),
+                "conservative_mt_tests_per_ray": conservative / (self.S * sub.shape[0]),
                 "mt_tests_per_ray": tests / (self.S * sub.shape[0]),
                 "cells_per_ray": cells / (self.S * sub.shape[0]),
+                "macro_blocks_per_ray": macros / (self.S * sub.shape[0]),
                 "blocked_fraction": blocked / (self.S * sub.shape[0]),
                 "work_counted_on": f"every {self.stats_stride}th sink ({sub.shape[0]} sinks), "
                                    f"scaled x{scale:g}",

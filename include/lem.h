@@ -139,7 +139,8 @@ typedef struct {
  * 1) lem_grid_plan: bounds, resolution and per-cell counts; synchronises the
  *    stream and fills *info (ncells, nrefs).
  * 2) the caller allocates cell_start (ncells + 1 int32), tri_ids (nrefs int32),
- *    recs (nrefs x 12 float32: v0, e1, e2 and padding, cell by cell) and
+ *    recs (nrefs x 12 float32: v0, e1, e2, the triangle id as int32 bits,
+ *    padding; cell by cell, ids ascending within a cell) and
  *    cell_box ((ncells + nmacro) x 8 float32: the tight box of each cell's
  *    contents, lo.xyz, 0, hi.xyz, 0, then the same for each macro block), and
  * 3) lem_grid_fill writes them; the same workspace must be passed to both. */
@@ -152,7 +153,11 @@ int lem_grid_fill(const float *tris, int64_t T, const lem_grid_info *info, int32
 
 /* Dense S x K visibility through the grid (DDA): the same masks (and, with
  * LEM_SEM_NUMBA, kernel1Dv3 distances) as lem_visibility_dense with the same
- * semantics.  Replaces the tile traversal raycast_tiles / DDA
+ * semantics.  kernel1Dv3's distance depends on the order of near-tied hits
+ * (its window shrinks to 2e-7 below each accepted hit); the grid replays that
+ * window in triangle-id order over the hits near the closest one, so distances
+ * match the brute-force order too (exact unless more than 8 hits chain within
+ * 2e-7 + one float32 ulp of each other).  Replaces the tile traversal raycast_tiles / DDA
  * (raycasting_accelerated.cuh:106-268, 458-482) for source -> sink rays.
  * Flags: LEM_VIS_ANY_HIT, LEM_VIS_SINK_MAJOR, LEM_VIS_SKIP_SELF.
  * LEM_SEM_NATIVE is any-hit (no distance). */
@@ -202,8 +207,11 @@ int lem_gw_mfma(const float *src, int64_t S, float pol_x, float pol_y, float pol
                 int64_t F, double alpha, const float *W, int64_t B, float *out, void *stream);
 
 /* Instrumented grid traversal for measurement: adds, over all S x K rays, the
- * Moller-Trumbore tests performed (stats[0]), cells visited (stats[1]) and
- * blocked rays (stats[2]) to the device uint64 array `stats` (zero it first).
+ * Moller-Trumbore tests performed (stats[0]), cells of live macro blocks visited
+ * (stats[1]), blocked rays (stats[2]), macro blocks entered (stats[3]) and the
+ * tests a conservative front-to-back DDA would perform up to the first hit (the
+ * triangle references of every crossed cell, no pruning; stats[4]) to the
+ * device uint64 array `stats` of 5 entries (zero it first).
  * Writes no masks.  No counterpart in the reference (it reports no work counts). */
 int lem_visibility_grid_stats(const float *src, int64_t S, const float *snk, int64_t K,
                               const lem_grid_info *info, const int32_t *cell_start,

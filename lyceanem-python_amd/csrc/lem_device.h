@@ -9,7 +9,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#ifndef LEM_DEV
 #define LEM_DEV __device__ __forceinline__
+#endif
 
 namespace lem {
 

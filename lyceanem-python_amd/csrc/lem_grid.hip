@@ -34,6 +34,11 @@ namespace lem {
 
 constexpr int GB = 256;
 
+// Macro blocks hold 2^LEM_GRID_MSHIFT cells per axis.
+#ifndef LEM_GRID_MSHIFT
+#define LEM_GRID_MSHIFT 2
+#endif
+
 // Order-preserving float <-> uint key (for atomic min / max of floats).
 LEM_DEV unsigned int fkey(float f)
 {
@@ -271,7 +276,7 @@ __global__ __launch_bounds__(GB) void k_gather(const float *__restrict__ tris,
     Tri t = make_tri(tris + 9 * (int64_t)ids[i]);
     recs[3 * i + 0] = make_float4(t.v0x, t.v0y, t.v0z, t.e1x);
     recs[3 * i + 1] = make_float4(t.e1y, t.e1z, t.e2x, t.e2y);
-    recs[3 * i + 2] = make_float4(t.e2z, 0.0f, 0.0f, 0.0f);
+    recs[3 * i + 2] = make_float4(t.e2z, __int_as_float(ids[i]), 0.0f, 0.0f);
 }
 
 // Tight box of each cell's contents: the union of its triangles' eps-inflated
@@ -341,6 +346,12 @@ __global__ __launch_bounds__(GB) void k_macro_boxes(GridDev G, float4 *__restric
 // DDA traversal
 // ---------------------------------------------------------------------------
 
+// Triangle id of record i (kept in the record's padding).
+LEM_DEV int32_t rec_id(const float4 *__restrict__ recs, int32_t i)
+{
+    return __builtin_bit_cast(int32_t, recs[3 * (int64_t)i + 2].y);
+}
+
 LEM_DEV Tri load_rec(const float4 *__restrict__ recs, int32_t i)
 {
     float4 a = recs[3 * (int64_t)i], b = recs[3 * (int64_t)i + 1], c = recs[3 * (int64_t)i + 2];
@@ -351,153 +362,369 @@ LEM_DEV Tri load_rec(const float4 *__restrict__ recs, int32_t i)
     return t;
 }
 
-// Casts one charged ray through the grid.  Returns the blocked flag and
-// leaves kernel1Dv3's `dist` in `dist` (closest-hit mode).
+// One slab of the segment-box test, without branches: the t interval in
+// which the ray is between lo and hi on this axis narrows [tn, tf].  A ray
+// parallel to the axis (par; its inverse direction is infinite) is inside the
+// slab for every t or for none.
+LEM_DEV void slab(bool par, float o, float lo, float hi, float id, float &tn, float &tf)
+{
+    const float INF = __builtin_huge_valf();
+    const float a = (lo - o) * id, b = (hi - o) * id;
+    const bool inside = o >= lo && o <= hi;
+    const float n = par ? (inside ? -INF : INF) : fminf(a, b);
+    const float f = par ? (inside ? INF : -INF) : fmaxf(a, b);
+    tn = fmaxf(tn, n);
+    tf = fminf(tf, f);
+}
+
 // True when the segment t in [t_lo, t_hi] of the ray meets the box (slab test
-// with a margin of eps in t).
+// with a margin of m in t).
 LEM_DEV bool seg_box(const Ray &r, float idx, float idy, float idz, float4 lo, float4 hi,
                      float t_lo, float t_hi, float m)
 {
     float tn = t_lo, tf = t_hi;
-    if (r.dx != 0.0f) {
-        float a = (lo.x - r.ox) * idx, b = (hi.x - r.ox) * idx;
-        tn = fmaxf(tn, fminf(a, b));
-        tf = fminf(tf, fmaxf(a, b));
-    } else if (r.ox < lo.x || r.ox > hi.x) {
-        return false;
-    }
-    if (r.dy != 0.0f) {
-        float a = (lo.y - r.oy) * idy, b = (hi.y - r.oy) * idy;
-        tn = fmaxf(tn, fminf(a, b));
-        tf = fminf(tf, fmaxf(a, b));
-    } else if (r.oy < lo.y || r.oy > hi.y) {
-        return false;
-    }
-    if (r.dz != 0.0f) {
-        float a = (lo.z - r.oz) * idz, b = (hi.z - r.oz) * idz;
-        tn = fmaxf(tn, fminf(a, b));
-        tf = fminf(tf, fmaxf(a, b));
-    } else if (r.oz < lo.z || r.oz > hi.z) {
-        return false;
-    }
+    slab(r.dx == 0.0f, r.ox, lo.x, hi.x, idx, tn, tf);
+    slab(r.dy == 0.0f, r.oy, lo.y, hi.y, idy, tn, tf);
+    slab(r.dz == 0.0f, r.oz, lo.z, hi.z, idz, tn, tf);
     return tn <= tf + m;
 }
 
+// Distance along the ray to the far boundary of cell i on one axis (s = step
+// direction, 0 when the ray is parallel to the axis), from the ray origin.
+LEM_DEV float tnext(int i, int s, float lo, float c, float o, float id)
+{
+    if (s == 0)
+        return __builtin_huge_valf();
+    float bnd = lo + c * (float)(i + (s > 0 ? 1 : 0));
+    return (bnd - o) * id;
+}
+
+// Start of a cell walk: the segment [0, len] clipped to the grid box
+// ([t0, t1]), the cell holding the clipped start, step directions and the
+// distances to the first boundary on each axis.
+struct Walk {
+    float idx, idy, idz;
+    float t1;
+    int ix, iy, iz, sx, sy, sz;
+    float tx, ty, tz;
+};
+
+LEM_DEV bool walk_begin(const Ray &r, const GridDev &G, Walk &w)
+{
+    float hx = G.lox + G.cx * G.nx, hy = G.loy + G.cy * G.ny, hz = G.loz + G.cz * G.nz;
+    w.idx = 1.0f / r.dx;
+    w.idy = 1.0f / r.dy;
+    w.idz = 1.0f / r.dz;
+    float t0 = 0.0f, t1 = r.len;
+    float a = (G.lox - r.ox) * w.idx, b = (hx - r.ox) * w.idx;
+    if (r.dx == 0.0f) {
+        if (r.ox < G.lox || r.ox > hx) return false;
+    } else {
+        t0 = fmaxf(t0, fminf(a, b));
+        t1 = fminf(t1, fmaxf(a, b));
+    }
+    a = (G.loy - r.oy) * w.idy; b = (hy - r.oy) * w.idy;
+    if (r.dy == 0.0f) {
+        if (r.oy < G.loy || r.oy > hy) return false;
+    } else {
+        t0 = fmaxf(t0, fminf(a, b));
+        t1 = fminf(t1, fmaxf(a, b));
+    }
+    a = (G.loz - r.oz) * w.idz; b = (hz - r.oz) * w.idz;
+    if (r.dz == 0.0f) {
+        if (r.oz < G.loz || r.oz > hz) return false;
+    } else {
+        t0 = fmaxf(t0, fminf(a, b));
+        t1 = fminf(t1, fmaxf(a, b));
+    }
+    if (!(t0 <= t1 + G.eps))
+        return false;
+    w.t1 = t1;
+    float pm = t0 > 0.0f ? t0 : 0.0f;
+    w.ix = clampi((int)floorf((r.ox + r.dx * pm - G.lox) * G.icx), 0, G.nx - 1);
+    w.iy = clampi((int)floorf((r.oy + r.dy * pm - G.loy) * G.icy), 0, G.ny - 1);
+    w.iz = clampi((int)floorf((r.oz + r.dz * pm - G.loz) * G.icz), 0, G.nz - 1);
+    w.sx = r.dx > 0.0f ? 1 : (r.dx < 0.0f ? -1 : 0);
+    w.sy = r.dy > 0.0f ? 1 : (r.dy < 0.0f ? -1 : 0);
+    w.sz = r.dz > 0.0f ? 1 : (r.dz < 0.0f ? -1 : 0);
+    w.tx = tnext(w.ix, w.sx, G.lox, G.cx, r.ox, w.idx);
+    w.ty = tnext(w.iy, w.sy, G.loy, G.cy, r.oy, w.idy);
+    w.tz = tnext(w.iz, w.sz, G.loz, G.cz, r.oz, w.idz);
+    return true;
+}
+
+// kernel1Dv3's distance (rayfunctions.py:212-244) is order dependent: it
+// tests triangles in index order and accepts a hit only below the current
+// distance minus 2e-7, storing the distance as float32.  Given the exact
+// closest hit t* (float64), only hits within a short chain of such windows
+// above t* can change the outcome; hits farther than delta = 8 (2e-7 + ulp(t*))
+// never reject one of them.  This replays the window from dist = len over
+// every hit in [t*, t* + delta], in triangle-id order, taken from the cells
+// the piece [t* - delta, t* + delta] of the ray passes through (each cell
+// lists its triangles by ascending id; triangle boxes are inflated by eps,
+// so each such hit's triangle is in one of these cells).
+LEM_DEV float replay_window(const Ray &r, double tstar, const GridDev &G,
+                            const int32_t *__restrict__ cell_start,
+                            const float4 *__restrict__ recs)
+{
+    const float fs = (float)tstar;
+    const float ulp = nextafterf(fs, __builtin_huge_valf()) - fs;
+    const double delta = 8.0 * (2e-7 + (double)ulp);
+    const double ta = tstar - delta > 0.0 ? tstar - delta : 0.0, tb = tstar + delta;
+    const float o[3] = {r.ox, r.oy, r.oz}, d[3] = {r.dx, r.dy, r.dz};
+    const float glo[3] = {G.lox, G.loy, G.loz}, gic[3] = {G.icx, G.icy, G.icz};
+    const int gn[3] = {G.nx, G.ny, G.nz};
+    int lo[3], hi[3];
+    int ncell = 1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float pa = o[a] + d[a] * (float)ta, pb = o[a] + d[a] * (float)tb;
+        const int ca = clampi((int)floorf((pa - glo[a]) * gic[a]), 0, gn[a] - 1);
+        const int cb = clampi((int)floorf((pb - glo[a]) * gic[a]), 0, gn[a] - 1);
+        lo[a] = min(ca, cb);
+        hi[a] = max(ca, cb);
+        ncell *= hi[a] - lo[a] + 1;
+    }
+    if (ncell > 64)
+        return fs;  // a piece across more than 64 cells: keep the closest hit
+    float D = r.len;
+    int last = -1;
+    for (int it = 0; it < 64; ++it) {
+        // next hit of the piece in id order
+        int best = 0x7fffffff;
+        double bt = 0.0;
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int x = lo[0]; x <= hi[0]; ++x) {
+                    const int c = (z * G.ny + y) * G.nx + x;
+                    for (int32_t j = cell_start[c]; j < cell_start[c + 1]; ++j) {
+                        const int id = rec_id(recs, j);
+                        if (id <= last)
+                            continue;
+                        if (id >= best)
+                            break;
+                        double tt;
+                        if (hit_numba(r, r.len, load_rec(recs, j), tt) && tt <= tb) {
+                            best = id;
+                            bt = tt;
+                            break;
+                        }
+                    }
+                }
+        if (best == 0x7fffffff)
+            break;
+        if (bt < (double)D - 2e-7)
+            D = (float)bt;
+        last = best;
+    }
+    return D;
+}
+
+// Casts one charged ray through the grid.  Returns the blocked flag and
+// leaves kernel1Dv3's `dist` in `dist` (closest-hit mode).
+//
+// Two-level walk.  Macro blocks (2^mshift cells per axis) are stepped with
+// their own boundary distances T*, taken from the same tnext formula at the
+// block's last cell, so a block face is the identical float32 value at both
+// levels.  A block whose content box the segment misses is left in one step;
+// inside a live block the cells are walked one by one and each cell's
+// triangles are tested when the segment meets the cell's box.  On entering a
+// live block after skipping, the entry cell on the two axes not crossed comes
+// from the entry point (clamped into the block; a cell off by one at an edge
+// or corner only shifts the walk within the eps inflation of the boxes).
 template <bool ANY, int SEM = 0>
 LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
                       const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
                       const float4 *__restrict__ box, unsigned int *n_tests = nullptr,
-                      unsigned int *n_cells = nullptr)
+                      unsigned int *n_cells = nullptr, unsigned int *n_macro = nullptr)
 {
-    const float INF = __builtin_huge_valf();
-    // clip the segment [0, len] to the grid box
-    float hx = G.lox + G.cx * G.nx, hy = G.loy + G.cy * G.ny, hz = G.loz + G.cz * G.nz;
-    float idx = 1.0f / r.dx, idy = 1.0f / r.dy, idz = 1.0f / r.dz;
-    float t0 = 0.0f, t1 = r.len;
-    {
-        float a = (G.lox - r.ox) * idx, b = (hx - r.ox) * idx;
-        if (r.dx == 0.0f) {
-            if (r.ox < G.lox || r.ox > hx) return false;
-        } else {
-            t0 = fmaxf(t0, fminf(a, b));
-            t1 = fminf(t1, fmaxf(a, b));
-        }
-        a = (G.loy - r.oy) * idy; b = (hy - r.oy) * idy;
-        if (r.dy == 0.0f) {
-            if (r.oy < G.loy || r.oy > hy) return false;
-        } else {
-            t0 = fmaxf(t0, fminf(a, b));
-            t1 = fminf(t1, fmaxf(a, b));
-        }
-        a = (G.loz - r.oz) * idz; b = (hz - r.oz) * idz;
-        if (r.dz == 0.0f) {
-            if (r.oz < G.loz || r.oz > hz) return false;
-        } else {
-            t0 = fmaxf(t0, fminf(a, b));
-            t1 = fminf(t1, fmaxf(a, b));
-        }
-    }
-    if (!(t0 <= t1 + G.eps))
+    Walk w;
+    if (!walk_begin(r, G, w))
         return false;
-    float pm = t0 > 0.0f ? t0 : 0.0f;
-    int ix = clampi((int)floorf((r.ox + r.dx * pm - G.lox) * G.icx), 0, G.nx - 1);
-    int iy = clampi((int)floorf((r.oy + r.dy * pm - G.loy) * G.icy), 0, G.ny - 1);
-    int iz = clampi((int)floorf((r.oz + r.dz * pm - G.loz) * G.icz), 0, G.nz - 1);
-    const int sx = r.dx > 0.0f ? 1 : (r.dx < 0.0f ? -1 : 0);
-    const int sy = r.dy > 0.0f ? 1 : (r.dy < 0.0f ? -1 : 0);
-    const int sz = r.dz > 0.0f ? 1 : (r.dz < 0.0f ? -1 : 0);
-    auto tnext = [&](int i, int s, float lo, float c, float o, float id) -> float {
-        if (s == 0)
-            return INF;
-        float bnd = lo + c * (float)(i + (s > 0 ? 1 : 0));
-        return (bnd - o) * id;
-    };
-    float tx = tnext(ix, sx, G.lox, G.cx, r.ox, idx);
-    float ty = tnext(iy, sy, G.loy, G.cy, r.oy, idy);
-    float tz = tnext(iz, sz, G.loz, G.cz, r.oz, idz);
+    const float idx = w.idx, idy = w.idy, idz = w.idz, t1 = w.t1;
+    const int sx = w.sx, sy = w.sy, sz = w.sz;
+    int ix = w.ix, iy = w.iy, iz = w.iz;
+    float tx = w.tx, ty = w.ty, tz = w.tz;
     const float margin = 2.0f * G.eps;
+    const int ms = G.mshift, B = 1 << ms;
+    const float4 *__restrict__ mbox = box + 2 * G.ncells;  // macro-block boxes
+    // last cell of block j along an axis in the direction of travel
+    auto last = [&](int j, int s, int n) { return s > 0 ? min((j << ms) + B - 1, n - 1) : (j << ms); };
+    int jx = ix >> ms, jy = iy >> ms, jz = iz >> ms;
+    float Tx = tnext(last(jx, sx, G.nx), sx, G.lox, G.cx, r.ox, idx);
+    float Ty = tnext(last(jy, sy, G.ny), sy, G.loy, G.cy, r.oy, idy);
+    float Tz = tnext(last(jz, sz, G.nz), sz, G.loz, G.cz, r.oz, idz);
     bool hit = false;
-    int64_t cur_m = -1;
-    bool m_live = true;
-    for (;;) {
-        // Macro block of this cell: when the ray misses the block's content box,
-        // its cells are stepped through without touching their data.
-        const int64_t m = ((int64_t)(iz >> G.mshift) * G.my + (iy >> G.mshift)) * G.mx +
-                          (ix >> G.mshift);
-        if (m != cur_m) {
-            cur_m = m;
-            const float4 mlo = box[2 * (G.ncells + m)], mhi = box[2 * (G.ncells + m) + 1];
-            m_live = mlo.x <= mhi.x &&
-                     seg_box(r, idx, idy, idz, mlo, mhi, -margin, dist + margin, margin);
-        }
-        int32_t b = 0, e = 0;
-        if (m_live) {
-            int64_t c = ((int64_t)iz * G.ny + iy) * G.nx + ix;
-            b = cell_start[c];
-            e = cell_start[c + 1];
-            if (n_cells)
-                *n_cells += 1;
-            if (e > b && !seg_box(r, idx, idy, idz, box[2 * c], box[2 * c + 1], -margin,
-                                  dist + margin, margin))
-                e = b;
-        }
-        if (n_tests)
-            *n_tests += (unsigned int)(e - b);
-        for (int32_t j = b; j < e; ++j) {
-            Tri t = load_rec(recs, j);
-            if (SEM == 1) {
-                if (hit_native(r, t))
-                    return true;
-                continue;
+    double tmin = (double)r.len;  // closest hit so far (closest-hit mode)
+    bool fine = true;   // (ix, iy, iz) is the walk's cell in block (jx, jy, jz)
+    int entry_axis = 0;
+    float t_entry = 0.0f;
+    // Every iteration moves a block index one step in the direction of travel
+    // or walks cells inside one block (at most 3 * 2^mshift steps); block indices
+    // never move back, so the walk ends; the caps only guard the impossible.
+    for (int guard = G.mx + G.my + G.mz + 4; guard > 0; --guard) {
+        const int m = (jz * G.my + jy) * G.mx + jx;
+        const float4 mlo = mbox[2 * m], mhi = mbox[2 * m + 1];
+        // both halves of the box are loaded before either is tested
+        const bool live = (mlo.x <= mhi.x) &
+                          seg_box(r, idx, idy, idz, mlo, mhi, -margin, dist + margin, margin);
+        if (n_macro)
+            *n_macro += 1;
+        if (!live) {
+            const int a = (Tx <= Ty && Tx <= Tz) ? 0 : (Ty <= Tz ? 1 : 2);
+            const float tex = a == 0 ? Tx : (a == 1 ? Ty : Tz);
+            if (tex >= t1 + margin || (hit && dist + margin <= tex))
+                break;
+            if (a == 0) {
+                jx += sx;
+                if (jx < 0 || jx >= G.mx) break;
+                Tx = tnext(last(jx, sx, G.nx), sx, G.lox, G.cx, r.ox, idx);
+            } else if (a == 1) {
+                jy += sy;
+                if (jy < 0 || jy >= G.my) break;
+                Ty = tnext(last(jy, sy, G.ny), sy, G.loy, G.cy, r.oy, idy);
+            } else {
+                jz += sz;
+                if (jz < 0 || jz >= G.mz) break;
+                Tz = tnext(last(jz, sz, G.nz), sz, G.loz, G.cz, r.oz, idz);
             }
-            double tt;
-            if (hit_numba(r, dist, t, tt) && tt < (double)dist) {
-                dist = (float)tt;
-                hit = true;
-                if (ANY)
-                    return true;
-            }
+            entry_axis = a;
+            t_entry = tex;
+            fine = false;
+            continue;
         }
-        float texit = fminf(tx, fminf(ty, tz));
-        if (hit && dist + margin <= texit)
-            break;
-        if (texit >= t1 + margin)
-            break;
-        if (tx <= ty && tx <= tz) {
-            ix += sx;
-            if (ix < 0 || ix >= G.nx) break;
+        if (!fine) {
+            // first cell of the block: the crossed axis enters at the block's
+            // first cell in the direction of travel, the others at the entry point
+            const int bx = jx << ms, by = jy << ms, bz = jz << ms;
+            const int ex = min(bx + B - 1, G.nx - 1), ey = min(by + B - 1, G.ny - 1),
+                      ez = min(bz + B - 1, G.nz - 1);
+            const float px = r.ox + r.dx * t_entry, py = r.oy + r.dy * t_entry,
+                        pz = r.oz + r.dz * t_entry;
+            ix = clampi((int)floorf((px - G.lox) * G.icx), bx, ex);
+            iy = clampi((int)floorf((py - G.loy) * G.icy), by, ey);
+            iz = clampi((int)floorf((pz - G.loz) * G.icz), bz, ez);
+            if (entry_axis == 0)
+                ix = sx > 0 ? bx : ex;
+            else if (entry_axis == 1)
+                iy = sy > 0 ? by : ey;
+            else
+                iz = sz > 0 ? bz : ez;
             tx = tnext(ix, sx, G.lox, G.cx, r.ox, idx);
-        } else if (ty <= tz) {
-            iy += sy;
-            if (iy < 0 || iy >= G.ny) break;
             ty = tnext(iy, sy, G.loy, G.cy, r.oy, idy);
-        } else {
-            iz += sz;
-            if (iz < 0 || iz >= G.nz) break;
             tz = tnext(iz, sz, G.loz, G.cz, r.oz, idz);
+            fine = true;
+        }
+        bool done = false;
+        for (int g = 3 * B + 3; g > 0; --g) {
+            int32_t b, e;
+            {
+                const int c = (iz * G.ny + iy) * G.nx + ix;
+                b = cell_start[c];
+                e = cell_start[c + 1];
+                if (n_cells)
+                    *n_cells += 1;
+                if (e > b && !seg_box(r, idx, idy, idz, box[2 * c], box[2 * c + 1], -margin,
+                                      dist + margin, margin))
+                    e = b;
+            }
+            if (n_tests)
+                *n_tests += (unsigned int)(e - b);
+            for (int32_t j = b; j < e; ++j) {
+                Tri t = load_rec(recs, j);
+                if (SEM == 1) {
+                    if (hit_native(r, t))
+                        return true;
+                    continue;
+                }
+                double tt;
+                if (ANY) {
+                    if (hit_numba(r, dist, t, tt)) {
+                        dist = (float)tt;
+                        return true;
+                    }
+                } else if (hit_numba(r, r.len, t, tt) && tt < tmin) {
+                    // exact closest hit; the order-dependent distance is
+                    // replayed once the walk is over
+                    tmin = tt;
+                    dist = (float)tt;
+                    hit = true;
+                }
+            }
+            const float texit = fminf(tx, fminf(ty, tz));
+            if ((hit && dist + margin <= texit) || texit >= t1 + margin) {
+                done = true;
+                break;
+            }
+            if (tx <= ty && tx <= tz) {
+                ix += sx;
+                if (ix < 0 || ix >= G.nx) { done = true; break; }
+                tx = tnext(ix, sx, G.lox, G.cx, r.ox, idx);
+                if ((ix >> ms) != jx) {
+                    jx = ix >> ms;
+                    Tx = tnext(last(jx, sx, G.nx), sx, G.lox, G.cx, r.ox, idx);
+                    break;
+                }
+            } else if (ty <= tz) {
+                iy += sy;
+                if (iy < 0 || iy >= G.ny) { done = true; break; }
+                ty = tnext(iy, sy, G.loy, G.cy, r.oy, idy);
+                if ((iy >> ms) != jy) {
+                    jy = iy >> ms;
+                    Ty = tnext(last(jy, sy, G.ny), sy, G.loy, G.cy, r.oy, idy);
+                    break;
+                }
+            } else {
+                iz += sz;
+                if (iz < 0 || iz >= G.nz) { done = true; break; }
+                tz = tnext(iz, sz, G.loz, G.cz, r.oz, idz);
+                if ((iz >> ms) != jz) {
+                    jz = iz >> ms;
+                    Tz = tnext(last(jz, sz, G.nz), sz, G.loz, G.cz, r.oz, idz);
+                    break;
+                }
+            }
+        }
+        if (done)
+            break;
+    }
+    if (!ANY && SEM == 0 && hit)
+        dist = replay_window(r, tmin, G, cell_start, recs);
+    return hit;
+}
+
+// Work of a conservative front-to-back DDA: the triangle references of every
+// cell the segment crosses from its start up to the cell holding the first hit
+// at distance t_stop (no box or macro-block pruning).  Measurement only.
+LEM_DEV unsigned int conservative_refs(const Ray &r, float t_stop, const GridDev &G,
+                                       const int32_t *__restrict__ cell_start)
+{
+    Walk w;
+    if (!walk_begin(r, G, w))
+        return 0;
+    const float t_end = fminf(w.t1, t_stop);
+    unsigned int n = 0;
+    for (int guard = G.nx + G.ny + G.nz + 4; guard > 0; --guard) {
+        const int c = (w.iz * G.ny + w.iy) * G.nx + w.ix;
+        n += (unsigned int)(cell_start[c + 1] - cell_start[c]);
+        if (w.tx <= w.ty && w.tx <= w.tz) {
+            if (w.tx >= t_end) break;
+            w.ix += w.sx;
+            if (w.ix < 0 || w.ix >= G.nx) break;
+            w.tx = tnext(w.ix, w.sx, G.lox, G.cx, r.ox, w.idx);
+        } else if (w.ty <= w.tz) {
+            if (w.ty >= t_end) break;
+            w.iy += w.sy;
+            if (w.iy < 0 || w.iy >= G.ny) break;
+            w.ty = tnext(w.iy, w.sy, G.loy, G.cy, r.oy, w.idy);
+        } else {
+            if (w.tz >= t_end) break;
+            w.iz += w.sz;
+            if (w.iz < 0 || w.iz >= G.nz) break;
+            w.tz = tnext(w.iz, w.sz, G.loz, G.cz, r.oz, w.idz);
         }
     }
-    return hit;
+    return n;
 }
 
 // Dense S x K through the grid.
@@ -558,25 +785,30 @@ __global__ __launch_bounds__(GB) void k_grid_stats(
 {
     const int64_t k = (int64_t)blockIdx.x / nbl;
     const int64_t s = ((int64_t)blockIdx.x % nbl) * GB + threadIdx.x;
-    unsigned int nt = 0, nc = 0;
+    unsigned int nt = 0, nc = 0, nm = 0, nw = 0;
     bool hit = false;
     if (s < S) {
         Ray r = charge(src[3 * s], src[3 * s + 1], src[3 * s + 2], snk[3 * k], snk[3 * k + 1],
                        snk[3 * k + 2]);
         float dist = r.len;
-        hit = dda_cast<ANY>(r, dist, G, cell_start, recs, box, &nt, &nc);
+        hit = dda_cast<ANY>(r, dist, G, cell_start, recs, box, &nt, &nc, &nm);
+        nw = conservative_refs(r, dist, G, cell_start);
     }
     // wave sums, one atomic per wave
-    unsigned long long t64 = nt, c64 = nc, h64 = hit ? 1ull : 0ull;
+    unsigned long long t64 = nt, c64 = nc, h64 = hit ? 1ull : 0ull, m64 = nm, w64 = nw;
     for (int off = 32; off > 0; off >>= 1) {
         t64 += __shfl_down(t64, off);
         c64 += __shfl_down(c64, off);
         h64 += __shfl_down(h64, off);
+        m64 += __shfl_down(m64, off);
+        w64 += __shfl_down(w64, off);
     }
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(&stats[0], t64);
         atomicAdd(&stats[1], c64);
         atomicAdd(&stats[2], h64);
+        atomicAdd(&stats[3], m64);
+        atomicAdd(&stats[4], w64);
     }
 }
 
@@ -785,7 +1017,7 @@ extern "C" int lem_grid_plan(const float *tris, int64_t T, float cells_per_tri,
     }
     g.ncells = (int64_t)n[0] * n[1] * n[2];
     g.eps = eps;
-    g.mshift = 2;
+    g.mshift = LEM_GRID_MSHIFT;
     {
         const int B = 1 << g.mshift;
         g.nmacro = (int32_t)(((n[0] + B - 1) / B) * ((n[1] + B - 1) / B) * ((n[2] + B - 1) / B));

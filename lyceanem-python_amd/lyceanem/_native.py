@@ -355,12 +355,13 @@ def field_los_grid(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, beta=2 *
 
 
 def grid_stats(grid, src, snk, any_hit=False):
-    """(MT tests, cells visited, blocked rays) of a sink-major grid traversal."""
+    """(MT tests, cells visited, blocked rays, macro blocks entered, conservative
+    DDA tests up to the first hit) of a sink-major grid traversal."""
     torch = _torch()
     lib = load_library()
     src = to_dev(src, torch.float32).reshape(-1, 3)
     snk = to_dev(snk, torch.float32).reshape(-1, 3)
-    stats = torch.zeros(3, dtype=torch.int64, device=src.device)
+    stats = torch.zeros(5, dtype=torch.int64, device=src.device)
     check(lib.lem_visibility_grid_stats(ptr(src), src.shape[0], ptr(snk), snk.shape[0],
                                         ctypes.byref(grid.info), ptr(grid.cell_start),
                                         ptr(grid.recs), ptr(grid.cell_box),

@@ -15,6 +15,22 @@ from .. import em
 from ..mesh import triangle_cells
 
 
+def source_chunks(S, chunk_count):
+    """Source row ranges of the chunked tile calls (acceleration_structures.py:
+    76-113): chunks of floor(S / chunk_count) rows, the last one running to S.
+    With self_to_self=False the native kernel skips pairs whose chunk-local source
+    row equals the sink row (raycasting_accelerated.cuh:61-66), so these bounds
+    decide which pairs are skipped."""
+    step = S // chunk_count
+    out, a, b = [], 0, step
+    for i in range(chunk_count):
+        if i == chunk_count - 1:
+            b = S
+        out.append((a, b))
+        a, b = b, b + step
+    return out
+
+
 class Tile_acceleration_structure:
     """(y, z) tile binning of a blocking mesh (acceleration_structures.py:8-115)."""
 
@@ -48,8 +64,7 @@ class Tile_acceleration_structure:
         S = source_mesh.points.shape[0]
         K = sink_mesh.points.shape[0]
         out = np.zeros((S, K, 3), dtype=np.complex64)
-        bounds = np.linspace(0, S, chunk_count + 1).astype(int)
-        for a, b in zip(bounds[:-1], bounds[1:]):
+        for a, b in source_chunks(S, chunk_count):
             if b <= a:
                 continue
             ex = np.asarray(source_mesh.point_data["ex"])[a:b]

@@ -223,3 +223,20 @@ def test_tile_binning_covers_vertex_cells():
             assert tuple(tr) in {tuple(r) for r in bt[starts[c]:starts[c + 1]]}
         assert sum(tuple(tr) in {tuple(r) for r in bt[starts[c]:starts[c + 1]]}
                    for c in range(len(t.bin_counts))) == len(cells)
+
+
+@pytest.mark.parametrize("S,chunks", [(40, 3), (10, 4), (3, 5), (7, 1), (8, 2), (0, 2)])
+def test_tile_source_chunks_follow_the_reference_loop(S, chunks):
+    """acceleration_structures.py:76-113 steps source_end by floor(S / chunks) and
+    sets it to S after the second-to-last chunk; a literal run of that loop."""
+    from lyceanem.models.acceleration_structures import source_chunks
+
+    exp, start, end = [], 0, int(np.floor(S / chunks))
+    for i in range(chunks):
+        exp.append((start, end))
+        start = end
+        end = int(end + np.floor(S / chunks))
+        if i == chunks - 2:
+            end = S
+    assert source_chunks(S, chunks) == exp
+    assert exp[-1][1] == S
