@@ -218,6 +218,48 @@ int lem_visibility_grid_stats(const float *src, int64_t S, const float *snk, int
                               const float *recs, const float *cell_box, int flags,
                               uint64_t *stats, void *stream);
 
+/* ---- time domain (models/time_domain.py, TimeDomainv3) ------------------- */
+
+/* Per path row of a 1-based index (V x ncol int32, 0 = unused column, the
+ * reference's negative-index wrap reproduced): the real polarisation vector
+ * carried along the path scaled by |product of the lossy_propagation factors|
+ * (amp, V x 3 float64) and the arrival time path length / c + the imaginary
+ * parts of the point weights (arrival, V float64).  Points: N x 3 float32 with
+ * normals, weights N x 3 complex64.  Replaces the per-path part of
+ * timedomainkernal (empropagation.py:1345-1497); electric sources only.
+ * A row whose first entry is 0 or with an entry outside [0, N] sets *err
+ * (device int32, zero it first) and gets NaN outputs. */
+int lem_time_paths(const float *pos, const float *nrm, const float *w, int64_t N,
+                   const int32_t *index, int64_t V, int ncol, double alpha, double beta,
+                   double *amp, double *arrival, int32_t *err, void *stream);
+
+/* *wake (device float64) = min(1 s, min arrival): the wake time of
+ * timedomainkernal's atomic minimum (empropagation.py:1490-1495), taken over
+ * every path (the reference's read after a block barrier is launch-order
+ * dependent). */
+int lem_time_wake(const double *arrival, int64_t V, double *wake, void *stream);
+
+/* Start sample of each path: (arrival - *wake) // (1 / sampling_freq) with
+ * Python's float floor division (empropagation.py:1503-1506), as int64. */
+int lem_time_bins(const double *arrival, int64_t V, const double *wake, double sampling_freq,
+                  int64_t *bin, void *stream);
+
+/* Time map per (source, sink) slot: out[slot][n][c] (float64, N samples) =
+ * sum over the slot's paths of excitation[n - bin] * amp[c] for
+ * 0 <= n - bin < E -- timedomainkernal's atomic placement
+ * (empropagation.py:1507-1600) as a gather in path order.  Paths grouped by
+ * slot as for lem_field_paths (order, seg_start[nseg + 1], seg_slot[nseg]);
+ * slots without paths are not written. */
+int lem_time_map(const double *amp, const int64_t *bin, const int64_t *order,
+                 const int64_t *seg_start, const int64_t *seg_slot, int64_t nseg,
+                 const double *excitation, int64_t E, int64_t N, double *out, void *stream);
+
+/* The time map summed over every path: out[n][c] (N x 3 float64), the source
+ * and sink sums of time_domain.calculate_scattering (time_domain.py:383-428),
+ * in path order. */
+int lem_time_trace(const double *amp, const int64_t *bin, int64_t V, const double *excitation,
+                   int64_t E, int64_t N, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

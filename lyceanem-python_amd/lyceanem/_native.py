@@ -45,6 +45,11 @@ SIGNATURES = {
     "lem_grid_fill": ([_P, _I64, _P, _P, _P, _P, _P, _P, _I64, ctypes.c_float, _P], _INT),
     "lem_visibility_grid": ([_P, _I64, _P, _I64, _P, _P, _P, _P, _INT, _INT, _P, _P, _P], _INT),
     "lem_visibility_grid_stats": ([_P, _I64, _P, _I64, _P, _P, _P, _P, _INT, _P, _P], _INT),
+    "lem_time_paths": ([_P, _P, _P, _I64, _P, _I64, _INT, _D, _D, _P, _P, _P, _P], _INT),
+    "lem_time_wake": ([_P, _I64, _P, _P], _INT),
+    "lem_time_bins": ([_P, _I64, _P, _D, _P, _P], _INT),
+    "lem_time_map": ([_P, _P, _P, _P, _P, _I64, _P, _I64, _I64, _P, _P], _INT),
+    "lem_time_trace": ([_P, _P, _I64, _P, _I64, _I64, _P, _P], _INT),
     "lem_field_los_grid": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _P, _P, _P, _D, _D, _INT, _P,
                             _P, _I64, _P], _INT),
     "lem_gw_mfma": ([_P, _I64, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P, _I64, _P,
@@ -276,6 +281,91 @@ def field_paths(pos, nrm, w, index, order, seg_start, seg_slot, S, K, alpha, bet
                               ptr(out), ptr(err), stream_ptr()), "lem_field_paths")
     if int(err.item()) != 0:
         raise LemError("lem_field_paths: path index points outside the point table")
+    return out
+
+
+def time_paths(pos, nrm, w, index, alpha, beta):
+    """Per path (amp (V, 3), arrival (V,)) float64 tensors of a 1-based index."""
+    torch = _torch()
+    lib = load_library()
+    pos = to_dev(pos, torch.float32).reshape(-1, 3)
+    nrm = to_dev(nrm, torch.float32).reshape(-1, 3)
+    N = pos.shape[0]
+    w = to_dev(w, torch.complex64).reshape(N, 3)
+    idx = to_dev(index, torch.int32)
+    if idx.dim() != 2:
+        raise LemError("time_paths: index must be a 2-D array")
+    V, ncol = idx.shape
+    amp = torch.empty((V, 3), dtype=torch.float64, device=pos.device)
+    arrival = torch.empty(V, dtype=torch.float64, device=pos.device)
+    err = torch.zeros(1, dtype=torch.int32, device=pos.device)
+    check(lib.lem_time_paths(ptr(pos), ptr(nrm), ptr(w), N, ptr(idx), V, ncol, float(alpha),
+                             float(beta), ptr(amp), ptr(arrival), ptr(err), stream_ptr()),
+          "lem_time_paths")
+    if V and int(err.item()) != 0:
+        raise LemError("lem_time_paths: path index points outside the point table")
+    return amp, arrival
+
+
+def time_wake(arrival):
+    """min(1 s, earliest arrival) as a (1,) float64 device tensor."""
+    torch = _torch()
+    lib = load_library()
+    arrival = to_dev(arrival, torch.float64).reshape(-1)
+    wake = torch.empty(1, dtype=torch.float64, device=arrival.device)
+    check(lib.lem_time_wake(ptr(arrival), arrival.shape[0], ptr(wake), stream_ptr()),
+          "lem_time_wake")
+    return wake
+
+
+def time_bins(arrival, wake, sampling_freq):
+    """Start sample of each path (int64 tensor)."""
+    torch = _torch()
+    lib = load_library()
+    arrival = to_dev(arrival, torch.float64).reshape(-1)
+    wake = to_dev(wake, torch.float64).reshape(1)
+    b = torch.empty(arrival.shape[0], dtype=torch.int64, device=arrival.device)
+    check(lib.lem_time_bins(ptr(arrival), arrival.shape[0], ptr(wake), float(sampling_freq),
+                            ptr(b), stream_ptr()), "lem_time_bins")
+    return b
+
+
+def time_map(amp, bins, slot, nslots, excitation, N):
+    """(nslots, N, 3) float64 time map: the paths of each slot (slot[v] in
+    [0, nslots)) summed in path order."""
+    torch = _torch()
+    lib = load_library()
+    amp = to_dev(amp, torch.float64).reshape(-1, 3)
+    bins = to_dev(bins, torch.int64).reshape(-1)
+    exc = to_dev(np.asarray(excitation, np.float64), torch.float64).reshape(-1)
+    slot = np.asarray(slot, np.int64)
+    out = torch.zeros((int(nslots), int(N), 3), dtype=torch.float64, device=amp.device)
+    if slot.shape[0] == 0 or N == 0:
+        return out
+    if slot.min() < 0 or slot.max() >= nslots:
+        raise LemError("time_map: slot outside [0, nslots)")
+    order = np.argsort(slot, kind="stable")
+    ss = slot[order]
+    starts = np.flatnonzero(np.r_[True, ss[1:] != ss[:-1]])
+    seg_start = to_dev(np.append(starts, len(ss)).astype(np.int64), torch.int64)
+    seg_slot = to_dev(ss[starts].astype(np.int64), torch.int64)
+    order = to_dev(order.astype(np.int64), torch.int64)
+    check(lib.lem_time_map(ptr(amp), ptr(bins), ptr(order), ptr(seg_start), ptr(seg_slot),
+                           seg_slot.shape[0], ptr(exc), exc.shape[0], int(N), ptr(out),
+                           stream_ptr()), "lem_time_map")
+    return out
+
+
+def time_trace(amp, bins, excitation, N):
+    """(N, 3) float64: every path's placed excitation, summed in path order."""
+    torch = _torch()
+    lib = load_library()
+    amp = to_dev(amp, torch.float64).reshape(-1, 3)
+    bins = to_dev(bins, torch.int64).reshape(-1)
+    exc = to_dev(np.asarray(excitation, np.float64), torch.float64).reshape(-1)
+    out = torch.zeros((int(N), 3), dtype=torch.float64, device=amp.device)
+    check(lib.lem_time_trace(ptr(amp), ptr(bins), amp.shape[0], ptr(exc), exc.shape[0], int(N),
+                             ptr(out), stream_ptr()), "lem_time_trace")
     return out
 
 

@@ -116,3 +116,49 @@ def calculate_conformalVectors(desired_E_vector, source_normals, antenna_axes=np
         e = dv[i, :] if per_point else dv.ravel()
         out[i, :] = vector_mapping(e, normals[i, :], ax)
     return out
+
+
+def _path_table(point_information):
+    pi = np.asarray(point_information)
+    if pi.shape[0] and not np.all(pi["Electric"]):
+        raise NotImplementedError("time domain: magnetic (Electric=False) sources")
+    pos = np.stack([pi["px"], pi["py"], pi["pz"]], axis=1).astype(np.float32)
+    nrm = np.stack([pi["nx"], pi["ny"], pi["nz"]], axis=1).astype(np.float32)
+    w = np.stack([pi["ex"], pi["ey"], pi["ez"]], axis=1).astype(np.complex64)
+    return pos, nrm, w
+
+
+def time_domain_paths(point_information, full_index, alpha, beta, sampling_freq):
+    """(amp (V, 3), arrival (V,), wake (1,), bins (V,)) device tensors of every
+    row of full_index: timedomainkernal's per-path part and start samples."""
+    pos, nrm, w = _path_table(point_information)
+    fi = np.ascontiguousarray(full_index, np.int32)
+    amp, arrival = _native.time_paths(pos, nrm, w, fi, alpha, beta)
+    wake = _native.time_wake(arrival)
+    bins = _native.time_bins(arrival, wake, sampling_freq)
+    return amp, arrival, wake, bins
+
+
+def TimeDomainv3(source_num, sink_num, point_informationv2, full_index, scattering_coefficient,
+                 wavelength, excitation_signal, sampling_freq, num_samples, alpha, beta):
+    """Time-domain response of every (source, sink) pair (empropagation.py:2910-3170).
+
+    Returns (time_map, wake_time): time_map float64 (source_num, sink_num,
+    num_samples, 3) holds, per pair, the excitation placed at each path's start
+    sample (arrival - wake) // (1 / sampling_freq) and scaled by the path's
+    polarised amplitude; wake_time is the earliest arrival (at most 1 s).  The
+    paths of a pair are added in index order (the reference uses float64
+    atomics), and the wake time is the minimum over every path (the
+    reference's 1 GB source chunking, with a wake time per chunk re-aligned by
+    rounding, is not reproduced).  scattering_coefficient and wavelength are
+    unused, as in the reference."""
+    S, K, N = int(source_num), int(sink_num), int(num_samples)
+    fi = np.asarray(full_index)
+    if fi.shape[0] == 0:
+        return np.zeros((S, K, N, 3), np.float64), 1.0
+    amp, _, wake, bins = time_domain_paths(point_informationv2, fi, alpha, beta, sampling_freq)
+    s, k = path_slots(fi, S, K)
+    if s.min() < 0 or s.max() >= S or k.min() < 0 or k.max() >= K:
+        raise IndexError("full_index rows do not start on a source and end on a sink")
+    tm = _native.time_map(amp, bins, s * K + k, S * K, excitation_signal, N)
+    return tm.cpu().numpy().reshape(S, K, N, 3), float(wake.item())

@@ -523,3 +523,180 @@ ORC_API void orc_field_paths(const float *pos, const float *nrm, const float *w,
         }
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Time domain (timedomainkernal, empropagation.py:1345-1612)                */
+/* ------------------------------------------------------------------------ */
+
+/* Python float floor division a // b as CPython's float_floor_div computes it
+ * (numba's float floordiv follows it): fmod, then exact adjustment.  Used for
+ * the start sample time_offset // time_sep (empropagation.py:1503-1506). */
+ORC_API double orc_py_floordiv(double a, double b)
+{
+    double mod = fmod(a, b);
+    double div = (a - mod) / b;
+    if (mod != 0.0 && ((b < 0.0) != (mod < 0.0)))
+        div -= 1.0;
+    if (div != 0.0) {
+        double fl = floor(div);
+        if (div - fl > 0.5)
+            fl += 1.0;
+        return fl;
+    }
+    return copysign(0.0, a / b);
+}
+
+/* point table index of a 1-based entry; 0 reads point -1 (numba wraps
+ * negative indices to the end of the array) */
+static int64_t orc_pt(int32_t e, int64_t N) { return e >= 1 ? (int64_t)e - 1 : N - 1; }
+
+/* calc_sep (empropagation.py:82-93): float32 differences, float64 squares,
+ * sum and square root */
+static double orc_sep(const float *p, int64_t a, int64_t b)
+{
+    double dx = (double)(p[3 * b] - p[3 * a]), dy = (double)(p[3 * b + 1] - p[3 * a + 1]),
+           dz = (double)(p[3 * b + 2] - p[3 * a + 2]);
+    return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+/* lossy_propagation (empropagation.py:768-801) as a complex value:
+ *   length  float32(calc_sep)            (the float32 local array, :771-772)
+ *   d       float32(float32 diff / float64 norm)   (calc_dv into float32)
+ *   dot     float32 dot of d and the end point's normal
+ *   G       exp(-(alpha + j beta) len) / len       (complex128)
+ *   dG      (-(alpha + j beta) - complex64(1/len)) G
+ *   loss    -(1/2pi) dG dot */
+static void orc_lossy(const float *p, const float *n, int64_t a, int64_t b, double alpha,
+                      double beta, double *re, double *im)
+{
+    float dx = p[3 * b] - p[3 * a], dy = p[3 * b + 1] - p[3 * a + 1], dz = p[3 * b + 2] - p[3 * a + 2];
+    double nrm = sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz);
+    float len = (float)nrm;
+    float ux = (float)((double)dx / nrm), uy = (float)((double)dy / nrm), uz = (float)((double)dz / nrm);
+    float dot = ux * n[3 * b] + uy * n[3 * b + 1];
+    dot = dot + uz * n[3 * b + 2];
+    double inv = 1.0 / (double)len;
+    double ea = exp(-alpha * (double)len);
+    double gr = ea * cos(-beta * (double)len) * inv, gi = ea * sin(-beta * (double)len) * inv;
+    double ar = -alpha - (double)(float)inv, ai = -beta;
+    double dgr = ar * gr - ai * gi, dgi = ar * gi + ai * gr;
+    double f = -(1.0 / (2.0 * M_PI)) * (double)dot;
+    *re = f * dgr;
+    *im = f * dgi;
+}
+
+/* sourcelaunchtransformGPU (empropagation.py:178-249) applied to a real
+ * vector: the components along u and v, the normalised axis x d / |x d| (the
+ * global axis with the largest |axis x d|) and d x u / |d x u|, for the
+ * normalised direction d. */
+static void orc_launch(const double d_in[3], double v[3])
+{
+    double m = sqrt(d_in[0] * d_in[0] + d_in[1] * d_in[1] + d_in[2] * d_in[2]);
+    double d[3] = {d_in[0] / m, d_in[1] / m, d_in[2] / m};
+    double ax[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double cn[3];
+    for (int k = 0; k < 3; ++k) {
+        double c0 = ax[k][1] * d[2] - ax[k][2] * d[1], c1 = ax[k][2] * d[0] - ax[k][0] * d[2],
+               c2 = ax[k][0] * d[1] - ax[k][1] * d[0];
+        cn[k] = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    }
+    int k = 2;
+    if (fabs(cn[0]) > fabs(cn[1]) && fabs(cn[0]) > fabs(cn[2]))
+        k = 0;
+    else if (fabs(cn[1]) >= fabs(cn[0]) && fabs(cn[1]) > fabs(cn[2]))
+        k = 1;
+    double u[3] = {ax[k][1] * d[2] - ax[k][2] * d[1], ax[k][2] * d[0] - ax[k][0] * d[2],
+                   ax[k][0] * d[1] - ax[k][1] * d[0]};
+    double un = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    u[0] /= un; u[1] /= un; u[2] /= un;
+    double w[3] = {d[1] * u[2] - d[2] * u[1], d[2] * u[0] - d[0] * u[2], d[0] * u[1] - d[1] * u[0]};
+    double wn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    w[0] /= wn; w[1] /= wn; w[2] /= wn;
+    double eu = v[0] * u[0] + v[1] * u[1] + v[2] * u[2];
+    double ev = v[0] * w[0] + v[1] * w[1] + v[2] * w[2];
+    for (int c = 0; c < 3; ++c)
+        v[c] = eu * u[c] + ev * w[c];
+}
+
+/* Per path row (timedomainkernal, empropagation.py:1361-1491): ray_component
+ * = Re(weight of the first point), then per leg i -> i+1: (i > 0) the launch
+ * transform with point i's normal and the leg length added; if the next entry
+ * is non-zero, the launch transform along the leg, times Re(next weight), the
+ * delay increased by Im(next ex), and (i > 0) the loss product extended.
+ * amp = ray_component * |loss product| (cmath.polar), arrival = length / c +
+ * delay.  w: N x 3 complex64 as interleaved float pairs. */
+ORC_API void orc_time_paths(const float *pos, const float *nrm, const float *w, int64_t N,
+                            const int32_t *index, int64_t V, int ncol, double alpha, double beta,
+                            double *amp, double *arrival)
+{
+    const double C = 299792458.0;
+    for (int64_t v = 0; v < V; ++v) {
+        const int32_t *row = index + v * ncol;
+        double rc[3] = {0, 0, 0}, len = 0.0, delay = 0.0;
+        for (int i = 0; i < ncol - 1; ++i) {
+            int64_t a = orc_pt(row[i], N), b = orc_pt(row[i + 1], N);
+            if (i == 0) {
+                len = orc_sep(pos, a, b);
+                delay += (double)w[6 * a + 1];
+                rc[0] = w[6 * a];
+                rc[1] = w[6 * a + 2];
+                rc[2] = w[6 * a + 4];
+            } else {
+                double nn[3] = {nrm[3 * a], nrm[3 * a + 1], nrm[3 * a + 2]};
+                orc_launch(nn, rc);
+                len = orc_sep(pos, a, b) + len;
+            }
+            if (row[i + 1] != 0) {
+                double dd[3] = {(double)(pos[3 * b] - pos[3 * a]), (double)(pos[3 * b + 1] - pos[3 * a + 1]),
+                                (double)(pos[3 * b + 2] - pos[3 * a + 2])};
+                orc_launch(dd, rc);
+                rc[0] *= (double)w[6 * b];
+                rc[1] *= (double)w[6 * b + 2];
+                rc[2] *= (double)w[6 * b + 4];
+                delay += (double)w[6 * b + 1];
+            }
+        }
+        double lr, li;
+        orc_lossy(pos, nrm, orc_pt(row[0], N), orc_pt(row[1], N), alpha, beta, &lr, &li);
+        for (int i = 1; i < ncol - 1; ++i) {
+            if (row[i + 1] == 0)
+                continue;
+            double br, bi;
+            orc_lossy(pos, nrm, orc_pt(row[i], N), orc_pt(row[i + 1], N), alpha, beta, &br, &bi);
+            double nr = lr * br - li * bi, ni = lr * bi + li * br;
+            lr = nr;
+            li = ni;
+        }
+        double mag = hypot(lr, li);
+        amp[3 * v] = rc[0] * mag;
+        amp[3 * v + 1] = rc[1] * mag;
+        amp[3 * v + 2] = rc[2] * mag;
+        arrival[v] = len / C + delay;
+    }
+}
+
+/* Placement (empropagation.py:1490-1600): wake = min(1.0, arrivals) (the
+ * atomic minimum from its initial 1 s), start sample = (arrival - wake) //
+ * (1 / fs), then excitation[j] * amp added at start + j for j < min(E,
+ * N - start), into out[slot[v]][n][c] (nslots x N x 3, zero on entry), paths
+ * in index order.  Writes *wake and bins[v]. */
+ORC_API void orc_time_place(const double *amp, const double *arrival, int64_t V,
+                            const int64_t *slot, const double *exc, int64_t E, double fs,
+                            int64_t N, double *out, double *wake, int64_t *bins)
+{
+    double wk = 1.0;
+    for (int64_t v = 0; v < V; ++v)
+        if (arrival[v] < wk)
+            wk = arrival[v];
+    *wake = wk;
+    const double sep = 1.0 / fs;
+    for (int64_t v = 0; v < V; ++v) {
+        double b = orc_py_floordiv(arrival[v] - wk, sep);
+        int64_t s0 = (int64_t)b;
+        bins[v] = s0;
+        int64_t cnt = s0 + E <= N ? E : N - s0;
+        for (int64_t j = 0; j < cnt; ++j)
+            for (int c = 0; c < 3; ++c)
+                out[(slot[v] * N + s0 + j) * 3 + c] += exc[j] * amp[3 * v + c];
+    }
+}

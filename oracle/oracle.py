@@ -64,6 +64,11 @@ def lib():
                                     ctypes.c_double, ctypes.c_int, P, ctypes.c_int]
         L.orc_field_paths.argtypes = [P, P, P, i64, P, i64, ctypes.c_int, i64, i64,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, P]
+        L.orc_py_floordiv.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_py_floordiv.restype = ctypes.c_double
+        L.orc_time_paths.argtypes = [P, P, P, i64, P, i64, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_double, P, P]
+        L.orc_time_place.argtypes = [P, P, i64, P, P, i64, ctypes.c_double, i64, P, P, P]
         _lib = L
     return _lib
 
@@ -359,3 +364,41 @@ def field_los_torch64(w, pos, snk, snk_n, visible, alpha, beta):
     dw = (d.to(torch.complex128) * w[:, None, :]).sum(-1)
     Q = w[:, None, :] - d.to(torch.complex128) * dw[..., None]
     return (L[..., None] * Q).sum(0)
+
+
+# ---------------------------------------------------------------------------
+# time domain (timedomainkernal, empropagation.py:1345-1612)
+# ---------------------------------------------------------------------------
+
+def py_floordiv(a, b):
+    """Python's float a // b as the C restatement computes it."""
+    return lib().orc_py_floordiv(float(a), float(b))
+
+
+def time_paths(pos, nrm, w, index, alpha, beta):
+    """(amp (V, 3), arrival (V,)) float64 of every row of a 1-based index."""
+    p = _f32(pos).reshape(-1, 3)
+    n = _f32(nrm).reshape(-1, 3)
+    N = p.shape[0]
+    ww = _c64x3(w, N)
+    idx = np.ascontiguousarray(index, np.int32)
+    V, ncol = idx.shape
+    amp = np.zeros((V, 3), np.float64)
+    arr = np.zeros(V, np.float64)
+    lib().orc_time_paths(_p(p), _p(n), _p(ww), N, _p(idx), V, ncol, float(alpha), float(beta),
+                         _p(amp), _p(arr))
+    return amp, arr
+
+
+def time_place(amp, arrival, slot, nslots, excitation, fs, N):
+    """(out (nslots, N, 3), wake, bins): the reference's per-path placement."""
+    amp = np.ascontiguousarray(amp, np.float64)
+    arr = np.ascontiguousarray(arrival, np.float64)
+    sl = np.ascontiguousarray(slot, np.int64)
+    exc = np.ascontiguousarray(excitation, np.float64)
+    out = np.zeros((int(nslots), int(N), 3), np.float64)
+    wake = np.zeros(1, np.float64)
+    bins = np.zeros(len(arr), np.int64)
+    lib().orc_time_place(_p(amp), _p(arr), len(arr), _p(sl), _p(exc), len(exc), float(fs),
+                         int(N), _p(out), _p(wake), _p(bins))
+    return out, float(wake[0]), bins
