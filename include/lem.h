@@ -255,10 +255,15 @@ int lem_time_map(const double *amp, const int64_t *bin, const int64_t *order,
                  const double *excitation, int64_t E, int64_t N, double *out, void *stream);
 
 /* The time map summed over every path: out[n][c] (N x 3 float64), the source
- * and sink sums of time_domain.calculate_scattering (time_domain.py:383-428),
- * in path order. */
+ * and sink sums of time_domain.calculate_scattering (time_domain.py:383-428).
+ * Paths are sorted stably by start sample, their amplitudes summed per sample
+ * in path order and convolved with the excitation.  V and N below 2^31; the
+ * workspace size comes from lem_time_trace_workspace (needs a HIP device: the
+ * sort's scratch size depends on it). */
+int64_t lem_time_trace_workspace(int64_t V, int64_t N);
 int lem_time_trace(const double *amp, const int64_t *bin, int64_t V, const double *excitation,
-                   int64_t E, int64_t N, double *out, void *stream);
+                   int64_t E, int64_t N, double *out, void *workspace, int64_t workspace_bytes,
+                   void *stream);
 
 #ifdef __cplusplus
 }

@@ -18,8 +18,12 @@
 //                  floor((arrival - wake) / (1 / fs)) (Python's float floor
 //                  division), summed per (source, sink) slot in path order --
 //                  no atomics, deterministic.  out[slot][n][c] float64.
-//   k_time_trace : the same summed over every path (the source and sink sums
-//                  of time_domain.calculate_scattering), out[n][c].
+//   trace        : the same summed over every path (the source and sink sums
+//                  of time_domain.calculate_scattering), out[n][c]: a stable
+//                  radix sort of the paths by start sample, per-sample sums of
+//                  the amplitudes in path order, then the excitation
+//                  convolution -- O(V + N E) instead of the reference's V x E
+//                  atomic additions, and deterministic.
 #include "lem_device.h"
 #include "lem_internal.h"
 #include "../../include/lem.h"
@@ -27,6 +31,8 @@
 #include <math.h>
 
 #include <algorithm>
+
+#include <hipcub/hipcub.hpp>
 
 namespace lem {
 
@@ -258,49 +264,130 @@ __global__ __launch_bounds__(TB) void k_time_map(
     o[2] = a2;
 }
 
-// Whole trace: one thread per sample, every path in index order, paths staged
-// through LDS 256 at a time.
-__global__ __launch_bounds__(TB) void k_time_trace(const double *__restrict__ amp,
-                                                   const int64_t *__restrict__ bin, int64_t V,
+// Trace: sort keys (start sample clamped to [0, N]; N = never placed).
+__global__ __launch_bounds__(TB) void k_trace_keys(const int64_t *__restrict__ bin, int64_t V,
+                                                   int64_t N, int32_t *__restrict__ key,
+                                                   int32_t *__restrict__ val)
+{
+    const int64_t p = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (p >= V)
+        return;
+    const int64_t b = bin[p];
+    key[p] = (int32_t)(b < 0 ? 0 : (b > N ? N : b));
+    val[p] = (int32_t)p;
+}
+
+// start[b] = first sorted position with key >= b, for b in [0, N]
+__global__ __launch_bounds__(TB) void k_trace_starts(const int32_t *__restrict__ key, int64_t V,
+                                                     int64_t N, int32_t *__restrict__ start)
+{
+    const int64_t b = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (b > N)
+        return;
+    int64_t lo = 0, hi = V;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (key[mid] < b)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    start[b] = (int32_t)lo;
+}
+
+// H[b] = sum of the amplitudes of the paths starting at sample b, in path order
+__global__ __launch_bounds__(TB) void k_trace_hist(const double *__restrict__ amp,
+                                                   const int32_t *__restrict__ val,
+                                                   const int32_t *__restrict__ start, int64_t N,
+                                                   double *__restrict__ H)
+{
+    const int64_t b = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (b >= N)
+        return;
+    double h0 = 0.0, h1 = 0.0, h2 = 0.0;
+    for (int32_t i = start[b]; i < start[b + 1]; ++i) {
+        const int64_t p = val[i];
+        h0 += amp[3 * p];
+        h1 += amp[3 * p + 1];
+        h2 += amp[3 * p + 2];
+    }
+    H[3 * b] = h0;
+    H[3 * b + 1] = h1;
+    H[3 * b + 2] = h2;
+}
+
+// out[n] = sum_{j <= min(E - 1, n)} exc[j] H[n - j]
+__global__ __launch_bounds__(TB) void k_trace_conv(const double *__restrict__ H,
                                                    const double *__restrict__ exc, int64_t E,
                                                    int64_t N, double *__restrict__ out)
 {
-    __shared__ double sa[TB][3];
-    __shared__ int64_t sb[TB];
     const int64_t n = (int64_t)blockIdx.x * TB + threadIdx.x;
-    const int64_t n0 = (int64_t)blockIdx.x * TB;
+    if (n >= N)
+        return;
+    const int64_t J = n + 1 < E ? n + 1 : E;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int64_t p0 = 0; p0 < V; p0 += TB) {
-        __syncthreads();
-        const int64_t p = p0 + threadIdx.x;
-        if (p < V) {
-            sb[threadIdx.x] = bin[p];
-            sa[threadIdx.x][0] = amp[3 * p];
-            sa[threadIdx.x][1] = amp[3 * p + 1];
-            sa[threadIdx.x][2] = amp[3 * p + 2];
-        } else {
-            sb[threadIdx.x] = -(int64_t)1 << 62;  // never in range
-        }
-        __syncthreads();
-        const int cnt = (int)(V - p0 < TB ? V - p0 : TB);
-        for (int q = 0; q < cnt; ++q) {
-            const int64_t b = sb[q];
-            if (b > n0 + TB - 1 || b + E <= n0)
-                continue;  // the whole block is outside this path's window
-            const int64_t j = n - b;
-            if (j >= 0 && j < E) {
-                const double e = exc[j];
-                a0 += e * sa[q][0];
-                a1 += e * sa[q][1];
-                a2 += e * sa[q][2];
-            }
-        }
+    for (int64_t j = 0; j < J; ++j) {
+        const double e = exc[j];
+        const double *h = H + 3 * (n - j);
+        a0 += e * h[0];
+        a1 += e * h[1];
+        a2 += e * h[2];
     }
-    if (n < N) {
-        out[3 * n] = a0;
-        out[3 * n + 1] = a1;
-        out[3 * n + 2] = a2;
+    out[3 * n] = a0;
+    out[3 * n + 1] = a1;
+    out[3 * n + 2] = a2;
+}
+
+struct TraceWs {
+    int32_t *k0, *k1, *v0, *v1, *start;
+    double *H;
+    void *cub;
+    size_t cub_bytes;
+};
+
+inline int key_bits(int64_t N)
+{
+    int b = 1;
+    while (b < 31 && ((int64_t)1 << b) <= N)
+        ++b;
+    return b;
+}
+
+inline int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
+
+// Lays the trace workspace out from base (or only sizes it when base is null).
+inline int64_t trace_layout(int64_t V, int64_t N, size_t cub_bytes, char *base, TraceWs *L)
+{
+    int64_t off = 0;
+    auto take = [&](int64_t bytes) {
+        char *p = base ? base + off : nullptr;
+        off += align256(bytes);
+        return p;
+    };
+    char *k0 = take(4 * V), *k1 = take(4 * V), *v0 = take(4 * V), *v1 = take(4 * V);
+    char *st = take(4 * (N + 2));
+    char *H = take(8 * 3 * N);
+    char *cub = take((int64_t)cub_bytes);
+    if (L) {
+        L->k0 = reinterpret_cast<int32_t *>(k0);
+        L->k1 = reinterpret_cast<int32_t *>(k1);
+        L->v0 = reinterpret_cast<int32_t *>(v0);
+        L->v1 = reinterpret_cast<int32_t *>(v1);
+        L->start = reinterpret_cast<int32_t *>(st);
+        L->H = reinterpret_cast<double *>(H);
+        L->cub = cub;
+        L->cub_bytes = cub_bytes;
     }
+    return off;
+}
+
+inline size_t cub_sort_bytes(int64_t V, int64_t N)
+{
+    size_t bytes = 0;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const int32_t *)nullptr,
+                                       (int32_t *)nullptr, (const int32_t *)nullptr,
+                                       (int32_t *)nullptr, (int)V, 0, key_bits(N));
+    return bytes;
 }
 
 }  // namespace lem
@@ -376,17 +463,52 @@ extern "C" int lem_time_map(const double *amp, const int64_t *bin, const int64_t
     return 0;
 }
 
+extern "C" int64_t lem_time_trace_workspace(int64_t V, int64_t N)
+{
+    if (V < 0 || N < 0 || V > 2000000000LL || N > 2000000000LL)
+        return -1;
+    return trace_layout(V, N, cub_sort_bytes(V > 0 ? V : 1, N), nullptr, nullptr);
+}
+
 extern "C" int lem_time_trace(const double *amp, const int64_t *bin, int64_t V,
                               const double *excitation, int64_t E, int64_t N, double *out,
-                              void *stream)
+                              void *workspace, int64_t workspace_bytes, void *stream)
 {
     clear_error();
     LEM_REQUIRE(V >= 0 && E >= 0 && N >= 0, "lem_time_trace: negative size");
+    LEM_REQUIRE(V <= 2000000000LL && N <= 2000000000LL, "lem_time_trace: too many paths or samples");
     if (N == 0)
         return 0;
     LEM_REQUIRE(out && (V == 0 || (amp && bin)) && (E == 0 || excitation),
                 "lem_time_trace: null pointer");
-    hipLaunchKernelGGL(k_time_trace, dim3((unsigned)ceil_div(N, TB)), dim3(TB), 0,
-                       as_stream(stream), amp, bin, V, excitation, E, N, out);
-    return check_launch("k_time_trace");
+    const int64_t Vs = V > 0 ? V : 1;
+    const size_t cub_bytes = cub_sort_bytes(Vs, N);
+    const int64_t need = trace_layout(Vs, N, cub_bytes, nullptr, nullptr);
+    if (!workspace || workspace_bytes < need) {
+        set_error("lem_time_trace: workspace smaller than lem_time_trace_workspace()");
+        return E_WORKSPACE;
+    }
+    TraceWs L;
+    trace_layout(Vs, N, cub_bytes, reinterpret_cast<char *>(workspace), &L);
+    hipStream_t st = as_stream(stream);
+    if (V > 0) {
+        hipLaunchKernelGGL(k_trace_keys, dim3((unsigned)ceil_div(V, TB)), dim3(TB), 0, st, bin, V,
+                           N, L.k0, L.v0);
+        int rc = check_launch("k_trace_keys");
+        if (rc)
+            return rc;
+        size_t tb = L.cub_bytes;
+        if (hipcub::DeviceRadixSort::SortPairs(L.cub, tb, L.k0, L.k1, L.v0, L.v1, (int)V, 0,
+                                               key_bits(N), st) != hipSuccess) {
+            set_error("lem_time_trace: radix sort failed");
+            return E_HIP;
+        }
+    }
+    hipLaunchKernelGGL(k_trace_starts, dim3((unsigned)ceil_div(N + 1, TB)), dim3(TB), 0, st,
+                       L.k1, V, N, L.start);
+    hipLaunchKernelGGL(k_trace_hist, dim3((unsigned)ceil_div(N, TB)), dim3(TB), 0, st, amp, L.v1,
+                       L.start, N, L.H);
+    hipLaunchKernelGGL(k_trace_conv, dim3((unsigned)ceil_div(N, TB)), dim3(TB), 0, st, L.H,
+                       excitation, E, N, out);
+    return check_launch("k_trace_conv");
 }

@@ -49,7 +49,8 @@ SIGNATURES = {
     "lem_time_wake": ([_P, _I64, _P, _P], _INT),
     "lem_time_bins": ([_P, _I64, _P, _D, _P, _P], _INT),
     "lem_time_map": ([_P, _P, _P, _P, _P, _I64, _P, _I64, _I64, _P, _P], _INT),
-    "lem_time_trace": ([_P, _P, _I64, _P, _I64, _I64, _P, _P], _INT),
+    "lem_time_trace_workspace": ([_I64, _I64], _I64),
+    "lem_time_trace": ([_P, _P, _I64, _P, _I64, _I64, _P, _P, _I64, _P], _INT),
     "lem_field_los_grid": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _P, _P, _P, _D, _D, _INT, _P,
                             _P, _I64, _P], _INT),
     "lem_gw_mfma": ([_P, _I64, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P, _I64, _P,
@@ -357,15 +358,21 @@ def time_map(amp, bins, slot, nslots, excitation, N):
 
 
 def time_trace(amp, bins, excitation, N):
-    """(N, 3) float64: every path's placed excitation, summed in path order."""
+    """(N, 3) float64: every path's placed excitation, summed (per start sample in
+    path order, then convolved with the excitation)."""
     torch = _torch()
     lib = load_library()
     amp = to_dev(amp, torch.float64).reshape(-1, 3)
     bins = to_dev(bins, torch.int64).reshape(-1)
     exc = to_dev(np.asarray(excitation, np.float64), torch.float64).reshape(-1)
     out = torch.zeros((int(N), 3), dtype=torch.float64, device=amp.device)
-    check(lib.lem_time_trace(ptr(amp), ptr(bins), amp.shape[0], ptr(exc), exc.shape[0], int(N),
-                             ptr(out), stream_ptr()), "lem_time_trace")
+    V = amp.shape[0]
+    wsb = int(lib.lem_time_trace_workspace(V, int(N)))
+    if wsb < 0:
+        raise LemError("time_trace: too many paths or samples")
+    ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=amp.device)
+    check(lib.lem_time_trace(ptr(amp), ptr(bins), V, ptr(exc), exc.shape[0], int(N), ptr(out),
+                             ptr(ws), wsb, stream_ptr()), "lem_time_trace")
     return out
 
 
