@@ -38,6 +38,7 @@ MT_FLOPS = 46          # FP32 flops of one Moller-Trumbore test (+1 divide)
 PAIR_FLOPS = 92        # FP32 flops of one Green's-function pair
 PEAK_FP32 = 157.3e12   # MI355X FP32 vector peak (packed)
 PEAK_HBM = 8.0e12
+PEAK_FP64 = 78.6e12    # MI355X FP64 vector peak
 
 
 def _events(torch, n):
@@ -497,7 +498,103 @@ class C5:
                 "sources": self.S, "sinks": self.K, "frequencies": self.F, "sets": 64}
 
 
-CONFIGS = {"c1": C1, "c2": C2, "c3": C3, "c4": C4, "c5": C5}
+# --------------------------------------------------------------------------
+# time domain: every line-of-sight path of a 64x64 array to 256 sinks
+# --------------------------------------------------------------------------
+
+TD_PATH_FLOPS = 75  # float64 work of one line-of-sight path in k_time_paths
+
+
+class TD:
+    name = "td"
+
+    def __init__(self, torch):
+        from lyceanem import _native, scenes
+
+        self.torch, self.nat = torch, _native
+        self.lib = _native.load_library()
+        sc = scenes.time_domain_td()
+        self.sc = sc
+        dev = _native.device()
+        self.pos = torch.from_numpy(sc["pos"]).to(dev)
+        self.nrm = torch.from_numpy(sc["nrm"]).to(dev)
+        self.w = torch.from_numpy(sc["w"]).to(dev)
+        self.index = torch.from_numpy(sc["index"]).to(dev)
+        self.exc = torch.from_numpy(sc["exc"]).to(dev)
+        self.V, self.N = sc["index"].shape[0], int(sc["n_samples"])
+        f64 = torch.float64
+        self.amp = torch.empty((self.V, 3), dtype=f64, device=dev)
+        self.arr = torch.empty(self.V, dtype=f64, device=dev)
+        self.wake = torch.empty(1, dtype=f64, device=dev)
+        self.bins = torch.empty(self.V, dtype=torch.int64, device=dev)
+        self.err = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.out = torch.empty((self.N, 3), dtype=f64, device=dev)
+        self.wsb = int(self.lib.lem_time_trace_workspace(self.V, self.N))
+        self.ws = torch.empty(self.wsb, dtype=torch.uint8, device=dev)
+
+    def rays(self):
+        return self.V
+
+    def step(self, ev=None):
+        nat, lib, p = self.nat, self.lib, self.nat.ptr
+        st = nat.stream_ptr()
+        if ev is not None:
+            ev[0].record()
+        nat.check(lib.lem_time_paths(p(self.pos), p(self.nrm), p(self.w), self.pos.shape[0],
+                                     p(self.index), self.V, 2, 0.0, float(self.sc["beta"]),
+                                     p(self.amp), p(self.arr), p(self.err), st), "time_paths")
+        if ev is not None:
+            ev[1].record()
+        nat.check(lib.lem_time_wake(p(self.arr), self.V, p(self.wake), st), "time_wake")
+        nat.check(lib.lem_time_bins(p(self.arr), self.V, p(self.wake), float(self.sc["fs"]),
+                                    p(self.bins), st), "time_bins")
+        nat.check(lib.lem_time_trace(p(self.amp), p(self.bins), self.V, p(self.exc),
+                                     self.exc.shape[0], self.N, p(self.out), p(self.ws), self.wsb,
+                                     st), "time_trace")
+        if ev is not None:
+            ev[2].record()
+
+    def record(self, times_ms):
+        tp = float(np.mean([a for a, _ in times_ms]))
+        rest = float(np.mean([b for _, b in times_ms]))
+        flops = self.V * TD_PATH_FLOPS
+        return {
+            "kernel": "k_time_paths",
+            "kernel_ms": tp,
+            "roofline": {"bound": "valu-fp64", "achieved": flops / (tp * 1e-3) / 1e12,
+                         "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                         "frac": flops / (tp * 1e-3) / PEAK_FP64, "traffic": None},
+            "extra": {"paths": self.V, "samples": self.N, "excitation_samples":
+                      int(self.exc.shape[0]), "wake_bins_trace_ms": rest,
+                      "time_paths_gpaths_per_s": self.V / (tp * 1e-3) / 1e9},
+        }
+
+    def cpu_baseline(self, budget_s):
+        from oracle import oracle as orc
+
+        sc = self.sc
+        n = 4096
+        t0 = time.perf_counter()
+        orc.time_paths(sc["pos"], sc["nrm"], sc["w"], sc["index"][:n], 0.0, sc["beta"])
+        dt = time.perf_counter() - t0
+        n = int(max(4096, min(self.V, n * budget_s / max(dt, 1e-6))))
+        t0 = time.perf_counter()
+        amp, arr = orc.time_paths(sc["pos"], sc["nrm"], sc["w"], sc["index"][:n], 0.0, sc["beta"])
+        orc.time_place(amp, arr, np.zeros(n, np.int64), 1, sc["exc"], sc["fs"], self.N)
+        dt = time.perf_counter() - t0
+        return {"value": n / dt / 1e9, "unit": "Gpaths/s", "cores": 1, "kind": "port",
+                "sample": f"first {n} of {self.V} paths: per-path amplitude/arrival and "
+                          f"placement of the {sc['exc'].shape[0]}-sample pulse into a "
+                          f"{self.N}-sample trace (the reference's per-path scatter), "
+                          f"{dt:.1f} s"}
+
+    def config(self):
+        return {"workload": "td: 64x64 array at 3 GHz -> 256 sinks 50-500 m, 1,048,576 "
+                            "line-of-sight paths, 256-sample pulse at 40 GS/s into a "
+                            "65,536-sample trace", "paths": self.V, "samples": self.N}
+
+
+CONFIGS = {"c1": C1, "c2": C2, "c3": C3, "c4": C4, "c5": C5, "td": TD}
 DEFAULT_CONFIG = "c2"
 
 
@@ -559,7 +656,8 @@ def main(argv=None) -> int:
         cpu = cfg.cpu_baseline(a.cpu_seconds)
         print(f"[bench] cpu baseline: {cpu['value']:.3f} {cpu['unit']}", file=sys.stderr,
               flush=True)
-    unit = {"c3": "Gpairs/s", "c4": "Gpairs/s", "c5": "G (pair x set)/s"}.get(a.config, "Mrays/s")
+    unit = {"c3": "Gpairs/s", "c4": "Gpairs/s", "c5": "G (pair x set)/s",
+            "td": "Gpaths/s"}.get(a.config, "Mrays/s")
     scale = 1e6 if unit == "Mrays/s" else 1e9
     line = {
         "metric": "Mrays/s (1M-tri DDA); Gpairs/s scattering + % roofline; adjoint/forward time",

@@ -241,3 +241,37 @@ def batched_c5(n_src_side=128, n_el=128, n_az=128, n_freq=32, f_lo=2e9, f_hi=4e9
     W = np.exp(-1j * kf[:, None, None] * proj[None]).astype(np.complex64)
     return dict(src=src, snk=snk, snk_n=snk_n, pol=np.array([0, 0, 1], np.float32),
                 wavenumbers=kf, W=W, directions=u)
+
+
+def time_domain_td(n_src_side=64, n_snk=256, freq=3e9, fs=40e9, n_samples=65536, n_exc=256,
+                   r_min=50.0, r_max=500.0, seed=13):
+    """Time-domain workload: an n x n array at lambda/2 in z=0 (normals +z,
+    x-polarised, weight = Area = (lambda/2)^2), n_snk sinks 50-500 m away in the
+    upper half-space (normals back towards the array), every source -> sink
+    line-of-sight path, a Gaussian-windowed sine pulse at `freq` sampled at fs
+    (n_exc samples) and an n_samples window."""
+    lam = C0 / freq
+    pitch = lam / 2
+    c = (np.arange(n_src_side) - (n_src_side - 1) / 2) * pitch
+    xx, yy = np.meshgrid(c, c)
+    S = n_src_side * n_src_side
+    src = np.stack([xx.ravel(), yy.ravel(), np.zeros(S)], 1).astype(np.float32)
+    rng = np.random.default_rng(seed)
+    th = np.arccos(rng.uniform(0.05, 1.0, n_snk))
+    ph = rng.uniform(-np.pi, np.pi, n_snk)
+    r = rng.uniform(r_min, r_max, n_snk)
+    u = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
+    snk = (u * r[:, None]).astype(np.float32)
+    snk_n = (-u).astype(np.float32)
+    w = np.zeros((S, 3), np.complex64)
+    w[:, 0] = pitch * pitch
+    pos = np.concatenate([src, snk])
+    nrm = np.concatenate([np.tile(np.array([0, 0, 1], np.float32), (S, 1)), snk_n])
+    weights = np.concatenate([w, np.ones((n_snk, 3), np.complex64)])
+    s_idx, k_idx = np.meshgrid(np.arange(S), np.arange(n_snk), indexing="ij")
+    index = np.stack([s_idx.ravel() + 1, k_idx.ravel() + S + 1], 1).astype(np.int32)
+    t = (np.arange(n_exc) - n_exc / 2) / fs
+    sig = n_exc / (8 * fs)
+    exc = np.sin(2 * np.pi * freq * t) * np.exp(-0.5 * (t / sig) ** 2)
+    return dict(pos=pos, nrm=nrm, w=weights, index=index, exc=exc, fs=fs, n_samples=n_samples,
+                beta=2 * np.pi / lam, S=S, K=n_snk)
