@@ -218,6 +218,17 @@ int lem_visibility_grid_stats(const float *src, int64_t S, const float *snk, int
                               const float *recs, const float *cell_box, int flags,
                               uint64_t *stats, void *stream);
 
+/* Projected-area pattern of visiblespace (rayfunctions.py:316-427; the
+ * angle_pop / patternsort loops, :431-468): out[k] (float32) = sum over the
+ * sources s whose ray to shell point k is unblocked (hit_bits: source-major
+ * S x ceil(K/64) from lem_visibility_dense) of area[s] * cos(angle between the
+ * ray and the normal of s), angles over 90 degrees contributing nothing;
+ * area == NULL gives every visible source a portion of 1 (the reference's
+ * vertex_area == 0).  Sources are summed in index order in float64. */
+int lem_projected_area(const float *src, const float *nrm, const float *area, int64_t S,
+                       const float *snk, int64_t K, const uint64_t *hit_bits, float *out,
+                       void *stream);
+
 /* ---- time domain (models/time_domain.py, TimeDomainv3) ------------------- */
 
 /* Per path row of a 1-based index (V x ncol int32, 0 = unused column, the

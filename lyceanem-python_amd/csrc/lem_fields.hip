@@ -218,6 +218,42 @@ int launch_los_finish(const float *ws_sum, const float *ws_cmp, int64_t NC, int6
     return check_launch("k_field_los_finish");
 }
 
+// Projected-area pattern of visiblespace (rayfunctions.py:316-427 with
+// angle_pop / patternsort, :431-468): for each shell point k, the sum over the
+// sources s with an unblocked ray s -> k of area[s] * cos(angle between the
+// ray and the source normal), angles beyond 90 degrees contributing nothing.
+// The reference's |cos(+-arccos(dp))| of the clamped dot product dp of the unit
+// vectors is dp itself; area == nullptr means a unit portion (its
+// vertex_area == 0 case).  One thread per shell point, sources in index order,
+// float64 sum.
+__global__ __launch_bounds__(256) void k_projected_area(
+    const float *__restrict__ src, const float *__restrict__ nrm,
+    const float *__restrict__ area, int64_t S, const float *__restrict__ snk, int64_t K,
+    const unsigned long long *__restrict__ bits, float *__restrict__ out)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= K)
+        return;
+    const int64_t W = (K + 63) / 64;
+    const unsigned long long bit = 1ull << (k & 63);
+    const float tx = snk[3 * k], ty = snk[3 * k + 1], tz = snk[3 * k + 2];
+    double acc = 0.0;
+    for (int64_t s = 0; s < S; ++s) {
+        if (bits[s * W + (k >> 6)] & bit)
+            continue;  // blocked
+        const Ray r = charge(src[3 * s], src[3 * s + 1], src[3 * s + 2], tx, ty, tz);
+        const float nx = nrm[3 * s], ny = nrm[3 * s + 1], nz = nrm[3 * s + 2];
+        const float nn = sqrtf(nx * nx + ny * ny + nz * nz);
+        const float rn = sqrtf(r.dx * r.dx + r.dy * r.dy + r.dz * r.dz);
+        float dp = (r.dx / rn) * (nx / nn) + (r.dy / rn) * (ny / nn);
+        dp = dp + (r.dz / rn) * (nz / nn);
+        dp = fminf(fmaxf(dp, -1.0f), 1.0f);
+        if (dp > 0.0f)
+            acc += area ? (double)area[s] * (double)dp : 1.0;
+    }
+    out[k] = (float)acc;
+}
+
 }  // namespace lem
 
 using namespace lem;
@@ -300,4 +336,20 @@ extern "C" int lem_field_paths(const float *pos, const float *nrm, const float *
                        as_stream(stream), pos, nrm, w, N, index, ncol, order, seg_start, seg_slot,
                        nseg, (float)alpha, beta, out, err_flag);
     return check_launch("k_field_paths");
+}
+
+extern "C" int lem_projected_area(const float *src, const float *nrm, const float *area,
+                                  int64_t S, const float *snk, int64_t K, const uint64_t *hit_bits,
+                                  float *out, void *stream)
+{
+    clear_error();
+    LEM_REQUIRE(S >= 0 && K >= 0, "lem_projected_area: negative size");
+    if (K == 0)
+        return 0;
+    LEM_REQUIRE(snk && out && (S == 0 || (src && nrm && hit_bits)),
+                "lem_projected_area: null pointer");
+    hipLaunchKernelGGL(k_projected_area, dim3((unsigned)ceil_div(K, 256)), dim3(256), 0,
+                       as_stream(stream), src, nrm, area, S, snk, K,
+                       reinterpret_cast<const unsigned long long *>(hit_bits), out);
+    return check_launch("k_projected_area");
 }

@@ -45,6 +45,7 @@ SIGNATURES = {
     "lem_grid_fill": ([_P, _I64, _P, _P, _P, _P, _P, _P, _I64, ctypes.c_float, _P], _INT),
     "lem_visibility_grid": ([_P, _I64, _P, _I64, _P, _P, _P, _P, _INT, _INT, _P, _P, _P], _INT),
     "lem_visibility_grid_stats": ([_P, _I64, _P, _I64, _P, _P, _P, _P, _INT, _P, _P], _INT),
+    "lem_projected_area": ([_P, _P, _P, _I64, _P, _I64, _P, _P, _P], _INT),
     "lem_time_paths": ([_P, _P, _P, _I64, _P, _I64, _INT, _D, _D, _P, _P, _P, _P], _INT),
     "lem_time_wake": ([_P, _I64, _P, _P], _INT),
     "lem_time_bins": ([_P, _I64, _P, _D, _P, _P], _INT),
@@ -282,6 +283,25 @@ def field_paths(pos, nrm, w, index, order, seg_start, seg_slot, S, K, alpha, bet
                               ptr(out), ptr(err), stream_ptr()), "lem_field_paths")
     if int(err.item()) != 0:
         raise LemError("lem_field_paths: path index points outside the point table")
+    return out
+
+
+def projected_area(src, nrm, area, snk, hit_bits):
+    """(K,) float32 projected-area pattern of visiblespace; area None = unit
+    portions."""
+    torch = _torch()
+    lib = load_library()
+    src = to_dev(src, torch.float32).reshape(-1, 3)
+    nrm = to_dev(nrm, torch.float32).reshape(-1, 3)
+    snk = to_dev(snk, torch.float32).reshape(-1, 3)
+    S, K = src.shape[0], snk.shape[0]
+    a = None if area is None else to_dev(np.asarray(area, np.float32).reshape(-1), torch.float32)
+    if a is not None and a.shape[0] != S:
+        raise LemError("projected_area: one area per source")
+    out = torch.empty(K, dtype=torch.float32, device=snk.device)
+    bits = to_dev(hit_bits, torch.int64)
+    check(lib.lem_projected_area(ptr(src), ptr(nrm), ptr(a), S, ptr(snk), K, ptr(bits),
+                                 ptr(out), stream_ptr()), "lem_projected_area")
     return out
 
 

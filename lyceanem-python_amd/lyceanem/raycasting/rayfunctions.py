@@ -238,34 +238,16 @@ def visiblespace(source_coords, source_normals, environment, vertex_area=0, az_r
     shell = TG.spherical_field(az_range, elev_range, outward_normals=True,
                                field_radius=shell_range)
     sinks = np.asarray(shell.points, np.float32)
-    K = sinks.shape[0]
     bits, _ = _native.visibility_dense(pts, sinks, _tris9(environment), any_hit=True)
-    blocked = _native.unpack_bits(bits, K)
-    s_idx, k_idx = np.nonzero(~blocked)
-    d, _ = math_functions.calc_dv_norm(pts[s_idx], sinks[k_idx])
-    nrm = np.asarray(source_normals, np.float32)[s_idx]
-    ang = _angle(d.astype(np.float64), nrm.astype(np.float64))
     va = np.asarray(vertex_area, dtype=np.float64).ravel() if np.ndim(vertex_area) else \
         np.asarray([vertex_area], np.float64)
     va = np.where(np.isnan(va), 0.0, va)
     if va.size == 1:
-        portion = np.ones(len(ang)) if va[0] == 0 else np.maximum(va[0] * np.abs(np.cos(ang)), 0)
+        area = None if va[0] == 0 else np.full(S, va[0], np.float32)
     else:
-        portion = va[s_idx] * np.abs(np.cos(ang))
-    portion[np.abs(ang) > np.pi / 2] = 0.0
-    pattern = np.zeros(K, np.float64)
-    np.add.at(pattern, k_idx, portion)
-    pattern = pattern.astype(np.float32).reshape(len(elev_range), len(az_range))
+        area = va.astype(np.float32)
+    pattern = _native.projected_area(pts, np.asarray(source_normals, np.float32), area, sinks,
+                                     bits).cpu().numpy()
+    pattern = pattern.reshape(len(elev_range), len(az_range))
     shell.point_data["Projected_Area"] = pattern.reshape(-1, 1)
     return pattern, shell
-
-
-def _angle(v1, v2):
-    """Signed angle between row vectors (rayfunctions.py:431-448): the sign comes
-    from the 2x2 minor of the last two components."""
-    u1 = v1 / np.linalg.norm(v1, axis=1, keepdims=True)
-    u2 = v2 / np.linalg.norm(v2, axis=1, keepdims=True)
-    minor = u1[:, 1] * u2[:, 2] - u1[:, 2] * u2[:, 1]
-    sign = np.where(minor == 0, 1.0, -np.sign(minor))
-    dp = np.clip(np.sum(u1 * u2, axis=1), -1.0, 1.0)
-    return sign * np.arccos(dp)

@@ -402,3 +402,46 @@ def time_place(amp, arrival, slot, nslots, excitation, fs, N):
     lib().orc_time_place(_p(amp), _p(arr), len(arr), _p(sl), _p(exc), len(exc), float(fs),
                          int(N), _p(out), _p(wake), _p(bins))
     return out, float(wake[0]), bins
+
+
+# ---------------------------------------------------------------------------
+# visiblespace projected-area pattern (rayfunctions.py:316-468)
+# ---------------------------------------------------------------------------
+
+def _signed_angle(v1, v2):
+    """angle() (rayfunctions.py:431-448) row-wise in float32: unit vectors, the
+    sign from the 2x2 minor of the last two components, arccos of the clamped
+    dot product."""
+    v1 = np.asarray(v1, np.float32)
+    v2 = np.asarray(v2, np.float32)
+    u1 = v1 / np.linalg.norm(v1, axis=1, keepdims=True)
+    u2 = v2 / np.linalg.norm(v2, axis=1, keepdims=True)
+    minor = u1[:, 1] * u2[:, 2] - u1[:, 2] * u2[:, 1]
+    sign = np.where(minor == 0, np.float32(1), -np.sign(minor)).astype(np.float32)
+    dp = np.clip(np.sum(u1 * u2, axis=1), -1.0, 1.0).astype(np.float32)
+    return (sign * np.arccos(dp)).astype(np.float32)
+
+
+def projected_area(src, nrm, vertex_area, snk, hit):
+    """visiblespace's pattern (rayfunctions.py:366-421): for every unblocked
+    (source, shell point) ray, the portion vertex_area * |cos(angle)| (1 when a
+    single vertex_area of 0 is given), zero beyond 90 degrees, summed per shell
+    point (patternsort).  hit: (S, K) bool, True = blocked."""
+    src = _f32(src).reshape(-1, 3)
+    snk = _f32(snk).reshape(-1, 3)
+    s_idx, k_idx = np.nonzero(~np.asarray(hit, bool))
+    d, _ = charge(src[s_idx], snk[k_idx])
+    ang = _signed_angle(d, np.asarray(nrm, np.float32)[s_idx])
+    va = np.atleast_1d(np.asarray(vertex_area, np.float64)).ravel()
+    va = np.where(np.isnan(va), 0.0, va)
+    if va.size == 1:
+        if va[0] == 0:
+            portion = np.ones(len(ang))
+        else:
+            portion = np.maximum(va[0] * np.abs(np.cos(ang)), 0)
+    else:
+        portion = va[s_idx] * np.abs(np.cos(ang))
+    portion[np.abs(ang) > np.pi / 2] = 0.0
+    out = np.zeros(len(snk), np.float64)
+    np.add.at(out, k_idx, portion)
+    return out.astype(np.float32)

@@ -208,3 +208,40 @@ def test_field_paths_vs_oracle(native, oracle, ncol):
     scale = np.abs(ref).max()
     assert np.abs(net - ref).max() < FIELD_RTOL * scale
     assert net.dtype == np.complex128 and net.shape == (S, K, 3)
+
+
+@pytest.mark.parametrize("area_mode", ["per_source", "zero"])
+def test_visiblespace_matches_oracle(native, oracle, area_mode):
+    """rayfunctions.visiblespace: GPU visibility + projected-area kernel vs the
+    oracle's visibility and the reference's angle / portion / patternsort."""
+    from lyceanem.mesh import Mesh
+    from lyceanem.raycasting import rayfunctions as RF
+    from lyceanem.geometry.targets import spherical_field
+
+    print(f"visiblespace {area_mode}", flush=True)
+    rng = np.random.default_rng(21)
+    g = (np.arange(8) - 3.5) * 0.1
+    X, Y = np.meshgrid(g, g)
+    src = np.stack([X.ravel(), Y.ravel(), np.full(64, 1e-4)], 1).astype(np.float32)
+    nrm = np.tile(np.array([0, 0, 1.0], np.float32), (64, 1))
+    tris = (np.array([0, 0, 0.6]) + rng.uniform(-0.5, 0.5, (30, 3, 3))).astype(np.float32)
+    env = Mesh(points=tris.reshape(-1, 3), cells=[("triangle", np.arange(90).reshape(30, 3))])
+    area = rng.uniform(0.005, 0.01, 64) if area_mode == "per_source" else 0
+    az, el = np.linspace(-180, 180, 37), np.linspace(-90, 90, 19)
+    pat, shell = RF.visiblespace(Mesh(points=src), nrm, env, vertex_area=area, az_range=az,
+                                 elev_range=el, shell_range=3.0)
+    snk = np.asarray(spherical_field(az, el, outward_normals=True, field_radius=3.0).points,
+                     np.float32)
+    hit, _ = oracle.visibility_dense(src, snk, tris.reshape(-1, 9), any_hit_only=True,
+                                     want_dist=False)
+    assert 0.05 < hit.mean() < 0.95
+    ref = oracle.projected_area(src, nrm, area, snk, hit).reshape(len(el), len(az))
+    assert pat.shape == ref.shape and pat.dtype == np.float32
+    tol = 1e-5 * np.abs(ref).max()
+    bad = np.abs(pat - ref) > tol
+    if area_mode == "zero":
+        # unit portions: a point whose dot product with the normal is within
+        # float32 rounding of 0 may count on one side and not the other
+        assert bad.sum() <= 2, np.argwhere(bad)
+    else:
+        np.testing.assert_allclose(pat, ref, rtol=1e-5, atol=tol)

@@ -196,3 +196,29 @@ def test_torch64_restatement_matches_c_oracle(oracle):
                                      torch.from_numpy(nrm.astype(np.float64)),
                                      torch.from_numpy(~hit), alpha, BETA).numpy()
         assert np.allclose(t, ref, rtol=1e-10, atol=1e-14 * np.abs(ref).max())
+
+
+def test_projected_area_closed_form(oracle):
+    """visiblespace's pattern of one unblocked source facing +z: area * cos(theta)
+    in the upper half-space, 0 below; with vertex_area == 0 every visible point
+    counts 1 (rayfunctions.py:405-417)."""
+    rng = np.random.default_rng(3)
+    d = rng.normal(size=(400, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    snk = (2.0 * d).astype(np.float32)
+    src = np.zeros((1, 3), np.float32)
+    nrm = np.array([[0, 0, 1]], np.float32)
+    free = np.zeros((1, 400), bool)
+    pat = oracle.projected_area(src, nrm, np.array([0.25]), snk, free)
+    np.testing.assert_allclose(pat, 0.25 * np.maximum(d[:, 2], 0), rtol=1e-5, atol=1e-6)
+    ones = oracle.projected_area(src, nrm, 0, snk, free)
+    np.testing.assert_array_equal(ones, (d[:, 2] > 1e-6).astype(np.float32) * (ones > 0))
+    assert ones[d[:, 2] > 1e-3].min() == 1.0 and ones[d[:, 2] < -1e-3].max() == 0.0
+    blocked = oracle.projected_area(src, nrm, np.array([0.25]), snk, np.ones((1, 400), bool))
+    assert not blocked.any()
+    # two sources add; per-source areas are used
+    src2 = np.array([[0, 0, 0], [0.1, 0, 0]], np.float32)
+    nrm2 = np.array([[0, 0, 1], [0, 0, 1]], np.float32)
+    both = oracle.projected_area(src2, nrm2, np.array([0.25, 0.5]), snk, np.zeros((2, 400), bool))
+    one = oracle.projected_area(src2[1:], nrm2[1:], np.array([0.5]), snk, np.zeros((1, 400), bool))
+    np.testing.assert_allclose(both, pat + one, rtol=1e-5, atol=1e-6)
