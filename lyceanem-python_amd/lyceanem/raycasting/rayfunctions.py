@@ -95,6 +95,13 @@ def rayHits1Dv2(ray_payload, point_indexing, sink_index):
 
 
 def _tris9(environment_local):
+    """(T, 9) float32 triangles of triangle_t records, a float array or meshes."""
+    if hasattr(environment_local, "points") or (
+            isinstance(environment_local, (list, tuple)) and environment_local
+            and hasattr(environment_local[0], "points")):
+        from ..geometry import geometryfunctions as GF
+
+        environment_local = GF.mesh_conversion(environment_local)
     return base_types.triangles_as_array(environment_local)
 
 

@@ -228,7 +228,10 @@ def test_visiblespace_matches_oracle(native, oracle, area_mode):
     env = Mesh(points=tris.reshape(-1, 3), cells=[("triangle", np.arange(90).reshape(30, 3))])
     area = rng.uniform(0.005, 0.01, 64) if area_mode == "per_source" else 0
     az, el = np.linspace(-180, 180, 37), np.linspace(-90, 90, 19)
-    pat, shell = RF.visiblespace(Mesh(points=src), nrm, env, vertex_area=area, az_range=az,
+    from lyceanem.geometry import geometryfunctions as GF
+
+    pat, shell = RF.visiblespace(Mesh(points=src), nrm, GF.mesh_conversion(env),
+                                 vertex_area=area, az_range=az,
                                  elev_range=el, shell_range=3.0)
     snk = np.asarray(spherical_field(az, el, outward_normals=True, field_radius=3.0).points,
                      np.float32)
