@@ -229,6 +229,20 @@ int lem_projected_area(const float *src, const float *nrm, const float *area, in
                        const float *snk, int64_t K, const uint64_t *hit_bits, float *out,
                        void *stream);
 
+/* Beamformed-pattern power of many weight rows on FP32 MFMA, the hot loop of
+ * MaximumDirectivityMap (beamforming.py:484-673): for each row b of W
+ * (nb x ne complex64), B = W[b] . Et and W[b] . Ep over the element patterns
+ * (ne x np complex64, points el-major / az-fastest); power[b] (float64) =
+ * sum_p q[p] (|Bt|^2 + |Bp|^2) and ustar[b] = (|Bt|^2, |Bp|^2) at point
+ * pstar[b] (float32; pstar < 0: not written).  nb a multiple of 32, ne of 32
+ * (zero-pad), np of 64 (zero-pad q).  Workspace: lem_beam_workspace bytes.
+ * No counterpart kernel in the reference (its GPUBeamformingMap is a stub,
+ * beamforming.py:1553-1562; the work is a numpy loop there). */
+int64_t lem_beam_workspace(int64_t nb, int64_t np);
+int lem_beam_power(const float *W, int64_t nb, const float *Et, const float *Ep, int64_t ne,
+                   int64_t np, const float *q, const int32_t *pstar, double *power, float *ustar,
+                   void *workspace, int64_t workspace_bytes, void *stream);
+
 /* ---- time domain (models/time_domain.py, TimeDomainv3) ------------------- */
 
 /* Per path row of a 1-based index (V x ncol int32, 0 = unused column, the

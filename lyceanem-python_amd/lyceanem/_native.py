@@ -45,6 +45,8 @@ SIGNATURES = {
     "lem_grid_fill": ([_P, _I64, _P, _P, _P, _P, _P, _P, _I64, ctypes.c_float, _P], _INT),
     "lem_visibility_grid": ([_P, _I64, _P, _I64, _P, _P, _P, _P, _INT, _INT, _P, _P, _P], _INT),
     "lem_visibility_grid_stats": ([_P, _I64, _P, _I64, _P, _P, _P, _P, _INT, _P, _P], _INT),
+    "lem_beam_workspace": ([_I64, _I64], _I64),
+    "lem_beam_power": ([_P, _I64, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _I64, _P], _INT),
     "lem_projected_area": ([_P, _P, _P, _I64, _P, _I64, _P, _P, _P], _INT),
     "lem_time_paths": ([_P, _P, _P, _I64, _P, _I64, _INT, _D, _D, _P, _P, _P, _P], _INT),
     "lem_time_wake": ([_P, _I64, _P, _P], _INT),
@@ -303,6 +305,39 @@ def projected_area(src, nrm, area, snk, hit_bits):
     check(lib.lem_projected_area(ptr(src), ptr(nrm), ptr(a), S, ptr(snk), K, ptr(bits),
                                  ptr(out), stream_ptr()), "lem_projected_area")
     return out
+
+
+def beam_power(W, Et, Ep, q, pstar):
+    """(power (nb,) float64, ustar (nb, 2) float32) of weight rows W (nb, ne)
+    over element patterns Et, Ep (ne, np); pads to the kernel's tile sizes."""
+    torch = _torch()
+    lib = load_library()
+    W = np.asarray(W, np.complex64)
+    Et = np.asarray(Et, np.complex64)
+    Ep = np.asarray(Ep, np.complex64)
+    nb, ne = W.shape
+    npts = Et.shape[1]
+    if Et.shape != (ne, npts) or Ep.shape != (ne, npts):
+        raise LemError("beam_power: patterns must be (elements, points)")
+    NB, NE, NP = -(-nb // 32) * 32, -(-ne // 32) * 32, -(-npts // 64) * 64
+    dev = device()
+    Wd = torch.zeros((NB, NE), dtype=torch.complex64, device=dev)
+    Wd[:nb, :ne] = torch.from_numpy(W).to(dev)
+    Etd = torch.zeros((NE, NP), dtype=torch.complex64, device=dev)
+    Etd[:ne, :npts] = torch.from_numpy(Et).to(dev)
+    Epd = torch.zeros((NE, NP), dtype=torch.complex64, device=dev)
+    Epd[:ne, :npts] = torch.from_numpy(Ep).to(dev)
+    qd = torch.zeros(NP, dtype=torch.float32, device=dev)
+    qd[:npts] = torch.from_numpy(np.asarray(q, np.float32)).to(dev)
+    ps = torch.full((NB,), -1, dtype=torch.int32, device=dev)
+    ps[:nb] = torch.from_numpy(np.asarray(pstar, np.int32)).to(dev)
+    power = torch.empty(NB, dtype=torch.float64, device=dev)
+    ustar = torch.zeros((NB, 2), dtype=torch.float32, device=dev)
+    wsb = int(lib.lem_beam_workspace(NB, NP))
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    check(lib.lem_beam_power(ptr(Wd), NB, ptr(Etd), ptr(Epd), NE, NP, ptr(qd), ptr(ps),
+                             ptr(power), ptr(ustar), ptr(ws), wsb, stream_ptr()), "lem_beam_power")
+    return power[:nb].cpu().numpy(), ustar[:nb].cpu().numpy()
 
 
 def time_paths(pos, nrm, w, index, alpha, beta):

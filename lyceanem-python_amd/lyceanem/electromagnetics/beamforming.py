@@ -1,0 +1,156 @@
+"""Beamforming consumers of the element-pattern network
+(lyceanem/electromagnetics/beamforming.py).
+
+MaximumDirectivityMap keeps the reference signature and result; its hot loop
+-- beamforming every element pattern with three weight sets per command angle
+and integrating the resulting power over the whole pattern -- runs as one
+FP32-MFMA contraction with a fused power epilogue (lem_beam.hip) instead of a
+numpy loop over command angles.  The weight builders and directivity_transform
+are host numpy restatements with the reference's arithmetic."""
+from __future__ import annotations
+
+import numpy as np
+
+from .. import _native
+
+LIGHT_SPEED = 299792458.0  # scipy.constants.speed_of_light
+
+
+def sph2cart(az, el, r):
+    """math_functions.py:46-52 (radians)."""
+    rc = r * np.cos(el)
+    return rc * np.cos(az), rc * np.sin(az), r * np.sin(el)
+
+
+def WavefrontWeights(source_coords, steering_vector, wavelength):
+    """Wavefront steering weights (beamforming.py:81-120): phase of the path
+    difference towards a point 1e9 m along the steering vector."""
+    p = np.asarray(source_coords, np.float64).reshape(-1, 3)
+    s = np.asarray(steering_vector, np.float64).ravel() * 1e9
+    dist = np.sqrt(np.abs((p[:, 0] - s[0]) ** 2 + (p[:, 1] - s[1]) ** 2 + (p[:, 2] - s[2]) ** 2))
+    dist = dist - np.min(dist)
+    delays = dist / LIGHT_SPEED
+    return np.exp(1j * 2 * np.pi * (LIGHT_SPEED / wavelength) * delays).astype(np.complex64)
+
+
+def EGCWeights(Etheta, Ephi, command_angles, polarization_switch="Etheta", az_range=None,
+               elev_range=None):
+    """Equal-gain-combining weights (beamforming.py:278-326): the conjugate phase
+    of each element's pattern at the grid point nearest the command angle."""
+    if az_range is None:
+        az_range = np.linspace(-180.0, 180.0, 19)
+    if elev_range is None:
+        elev_range = np.linspace(-90.0, 90.0, 19)
+    ai = np.argmin(np.abs(az_range - command_angles[0]))
+    ei = np.argmin(np.abs(elev_range - command_angles[1]))
+    E = Etheta if polarization_switch == "Etheta" else Ephi
+    ang = -np.angle(np.asarray(E)[:, ei, ai].astype(np.complex64))
+    return np.exp(1j * ang).astype(np.complex64)
+
+
+def WeightTruncation(weights, resolution):
+    """Phase quantisation to `resolution` bits (beamforming.py:1133-1147)."""
+    quant = 2 ** resolution
+    levels = np.round((quant - 1) * ((np.angle(weights) + np.pi) / (2 * np.pi))) / (quant - 1)
+    return np.abs(weights) * np.exp(1j * ((levels * 2 * np.pi) - np.pi))
+
+
+def _power_weights(az_range, elev_range):
+    """Per-point weights q of directivity_transform's power sum
+    (beamforming.py:1087-1110): each grid cell adds the mean of its four corner
+    powers times |sin(radians(az_j + az_j+1) / 2)| (the reference's factor, the
+    mean azimuth), so a point collects a quarter of that factor from each cell
+    it is a corner of.  Returned el-major, az-fastest."""
+    az = np.asarray(az_range, np.float64)
+    n_el, n_az = len(elev_range), len(az)
+    s = np.abs(np.sin(np.radians(az[:-1] + az[1:]) / 2.0))
+    q = np.zeros((n_el, n_az))
+    for di in (0, 1):
+        for dj in (0, 1):
+            q[di:n_el - 1 + di, dj:n_az - 1 + dj] += s[None, :] / 4.0
+    return q.ravel()
+
+
+def directivity_transform(Etheta, Ephi, az_range=None, elev_range=None,
+                          total_solid_angle=(4 * np.pi)):
+    """Directivity of a (n_el, n_az) pattern (beamforming.py:1037-1129):
+    returns (Dtheta, Dphi, Dtot, Dmax)."""
+    if az_range is None:
+        az_range = np.linspace(-180.0, 180.0, 19)
+    if elev_range is None:
+        elev_range = np.linspace(-90.0, 90.0, 19)
+    Ut = np.abs(np.asarray(Etheta) ** 2)
+    Up = np.abs(np.asarray(Ephi) ** 2)
+    Utot = Ut + Up
+    q = _power_weights(az_range, elev_range).reshape(Ut.shape)
+    power_sum = float(np.sum((Ut.astype(np.float32) + Up.astype(np.float32)) * q))
+    Uav = power_sum * (np.radians(az_range[1] - az_range[0]) *
+                       np.radians(elev_range[1] - elev_range[0])) / total_solid_angle
+    Umax = np.array([np.nanmax(Ut), np.nanmax(Up), np.nanmax(Utot)], np.float32)
+    return Ut / Uav, Up / Uav, Utot / Uav, Umax / Uav
+
+
+def MaximumDirectivityMap(Etheta, Ephi, source_coords, wavelength, az_range=None,
+                          elev_range=None, az_index=None, elev_index=None, forming="Total",
+                          total_solid_angle=(4 * np.pi), phase_resolution=24):
+    """Best achieved directivity per command angle over equal-gain combining on
+    Etheta, on Ephi and wavefront steering (beamforming.py:484-673).
+
+    Returns (n_elev_index, n_az_index, 3) float64: Dtheta, Dphi, Dtot of the
+    scheme that maximises the `forming` directivity.  As in the reference, the
+    value kept for command (elev_inc, az_inc) is read at grid point
+    (elev_inc, az_inc), the first scheme's directivity uses a 4 pi solid angle,
+    and the power sum weights cells by the sine of their mean azimuth."""
+    if az_range is None:
+        az_range = np.linspace(-180.0, 180.0, 19)
+    if elev_range is None:
+        elev_range = np.linspace(-90.0, 90.0, 19)
+    if elev_index is None:
+        elev_index = np.linspace(0, len(elev_range) - 1, len(elev_range)).astype(int)
+    if az_index is None:
+        az_index = np.linspace(0, len(az_range) - 1, len(az_range)).astype(int)
+    if forming not in ("Total", "Etheta", "Ephi"):
+        raise ValueError("forming must be 'Total', 'Etheta' or 'Ephi'")
+    Et = np.asarray(Etheta)
+    Ep = np.asarray(Ephi)
+    ne, n_el, n_az = Et.shape
+    az_res, el_res = len(az_index), len(elev_index)
+    pts = np.asarray(source_coords.points if hasattr(source_coords, "points") else source_coords)
+    Et64 = Et.astype(np.complex64)
+    Ep64 = Ep.astype(np.complex64)
+    # weight rows: command (az_inc, elev_inc), then scheme (EGC theta, EGC phi, wavefront)
+    cmd_az = np.asarray(az_range)[np.asarray(az_index)].astype(np.float32)
+    cmd_el = np.asarray(elev_range)[np.asarray(elev_index)].astype(np.float32)
+    W = np.zeros((az_res, el_res, 3, ne), np.complex64)
+    pstar = np.zeros((az_res, el_res, 3), np.int32)
+    for a in range(az_res):
+        ai = np.argmin(np.abs(az_range - cmd_az[a]))
+        for e in range(el_res):
+            ei = np.argmin(np.abs(elev_range - cmd_el[e]))
+            sv = np.array(sph2cart(np.radians(cmd_az[a]), np.radians(cmd_el[e]), 1))
+            W[a, e, 0] = np.exp(1j * -np.angle(Et64[:, ei, ai]))
+            W[a, e, 1] = np.exp(1j * -np.angle(Ep64[:, ei, ai]))
+            W[a, e, 2] = WavefrontWeights(pts, sv, wavelength)
+            pstar[a, e, :] = e * n_az + a
+    if phase_resolution <= 12:
+        W = WeightTruncation(W, phase_resolution).astype(np.complex64)
+    q = _power_weights(az_range, elev_range)
+    power, ustar = _native.beam_power(W.reshape(-1, ne), Et64.reshape(ne, -1),
+                                      Ep64.reshape(ne, -1), q, pstar.reshape(-1))
+    power = power.reshape(az_res, el_res, 3)
+    ustar = ustar.reshape(az_res, el_res, 3, 2).astype(np.float64)
+    d_area = np.radians(az_range[1] - az_range[0]) * np.radians(elev_range[1] - elev_range[0])
+    tsa = np.array([4 * np.pi, total_solid_angle, total_solid_angle])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        Uav = power * d_area / tsa[None, None, :]
+        Dt = ustar[..., 0] / Uav
+        Dp = ustar[..., 1] / Uav
+        Dtot = (ustar[..., 0] + ustar[..., 1]) / Uav
+    comp = {"Total": Dtot, "Etheta": Dt, "Ephi": Dp}[forming]
+    best = np.argmax(comp, axis=2)
+    out = np.zeros((el_res, az_res, 3))
+    ia, ie = np.meshgrid(np.arange(az_res), np.arange(el_res), indexing="ij")
+    out[ie, ia, 0] = Dt[ia, ie, best]
+    out[ie, ia, 1] = Dp[ia, ie, best]
+    out[ie, ia, 2] = Dtot[ia, ie, best]
+    return out

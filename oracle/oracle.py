@@ -445,3 +445,60 @@ def projected_area(src, nrm, vertex_area, snk, hit):
     out = np.zeros(len(snk), np.float64)
     np.add.at(out, k_idx, portion)
     return out.astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# beamforming (electromagnetics/beamforming.py)
+# ---------------------------------------------------------------------------
+
+def directivity_transform(Etheta, Ephi, az_range, elev_range, total_solid_angle=4 * np.pi):
+    """directivity_transform (beamforming.py:1037-1129) as its cell loop:
+    power_sum += mean(4 corner Utheta) |sin(rad(az_j + az_j+1) / 2)| + the same
+    for Uphi; Uav = power_sum * d_az * d_el / total_solid_angle (radians)."""
+    Ut = np.abs(np.asarray(Etheta, np.complex128) ** 2)
+    Up = np.abs(np.asarray(Ephi, np.complex128) ** 2)
+    power_sum = 0.0
+    for i in range(len(elev_range) - 1):
+        for j in range(len(az_range) - 1):
+            s = abs(np.sin(np.radians(az_range[j] + az_range[j + 1]) / 2.0))
+            r1 = np.mean([Ut[i, j], Ut[i + 1, j], Ut[i, j + 1], Ut[i + 1, j + 1]])
+            r2 = np.mean([Up[i, j], Up[i + 1, j], Up[i, j + 1], Up[i + 1, j + 1]])
+            power_sum += r1 * s + r2 * s
+    Uav = power_sum * (np.radians(az_range[1] - az_range[0]) *
+                       np.radians(elev_range[1] - elev_range[0])) / total_solid_angle
+    return Ut / Uav, Up / Uav, (Ut + Up) / Uav
+
+
+def max_directivity_map(Etheta, Ephi, points, wavelength, az_range, elev_range,
+                        forming="Total", total_solid_angle=4 * np.pi):
+    """MaximumDirectivityMap (beamforming.py:484-673) with every grid point a
+    command angle, in complex128: per command, equal-gain weights on Etheta and
+    on Ephi and wavefront weights, each beamformed pattern's directivity (the
+    first with the default 4 pi solid angle), the best by `forming` read at
+    (elev_inc, az_inc)."""
+    Et = np.asarray(Etheta, np.complex64)
+    Ep = np.asarray(Ephi, np.complex64)
+    pts = np.asarray(points, np.float64)
+    c = 299792458.0
+    out = np.zeros((len(elev_range), len(az_range), 3))
+    for a, az in enumerate(az_range):
+        for e, el in enumerate(elev_range):
+            sv = np.array([np.cos(np.radians(el)) * np.cos(np.radians(az)),
+                           np.cos(np.radians(el)) * np.sin(np.radians(az)),
+                           np.sin(np.radians(el))])
+            dist = np.sqrt(np.sum((pts - sv * 1e9) ** 2, axis=1))
+            dist = dist - dist.min()
+            ws = np.exp(1j * 2 * np.pi * (c / wavelength) * (dist / c))
+            sets = [np.exp(-1j * np.angle(Et[:, e, a])), np.exp(-1j * np.angle(Ep[:, e, a])), ws]
+            res = []
+            for n, w in enumerate(sets):
+                w = w.astype(np.complex64).astype(np.complex128)
+                bt = np.tensordot(w, Et.astype(np.complex128), axes=(0, 0))
+                bp = np.tensordot(w, Ep.astype(np.complex128), axes=(0, 0))
+                tsa = 4 * np.pi if n == 0 else total_solid_angle
+                dt, dp, dtot = directivity_transform(bt, bp, az_range, elev_range, tsa)
+                res.append((dt[e, a], dp[e, a], dtot[e, a]))
+            col = {"Total": 2, "Etheta": 0, "Ephi": 1}[forming]
+            best = int(np.argmax([r[col] for r in res]))
+            out[e, a] = res[best]
+    return out
