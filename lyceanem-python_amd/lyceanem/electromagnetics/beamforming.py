@@ -90,6 +90,43 @@ def directivity_transform(Etheta, Ephi, az_range=None, elev_range=None,
     return Ut / Uav, Up / Uav, Utot / Uav, Umax / Uav
 
 
+def command_weights(Et, Ep, points, wavelength, az_range, elev_range, az_index, elev_index,
+                    phase_resolution=24):
+    """Weight rows of MaximumDirectivityMap, (n_az_index, n_elev_index, 3, ne)
+    complex64 -- equal-gain combining on Etheta, on Ephi and wavefront steering
+    for each command angle (beamforming.py:560-600) -- and the grid point each
+    row's directivity is read at, (elev_inc, az_inc) flattened el-major."""
+    ne, n_el, n_az = Et.shape
+    cmd_az = np.asarray(az_range)[np.asarray(az_index)].astype(np.float32)
+    cmd_el = np.asarray(elev_range)[np.asarray(elev_index)].astype(np.float32)
+    ai = np.argmin(np.abs(np.asarray(az_range)[None, :] - cmd_az[:, None]), axis=1)
+    ei = np.argmin(np.abs(np.asarray(elev_range)[None, :] - cmd_el[:, None]), axis=1)
+    A, E = len(cmd_az), len(cmd_el)
+    W = np.empty((A, E, 3, ne), np.complex64)
+    # Et[:, ei, ai] for every (command az, command el)
+    W[:, :, 0] = np.exp(1j * -np.angle(Et[:, ei[None, :], ai[:, None]])).transpose(1, 2, 0)
+    W[:, :, 1] = np.exp(1j * -np.angle(Ep[:, ei[None, :], ai[:, None]])).transpose(1, 2, 0)
+    # the reference's command angles are float32, so is its steering vector
+    az_r = np.radians(cmd_az)[:, None]
+    el_r = np.radians(cmd_el)[None, :]
+    sv = np.stack(np.broadcast_arrays(*sph2cart(az_r, el_r, np.float32(1))), -1)
+    sv = sv.reshape(-1, 3).astype(np.float64)
+    p = np.asarray(points, np.float64).reshape(-1, 3)
+    for i in range(0, len(sv), 4096):
+        s = sv[i:i + 4096] * 1e9
+        dist = np.sqrt(np.abs((p[None, :, 0] - s[:, None, 0]) ** 2 +
+                              (p[None, :, 1] - s[:, None, 1]) ** 2 +
+                              (p[None, :, 2] - s[:, None, 2]) ** 2))
+        dist = dist - dist.min(axis=1, keepdims=True)
+        w = np.exp(1j * 2 * np.pi * (LIGHT_SPEED / wavelength) * (dist / LIGHT_SPEED))
+        W.reshape(-1, 3, ne)[i:i + 4096, 2] = w.astype(np.complex64)
+    if phase_resolution <= 12:
+        W = WeightTruncation(W, phase_resolution).astype(np.complex64)
+    a_idx, e_idx = np.meshgrid(np.arange(A), np.arange(E), indexing="ij")
+    pstar = np.repeat((e_idx * n_az + a_idx)[..., None], 3, axis=2).astype(np.int32)
+    return W, pstar
+
+
 def MaximumDirectivityMap(Etheta, Ephi, source_coords, wavelength, az_range=None,
                           elev_range=None, az_index=None, elev_index=None, forming="Total",
                           total_solid_angle=(4 * np.pi), phase_resolution=24):
@@ -118,22 +155,8 @@ def MaximumDirectivityMap(Etheta, Ephi, source_coords, wavelength, az_range=None
     pts = np.asarray(source_coords.points if hasattr(source_coords, "points") else source_coords)
     Et64 = Et.astype(np.complex64)
     Ep64 = Ep.astype(np.complex64)
-    # weight rows: command (az_inc, elev_inc), then scheme (EGC theta, EGC phi, wavefront)
-    cmd_az = np.asarray(az_range)[np.asarray(az_index)].astype(np.float32)
-    cmd_el = np.asarray(elev_range)[np.asarray(elev_index)].astype(np.float32)
-    W = np.zeros((az_res, el_res, 3, ne), np.complex64)
-    pstar = np.zeros((az_res, el_res, 3), np.int32)
-    for a in range(az_res):
-        ai = np.argmin(np.abs(az_range - cmd_az[a]))
-        for e in range(el_res):
-            ei = np.argmin(np.abs(elev_range - cmd_el[e]))
-            sv = np.array(sph2cart(np.radians(cmd_az[a]), np.radians(cmd_el[e]), 1))
-            W[a, e, 0] = np.exp(1j * -np.angle(Et64[:, ei, ai]))
-            W[a, e, 1] = np.exp(1j * -np.angle(Ep64[:, ei, ai]))
-            W[a, e, 2] = WavefrontWeights(pts, sv, wavelength)
-            pstar[a, e, :] = e * n_az + a
-    if phase_resolution <= 12:
-        W = WeightTruncation(W, phase_resolution).astype(np.complex64)
+    W, pstar = command_weights(Et64, Ep64, pts, wavelength, az_range, elev_range, az_index,
+                               elev_index, phase_resolution)
     q = _power_weights(az_range, elev_range)
     power, ustar = _native.beam_power(W.reshape(-1, ne), Et64.reshape(ne, -1),
                                       Ep64.reshape(ne, -1), q, pstar.reshape(-1))

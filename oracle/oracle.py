@@ -483,9 +483,11 @@ def max_directivity_map(Etheta, Ephi, points, wavelength, az_range, elev_range,
     out = np.zeros((len(elev_range), len(az_range), 3))
     for a, az in enumerate(az_range):
         for e, el in enumerate(elev_range):
-            sv = np.array([np.cos(np.radians(el)) * np.cos(np.radians(az)),
-                           np.cos(np.radians(el)) * np.sin(np.radians(az)),
-                           np.sin(np.radians(el))])
+            # command_angles is a float32 array (beamforming.py:557): so are the
+            # radians and the steering vector sph2cart builds from them
+            ra, re_ = np.radians(np.float32(az)), np.radians(np.float32(el))
+            sv = np.array([np.cos(re_) * np.cos(ra), np.cos(re_) * np.sin(ra), np.sin(re_)],
+                          np.float64)
             dist = np.sqrt(np.sum((pts - sv * 1e9) ** 2, axis=1))
             dist = dist - dist.min()
             ws = np.exp(1j * 2 * np.pi * (c / wavelength) * (dist / c))

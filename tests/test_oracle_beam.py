@@ -67,3 +67,25 @@ def test_weight_builders():
     # 1-bit truncation maps phases onto {-pi, pi}
     t = WeightTruncation(np.exp(1j * np.array([0.1, 2.0, -2.0])), 1)
     np.testing.assert_allclose(np.abs(np.imag(t)), 0.0, atol=1e-12)
+
+
+def test_command_weights_match_per_command_builders():
+    from lyceanem.electromagnetics import beamforming as BF
+
+    rng = np.random.default_rng(2)
+    az, el = np.linspace(-180, 180, 13), np.linspace(-90, 90, 7)
+    Et = (rng.normal(size=(5, 7, 13)) + 1j * rng.normal(size=(5, 7, 13))).astype(np.complex64)
+    Ep = (rng.normal(size=(5, 7, 13)) + 1j * rng.normal(size=(5, 7, 13))).astype(np.complex64)
+    pts = rng.uniform(-0.2, 0.2, (5, 3))
+    ai, ei = np.arange(0, 13, 2), np.arange(7)
+    W, pstar = BF.command_weights(Et, Ep, pts, 0.1, az, el, ai, ei)
+    for a, ia in enumerate(ai):
+        for e, ie in enumerate(ei):
+            cmd = np.array([az[ia], el[ie]], np.float32)
+            sv = np.array([BF.sph2cart(np.radians(cmd[0]), np.radians(cmd[1]), 1)])
+            np.testing.assert_allclose(W[a, e, 0], BF.EGCWeights(Et, Ep, cmd, "Etheta", az, el),
+                                       atol=1e-6)
+            np.testing.assert_allclose(W[a, e, 1], BF.EGCWeights(Et, Ep, cmd, "Ephi", az, el),
+                                       atol=1e-6)
+            np.testing.assert_allclose(W[a, e, 2], BF.WavefrontWeights(pts, sv, 0.1), atol=1e-6)
+            assert (pstar[a, e] == e * 13 + a).all()
