@@ -594,7 +594,102 @@ class TD:
                             "65,536-sample trace", "paths": self.V, "samples": self.N}
 
 
-CONFIGS = {"c1": C1, "c2": C2, "c3": C3, "c4": C4, "c5": C5, "td": TD}
+# --------------------------------------------------------------------------
+# beamforming: MaximumDirectivityMap's contraction for every command angle
+# --------------------------------------------------------------------------
+
+class BF:
+    name = "bf"
+
+    def __init__(self, torch):
+        from lyceanem import _native, scenes
+        from lyceanem.electromagnetics import beamforming as B
+
+        self.torch, self.nat = torch, _native
+        self.lib = _native.load_library()
+        sc = scenes.beamforming_bf()
+        self.sc = sc
+        Et, Ep = sc["Et"], sc["Ep"]
+        ne, n_el, n_az = Et.shape
+        ai, ei = np.arange(n_az), np.arange(n_el)
+        W, pstar = B.command_weights(Et, Ep, sc["points"], sc["wavelength"], sc["az"], sc["el"],
+                                     ai, ei)
+        q = B._power_weights(sc["az"], sc["el"])
+        nb, npts = W.shape[0] * W.shape[1] * 3, n_el * n_az
+        NB, NE, NP = -(-nb // 32) * 32, -(-ne // 32) * 32, -(-npts // 64) * 64
+        dev = _native.device()
+        self.W = torch.zeros((NB, NE), dtype=torch.complex64, device=dev)
+        self.W[:nb, :ne] = torch.from_numpy(W.reshape(nb, ne)).to(dev)
+        self.Et = torch.zeros((NE, NP), dtype=torch.complex64, device=dev)
+        self.Et[:ne, :npts] = torch.from_numpy(Et.reshape(ne, npts)).to(dev)
+        self.Ep = torch.zeros((NE, NP), dtype=torch.complex64, device=dev)
+        self.Ep[:ne, :npts] = torch.from_numpy(Ep.reshape(ne, npts)).to(dev)
+        self.q = torch.zeros(NP, dtype=torch.float32, device=dev)
+        self.q[:npts] = torch.from_numpy(q.astype(np.float32)).to(dev)
+        self.ps = torch.full((NB,), -1, dtype=torch.int32, device=dev)
+        self.ps[:nb] = torch.from_numpy(pstar.reshape(-1)).to(dev)
+        self.power = torch.empty(NB, dtype=torch.float64, device=dev)
+        self.ustar = torch.zeros((NB, 2), dtype=torch.float32, device=dev)
+        self.wsb = int(self.lib.lem_beam_workspace(NB, NP))
+        self.ws = torch.empty(self.wsb, dtype=torch.uint8, device=dev)
+        self.NB, self.NE, self.NP = NB, NE, NP
+        self.ncmd, self.ne, self.npts = n_el * n_az, ne, npts
+
+    def rays(self):
+        return self.ncmd
+
+    def step(self, ev=None):
+        nat, p = self.nat, self.nat.ptr
+        if ev is not None:
+            ev[0].record()
+        nat.check(self.lib.lem_beam_power(p(self.W), self.NB, p(self.Et), p(self.Ep), self.NE,
+                                          self.NP, p(self.q), p(self.ps), p(self.power),
+                                          p(self.ustar), p(self.ws), self.wsb, nat.stream_ptr()),
+                  "beam_power")
+        if ev is not None:
+            ev[1].record()
+            ev[2].record()
+
+    def record(self, times_ms):
+        t = float(np.mean([a for a, _ in times_ms]))
+        flops = 8.0 * self.NB * self.NE * self.NP * 2
+        return {
+            "kernel": "k_beam_power",
+            "kernel_ms": t,
+            "roofline": {"bound": "mfma", "achieved": flops / (t * 1e-3) / 1e12,
+                         "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                         "frac": flops / (t * 1e-3) / PEAK_FP32, "traffic": None},
+            "extra": {"mfma_flops": flops, "weight_rows": self.ncmd * 3, "elements": self.ne,
+                      "points": self.npts},
+        }
+
+    def cpu_baseline(self, budget_s):
+        # the reference's numpy beamforming of one command: three weight sets x
+        # np.sum(w.reshape(n, 1, 1) * E, axis=0) for both polarisations
+        sc = self.sc
+        Et, Ep = sc["Et"].astype(np.complex128), sc["Ep"].astype(np.complex128)
+        rng = np.random.default_rng(0)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget_s / 2 or n < 2:
+            w = np.exp(1j * rng.uniform(-np.pi, np.pi, (3, Et.shape[0])))
+            for k in range(3):
+                np.sum(w[k].reshape(-1, 1, 1) * Et, axis=0)
+                np.sum(w[k].reshape(-1, 1, 1) * Ep, axis=0)
+            n += 1
+        dt = time.perf_counter() - t0
+        return {"value": n / dt, "unit": "commands/s", "cores": 1, "kind": "port",
+                "sample": f"{n} command angles of {self.ncmd}: the reference's numpy "
+                          f"beamforming of {Et.shape[0]} element patterns on a "
+                          f"{Et.shape[1]}x{Et.shape[2]} grid with 3 weight sets, {dt:.1f} s "
+                          f"(the directivity integrals not included)"}
+
+    def config(self):
+        return {"workload": "bf: 32x32 array at 3 GHz, 91x181 pattern grid, every grid point "
+                            "a command angle x 3 weight schemes (MaximumDirectivityMap)",
+                "commands": self.ncmd, "elements": self.ne, "points": self.npts}
+
+
+CONFIGS = {"c1": C1, "c2": C2, "c3": C3, "c4": C4, "c5": C5, "td": TD, "bf": BF}
 DEFAULT_CONFIG = "c2"
 
 
@@ -657,8 +752,8 @@ def main(argv=None) -> int:
         print(f"[bench] cpu baseline: {cpu['value']:.3f} {cpu['unit']}", file=sys.stderr,
               flush=True)
     unit = {"c3": "Gpairs/s", "c4": "Gpairs/s", "c5": "G (pair x set)/s",
-            "td": "Gpaths/s"}.get(a.config, "Mrays/s")
-    scale = 1e6 if unit == "Mrays/s" else 1e9
+            "td": "Gpaths/s", "bf": "commands/s"}.get(a.config, "Mrays/s")
+    scale = {"Mrays/s": 1e6, "commands/s": 1.0}.get(unit, 1e9)
     line = {
         "metric": "Mrays/s (1M-tri DDA); Gpairs/s scattering + % roofline; adjoint/forward time",
         "value": cfg.rays() / (ms * 1e-3) / scale,

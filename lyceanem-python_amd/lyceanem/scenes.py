@@ -275,3 +275,29 @@ def time_domain_td(n_src_side=64, n_snk=256, freq=3e9, fs=40e9, n_samples=65536,
     exc = np.sin(2 * np.pi * freq * t) * np.exp(-0.5 * (t / sig) ** 2)
     return dict(pos=pos, nrm=nrm, w=weights, index=index, exc=exc, fs=fs, n_samples=n_samples,
                 beta=2 * np.pi / lam, S=S, K=n_snk)
+
+
+def beamforming_bf(n_side=32, freq=3e9, n_el=91, n_az=181, coupling=0.05, seed=14):
+    """Directivity-map workload: an n x n array at lambda/2 in z = 0 with
+    element patterns Etheta = max(cos(angle off +z), 0.05) e^{jk p.u} and
+    Ephi = 0.5 cos(az) e^{jk p.u} plus seeded random coupling, on an
+    n_el x n_az grid (el-major, az-fastest); every grid point is a command
+    angle."""
+    lam = C0 / freq
+    g = (np.arange(n_side) - (n_side - 1) / 2) * lam / 2
+    X, Y = np.meshgrid(g, g)
+    pts = np.stack([X.ravel(), Y.ravel(), np.zeros(n_side * n_side)], 1)
+    az = np.linspace(-180.0, 180.0, n_az)
+    el = np.linspace(-90.0, 90.0, n_el)
+    A, E = np.meshgrid(np.radians(az), np.radians(el))
+    u = np.stack([np.cos(E) * np.cos(A), np.cos(E) * np.sin(A), np.sin(E)], -1).reshape(-1, 3)
+    phase = np.exp(1j * (2 * np.pi / lam) * (pts @ u.T)).astype(np.complex64)
+    rng = np.random.default_rng(seed)
+    n, P = len(pts), n_el * n_az
+    ef = np.clip(u[:, 2], 0.05, None).astype(np.float32)
+    noise = lambda: (coupling * (rng.standard_normal((n, P), np.float32)
+                                 + 1j * rng.standard_normal((n, P), np.float32))).astype(np.complex64)
+    Et = (ef[None, :] * phase + noise()).reshape(n, n_el, n_az)
+    Ep = ((0.5 * np.cos(A).ravel()).astype(np.float32)[None, :] * phase + noise()).reshape(
+        n, n_el, n_az)
+    return dict(points=pts, Et=Et, Ep=Ep, az=az, el=el, wavelength=lam)
