@@ -21,6 +21,10 @@ def _table(rng, n, delays=True):
 
 def _index(rng, n, V, ncol):
     idx = rng.integers(1, n + 1, (V, ncol)).astype(np.int32)
+    for c in range(1, ncol):
+        # no zero-length legs (the reference's loss is then infinite too)
+        same = idx[:, c] == idx[:, c - 1]
+        idx[same, c] = idx[same, c] % n + 1
     if ncol > 2:
         # trailing zero columns on part of the rows (line-of-sight rows of a
         # wider index), which the reference reads as point -1
@@ -39,9 +43,9 @@ def test_time_paths_match_oracle(native, oracle, ncol):
     amp, arr = native.time_paths(pos, nrm, w, idx, 0.02, 2 * np.pi / 0.03)
     amp_o, arr_o = oracle.time_paths(pos, nrm, w, idx, 0.02, 2 * np.pi / 0.03)
     amp, arr = amp.cpu().numpy(), arr.cpu().numpy()
-    scale = np.abs(amp_o).max(axis=1, keepdims=True)
-    assert np.abs(amp - amp_o).max() <= 1e-12 * max(scale.max(), 1e-300)
-    np.testing.assert_allclose(amp, amp_o, rtol=1e-10, atol=1e-13 * scale.max())
+    assert np.array_equal(np.isnan(amp), np.isnan(amp_o))
+    scale = np.nanmax(np.abs(amp_o))
+    np.testing.assert_allclose(amp, amp_o, rtol=1e-10, atol=1e-13 * scale)
     np.testing.assert_allclose(arr, arr_o, rtol=1e-14, atol=0)
 
 
