@@ -556,6 +556,8 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
     float Tz = tnext(last(jz, sz, G.nz), sz, G.loz, G.cz, r.oz, idz);
     bool hit = false;
     double tmin = (double)r.len;  // closest hit so far (closest-hit mode)
+    double t_next = __builtin_huge_val();  // closest hit of another triangle
+    int32_t id_min = -1;
     bool fine = true;   // (ix, iy, iz) is the walk's cell in block (jx, jy, jz)
     int entry_axis = 0;
     float t_entry = 0.0f;
@@ -643,12 +645,22 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
                         dist = (float)tt;
                         return true;
                     }
-                } else if (hit_numba(r, r.len, t, tt) && tt < tmin) {
-                    // exact closest hit; the order-dependent distance is
-                    // replayed once the walk is over
-                    tmin = tt;
-                    dist = (float)tt;
-                    hit = true;
+                } else if (hit_numba(r, r.len, t, tt)) {
+                    // exact closest hit and the closest hit of any other
+                    // triangle (a triangle listed in several cells counts once)
+                    const int32_t id = rec_id(recs, j);
+                    if (id != id_min) {
+                        if (tt < tmin) {
+                            if (hit)
+                                t_next = fmin(t_next, tmin);
+                            tmin = tt;
+                            id_min = id;
+                            dist = (float)tt;
+                            hit = true;
+                        } else if (tt < t_next) {
+                            t_next = tt;
+                        }
+                    }
                 }
             }
             const float texit = fminf(tx, fminf(ty, tz));
@@ -688,8 +700,14 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
         if (done)
             break;
     }
-    if (!ANY && SEM == 0 && hit)
-        dist = replay_window(r, tmin, G, cell_start, recs);
+    if (!ANY && SEM == 0 && hit) {
+        // another triangle's hit close enough to matter: replay kernel1Dv3's
+        // window; alone, the closest hit is the answer
+        const float fs = (float)tmin;
+        const double delta = 8.0 * (2e-7 + (double)(nextafterf(fs, __builtin_huge_valf()) - fs));
+        if (t_next <= tmin + delta)
+            dist = replay_window(r, tmin, G, cell_start, recs);
+    }
     return hit;
 }
 

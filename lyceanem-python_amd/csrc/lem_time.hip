@@ -381,13 +381,16 @@ inline int64_t trace_layout(int64_t V, int64_t N, size_t cub_bytes, char *base, 
     return off;
 }
 
-inline size_t cub_sort_bytes(int64_t V, int64_t N)
+// Scratch bytes of the radix sort, or -1 when the size query fails.
+inline int64_t cub_sort_bytes(int64_t V, int64_t N)
 {
     size_t bytes = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const int32_t *)nullptr,
-                                       (int32_t *)nullptr, (const int32_t *)nullptr,
-                                       (int32_t *)nullptr, (int)V, 0, key_bits(N));
-    return bytes;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const int32_t *)nullptr,
+                                           (int32_t *)nullptr, (const int32_t *)nullptr,
+                                           (int32_t *)nullptr, (int)V, 0,
+                                           key_bits(N)) != hipSuccess)
+        return -1;
+    return (int64_t)bytes;
 }
 
 }  // namespace lem
@@ -467,7 +470,10 @@ extern "C" int64_t lem_time_trace_workspace(int64_t V, int64_t N)
 {
     if (V < 0 || N < 0 || V > 2000000000LL || N > 2000000000LL)
         return -1;
-    return trace_layout(V, N, cub_sort_bytes(V > 0 ? V : 1, N), nullptr, nullptr);
+    const int64_t cub = cub_sort_bytes(V > 0 ? V : 1, N);
+    if (cub < 0)
+        return -1;
+    return trace_layout(V, N, (size_t)cub, nullptr, nullptr);
 }
 
 extern "C" int lem_time_trace(const double *amp, const int64_t *bin, int64_t V,
@@ -482,7 +488,12 @@ extern "C" int lem_time_trace(const double *amp, const int64_t *bin, int64_t V,
     LEM_REQUIRE(out && (V == 0 || (amp && bin)) && (E == 0 || excitation),
                 "lem_time_trace: null pointer");
     const int64_t Vs = V > 0 ? V : 1;
-    const size_t cub_bytes = cub_sort_bytes(Vs, N);
+    const int64_t cub = cub_sort_bytes(Vs, N);
+    if (cub < 0) {
+        set_error("lem_time_trace: radix sort size query failed");
+        return E_HIP;
+    }
+    const size_t cub_bytes = (size_t)cub;
     const int64_t need = trace_layout(Vs, N, cub_bytes, nullptr, nullptr);
     if (!workspace || workspace_bytes < need) {
         set_error("lem_time_trace: workspace smaller than lem_time_trace_workspace()");

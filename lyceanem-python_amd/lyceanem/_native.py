@@ -424,7 +424,8 @@ def time_trace(amp, bins, excitation, N):
     V = amp.shape[0]
     wsb = int(lib.lem_time_trace_workspace(V, int(N)))
     if wsb < 0:
-        raise LemError("time_trace: too many paths or samples")
+        raise LemError("time_trace: too many paths or samples, or no HIP device for the "
+                       "sort's size query")
     ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=amp.device)
     check(lib.lem_time_trace(ptr(amp), ptr(bins), V, ptr(exc), exc.shape[0], int(N), ptr(out),
                              ptr(ws), wsb, stream_ptr()), "lem_time_trace")
