@@ -141,8 +141,9 @@ typedef struct {
  * 2) the caller allocates cell_start (ncells + 1 int32), tri_ids (nrefs int32),
  *    recs (nrefs x 12 float32: v0, e1, e2, the triangle id as int32 bits,
  *    padding; cell by cell, ids ascending within a cell) and
- *    cell_box ((ncells + nmacro) x 8 float32: the tight box of each cell's
- *    contents, lo.xyz, 0, hi.xyz, 0, then the same for each macro block), and
+ *    cell_box ((ncells + nmacro) x 8 + nmacro x 2 float32: the tight box of
+ *    each cell's contents, lo.xyz, 0, hi.xyz, 0, then the same for each macro
+ *    block, then each macro block's box quantised to 8 bytes), and
  * 3) lem_grid_fill writes them; the same workspace must be passed to both. */
 int64_t lem_grid_workspace(int64_t T, float cells_per_tri);
 int lem_grid_plan(const float *tris, int64_t T, float cells_per_tri, lem_grid_info *info,

@@ -34,10 +34,13 @@ namespace lem {
 
 constexpr int GB = 256;
 
-// Macro blocks hold 2^LEM_GRID_MSHIFT cells per axis.
+// Macro blocks hold 2^mshift cells per axis: the smallest shift from
+// LEM_GRID_MSHIFT up for which the quantised macro table (8 bytes a block)
+// fits the LDS budget of the persistent traversal.
 #ifndef LEM_GRID_MSHIFT
 #define LEM_GRID_MSHIFT 2
 #endif
+constexpr int MACRO_LDS_MAX = 4096;  // blocks (32 KB of LDS)
 
 // Order-preserving float <-> uint key (for atomic min / max of floats).
 LEM_DEV unsigned int fkey(float f)
@@ -102,6 +105,7 @@ struct GridDev {
     float eps;
     int mshift;           // macro blocks of 2^mshift cells per axis
     int mx, my, mz;       // macro blocks per axis
+    float qx, qy, qz;     // macro-block extent / 255 (quantised macro boxes)
     int64_t ncells;       // macro boxes follow the cell boxes
 };
 
@@ -118,6 +122,9 @@ inline GridDev to_dev(const lem_grid_info &g)
     d.mx = (g.n[0] + B - 1) / B;
     d.my = (g.n[1] + B - 1) / B;
     d.mz = (g.n[2] + B - 1) / B;
+    d.qx = d.cx * (float)B / 255.0f;
+    d.qy = d.cy * (float)B / 255.0f;
+    d.qz = d.cz * (float)B / 255.0f;
     d.ncells = g.ncells;
     return d;
 }
@@ -342,6 +349,43 @@ __global__ __launch_bounds__(GB) void k_macro_boxes(GridDev G, float4 *__restric
     box[2 * (G.ncells + m) + 1] = hi;
 }
 
+// Macro box m quantised to 8 bytes: per axis, lo and hi as steps of
+// extent / 255 from the block's origin, rounded outwards by a full step and
+// clamped to the block (x: lo, y: lo, z: lo, x: hi | y: hi, z: hi); an empty
+// block stores lo > hi.  Decoded with the same float32 origin and step
+// (macro_lo / macro_step below), the box covers the part of the union box that
+// lies in the block -- a hit inside the block lies inside it.
+LEM_DEV float macro_origin(float lo, float c, int j, int ms) { return lo + c * (float)(j << ms); }
+
+__global__ __launch_bounds__(GB) void k_macro_quant(GridDev G, const float4 *__restrict__ box,
+                                                    uint2 *__restrict__ mq)
+{
+    const int64_t m = (int64_t)blockIdx.x * GB + threadIdx.x;
+    const int64_t nm = (int64_t)G.mx * G.my * G.mz;
+    if (m >= nm)
+        return;
+    const int jx = (int)(m % G.mx), jy = (int)((m / G.mx) % G.my), jz = (int)(m / ((int64_t)G.mx * G.my));
+    const float4 lo = box[2 * (G.ncells + m)], hi = box[2 * (G.ncells + m) + 1];
+    if (!(lo.x <= hi.x)) {
+        mq[m] = make_uint2(0x00ffffffu, 0u);  // lo = 255 > hi = 0 on every axis
+        return;
+    }
+    const float o[3] = {macro_origin(G.lox, G.cx, jx, G.mshift),
+                        macro_origin(G.loy, G.cy, jy, G.mshift),
+                        macro_origin(G.loz, G.cz, jz, G.mshift)};
+    const float st[3] = {G.qx, G.qy, G.qz};
+    const float l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
+    unsigned int ql[3], qh[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double fl = floor(((double)l[a] - (double)o[a]) / (double)st[a]) - 1.0;
+        const double fh = ceil(((double)h[a] - (double)o[a]) / (double)st[a]) + 1.0;
+        ql[a] = (unsigned int)(fl < 0.0 ? 0.0 : (fl > 255.0 ? 255.0 : fl));
+        qh[a] = (unsigned int)(fh < 0.0 ? 0.0 : (fh > 255.0 ? 255.0 : fh));
+    }
+    mq[m] = make_uint2(ql[0] | (ql[1] << 8) | (ql[2] << 16) | (qh[0] << 24), qh[1] | (qh[2] << 8));
+}
+
 // ---------------------------------------------------------------------------
 // DDA traversal
 // ---------------------------------------------------------------------------
@@ -532,11 +576,14 @@ LEM_DEV float replay_window(const Ray &r, double tstar, const GridDev &G,
 // live block after skipping, the entry cell on the two axes not crossed comes
 // from the entry point (clamped into the block; a cell off by one at an edge
 // or corner only shifts the walk within the eps inflation of the boxes).
-template <bool ANY, int SEM = 0>
+// LDSM: macro boxes come from the quantised table `smq` (LDS) instead of the
+// float boxes in global memory.
+template <bool ANY, int SEM = 0, bool LDSM = false>
 LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
                       const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
                       const float4 *__restrict__ box, unsigned int *n_tests = nullptr,
-                      unsigned int *n_cells = nullptr, unsigned int *n_macro = nullptr)
+                      unsigned int *n_cells = nullptr, unsigned int *n_macro = nullptr,
+                      const uint2 *smq = nullptr)
 {
     Walk w;
     if (!walk_begin(r, G, w))
@@ -566,8 +613,20 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
     // never move back, so the walk ends; the caps only guard the impossible.
     for (int guard = G.mx + G.my + G.mz + 4; guard > 0; --guard) {
         const int m = (jz * G.my + jy) * G.mx + jx;
-        const float4 mlo = mbox[2 * m], mhi = mbox[2 * m + 1];
-        // both halves of the box are loaded before either is tested
+        float4 mlo, mhi;
+        if (LDSM) {
+            const uint2 qv = smq[m];
+            const float ox = macro_origin(G.lox, G.cx, jx, ms), oy = macro_origin(G.loy, G.cy, jy, ms),
+                        oz = macro_origin(G.loz, G.cz, jz, ms);
+            mlo = make_float4(ox + (float)(qv.x & 255u) * G.qx, oy + (float)((qv.x >> 8) & 255u) * G.qy,
+                              oz + (float)((qv.x >> 16) & 255u) * G.qz, 0.0f);
+            mhi = make_float4(ox + (float)(qv.x >> 24) * G.qx, oy + (float)(qv.y & 255u) * G.qy,
+                              oz + (float)((qv.y >> 8) & 255u) * G.qz, 0.0f);
+        } else {
+            // both halves of the box are loaded before either is tested
+            mlo = mbox[2 * m];
+            mhi = mbox[2 * m + 1];
+        }
         const bool live = (mlo.x <= mhi.x) &
                           seg_box(r, idx, idy, idz, mlo, mhi, -margin, dist + margin, margin);
         if (n_macro)
@@ -793,14 +852,63 @@ __global__ __launch_bounds__(GB) void k_visibility_grid(
         dist_out[outer * NI + inner] = dist;
 }
 
+// Persistent form of k_visibility_grid: each workgroup copies the quantised
+// macro table (at most MACRO_LDS_MAX blocks) into LDS once, then walks its
+// share of the (outer, 256 inner) tasks, task = blockIdx.x + i * gridDim.x.
+// Dead-block tests read LDS instead of global memory, and no workgroup churn.
+template <bool ANY, bool SINK_MAJOR, int SEM>
+__global__ __launch_bounds__(GB) void k_visibility_grid_lds(
+    const float *__restrict__ src, int64_t S, const float *__restrict__ snk, int64_t K, GridDev G,
+    const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
+    const float4 *__restrict__ box, const uint2 *__restrict__ mq, int nmacro,
+    unsigned long long *__restrict__ bits, float *__restrict__ dist_out, int64_t nbl,
+    int64_t ntasks, int skip_self)
+{
+    __shared__ uint2 smq[MACRO_LDS_MAX];
+    for (int i = threadIdx.x; i < nmacro; i += GB)
+        smq[i] = mq[i];
+    __syncthreads();
+    const int64_t NI = SINK_MAJOR ? S : K;
+    const int64_t W = (NI + 63) / 64;
+    for (int64_t task = blockIdx.x; task < ntasks; task += gridDim.x) {
+        const int64_t outer = task / nbl;
+        const int64_t inner = (task % nbl) * GB + threadIdx.x;
+        const bool active = inner < NI;
+        const int64_t s = SINK_MAJOR ? (active ? inner : 0) : outer;
+        const int64_t k = SINK_MAJOR ? outer : (active ? inner : 0);
+        Ray r = charge(src[3 * s], src[3 * s + 1], src[3 * s + 2], snk[3 * k], snk[3 * k + 1],
+                       snk[3 * k + 2]);
+        float dist = r.len;
+        bool hit = false;
+        if (active && !(skip_self && s == k))
+            hit = dda_cast<ANY, SEM, true>(r, dist, G, cell_start, recs, box, nullptr, nullptr,
+                                           nullptr, smq);
+        unsigned long long bw = __ballot(active && hit);
+        if ((threadIdx.x & 63) == 0) {
+            const int64_t word = inner >> 6;
+            if (word < W)
+                bits[outer * W + word] = bw;
+        }
+        if (dist_out && active)
+            dist_out[outer * NI + inner] = dist;
+    }
+}
+
 // Instrumented traversal: counts the Moller-Trumbore tests and cell visits a
 // traversal performs (the algorithmic work of the roofline), no outputs.
 template <bool ANY>
 __global__ __launch_bounds__(GB) void k_grid_stats(
     const float *__restrict__ src, int64_t S, const float *__restrict__ snk, int64_t K, GridDev G,
     const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
-    const float4 *__restrict__ box, unsigned long long *__restrict__ stats, int64_t nbl)
+    const float4 *__restrict__ box, const uint2 *__restrict__ mq, int nmacro,
+    unsigned long long *__restrict__ stats, int64_t nbl)
 {
+    __shared__ uint2 smq[MACRO_LDS_MAX];
+    const bool lds = nmacro <= MACRO_LDS_MAX;
+    if (lds)
+        for (int i = threadIdx.x; i < nmacro; i += GB)
+            smq[i] = mq[i];
+    __syncthreads();
     const int64_t k = (int64_t)blockIdx.x / nbl;
     const int64_t s = ((int64_t)blockIdx.x % nbl) * GB + threadIdx.x;
     unsigned int nt = 0, nc = 0, nm = 0, nw = 0;
@@ -809,7 +917,8 @@ __global__ __launch_bounds__(GB) void k_grid_stats(
         Ray r = charge(src[3 * s], src[3 * s + 1], src[3 * s + 2], snk[3 * k], snk[3 * k + 1],
                        snk[3 * k + 2]);
         float dist = r.len;
-        hit = dda_cast<ANY>(r, dist, G, cell_start, recs, box, &nt, &nc, &nm);
+        hit = lds ? dda_cast<ANY, 0, true>(r, dist, G, cell_start, recs, box, &nt, &nc, &nm, smq)
+                  : dda_cast<ANY>(r, dist, G, cell_start, recs, box, &nt, &nc, &nm);
         nw = conservative_refs(r, dist, G, cell_start);
     }
     // wave sums, one atomic per wave
@@ -1035,10 +1144,12 @@ extern "C" int lem_grid_plan(const float *tris, int64_t T, float cells_per_tri,
     }
     g.ncells = (int64_t)n[0] * n[1] * n[2];
     g.eps = eps;
-    g.mshift = LEM_GRID_MSHIFT;
-    {
-        const int B = 1 << g.mshift;
-        g.nmacro = (int32_t)(((n[0] + B - 1) / B) * ((n[1] + B - 1) / B) * ((n[2] + B - 1) / B));
+    for (g.mshift = LEM_GRID_MSHIFT;; ++g.mshift) {
+        const int64_t B = (int64_t)1 << g.mshift;
+        const int64_t nm = ((n[0] + B - 1) / B) * ((n[1] + B - 1) / B) * ((n[2] + B - 1) / B);
+        g.nmacro = (int32_t)nm;
+        if (nm <= MACRO_LDS_MAX || g.mshift >= 6)
+            break;
     }
     GridDev G = to_dev(g);
     hipLaunchKernelGGL(k_zero_int, dim3((unsigned)ceil_div(g.ncells + 1, GB)), dim3(GB), 0, st,
@@ -1100,6 +1211,9 @@ extern "C" int lem_grid_fill(const float *tris, int64_t T, const lem_grid_info *
                        cell_start, tri_ids, G, nc, reinterpret_cast<float4 *>(cell_box));
     hipLaunchKernelGGL(k_macro_boxes, dim3((unsigned)ceil_div(info->nmacro, GB)), dim3(GB), 0, st,
                        G, reinterpret_cast<float4 *>(cell_box));
+    hipLaunchKernelGGL(k_macro_quant, dim3((unsigned)ceil_div(info->nmacro, GB)), dim3(GB), 0, st,
+                       G, reinterpret_cast<const float4 *>(cell_box),
+                       reinterpret_cast<uint2 *>(cell_box + 8 * (nc + info->nmacro)));
     return check_launch("lem_grid_fill");
 }
 
@@ -1130,6 +1244,43 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
     unsigned long long *hb = reinterpret_cast<unsigned long long *>(hit_bits);
     const float4 *rc4 = reinterpret_cast<const float4 *>(recs);
     const float4 *bx4 = reinterpret_cast<const float4 *>(cell_box);
+    const uint2 *mq = reinterpret_cast<const uint2 *>(cell_box + 8 * (info->ncells + info->nmacro));
+    const int64_t ntasks = NO * nbl;
+    if (info->nmacro <= MACRO_LDS_MAX) {
+        // persistent launch: as many workgroups as fit on the device at once
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void *>(k_visibility_grid_lds<false, true, 0>), GB,
+                0) != hipSuccess) {
+            set_error("lem_visibility_grid: device query failed");
+            return E_HIP;
+        }
+        const int64_t nwg = std::max<int64_t>(1, std::min<int64_t>(ntasks, (int64_t)cus * per_cu));
+#define LEM_VGL(A, M, SM)                                                                     \
+    hipLaunchKernelGGL((k_visibility_grid_lds<A, M, SM>), dim3((unsigned)nwg), dim3(GB), 0, st,    \
+                       src, S, snk, K, G, cell_start, rc4, bx4, mq, info->nmacro, hb, dist, nbl,   \
+                       ntasks, skip_self)
+        if (semantics == LEM_SEM_NATIVE) {
+            if (sink_major)
+                LEM_VGL(true, true, 1);
+            else
+                LEM_VGL(true, false, 1);
+        } else if (any) {
+            if (sink_major)
+                LEM_VGL(true, true, 0);
+            else
+                LEM_VGL(true, false, 0);
+        } else {
+            if (sink_major)
+                LEM_VGL(false, true, 0);
+            else
+                LEM_VGL(false, false, 0);
+        }
+#undef LEM_VGL
+        return check_launch("k_visibility_grid_lds");
+    }
 #define LEM_VG(A, M, SM)                                                                      \
     hipLaunchKernelGGL((k_visibility_grid<A, M, SM>), grid, dim3(GB), 0, st, src, S, snk, K, G,   \
                        cell_start, rc4, bx4, hb, dist, nbl, skip_self)
@@ -1168,12 +1319,14 @@ extern "C" int lem_visibility_grid_stats(const float *src, int64_t S, const floa
     unsigned long long *sp = reinterpret_cast<unsigned long long *>(stats);
     const float4 *rc4 = reinterpret_cast<const float4 *>(recs);
     const float4 *bx4 = reinterpret_cast<const float4 *>(cell_box);
+    const uint2 *mq = reinterpret_cast<const uint2 *>(cell_box + 8 * (info->ncells + info->nmacro));
+    const int nm = info->nmacro;
     if (flags & LEM_VIS_ANY_HIT)
         hipLaunchKernelGGL((k_grid_stats<true>), dim3((unsigned)(K * nbl)), dim3(GB), 0, st, src,
-                           S, snk, K, G, cell_start, rc4, bx4, sp, nbl);
+                           S, snk, K, G, cell_start, rc4, bx4, mq, nm, sp, nbl);
     else
         hipLaunchKernelGGL((k_grid_stats<false>), dim3((unsigned)(K * nbl)), dim3(GB), 0, st, src,
-                           S, snk, K, G, cell_start, rc4, bx4, sp, nbl);
+                           S, snk, K, G, cell_start, rc4, bx4, mq, nm, sp, nbl);
     return check_launch("k_grid_stats");
 }
 

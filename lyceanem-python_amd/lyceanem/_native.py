@@ -452,8 +452,9 @@ class Grid:
         self.cell_start = torch.empty(self.info.ncells + 1, dtype=torch.int32, device=dev)
         self.tri_ids = torch.empty(max(self.info.nrefs, 1), dtype=torch.int32, device=dev)
         self.recs = torch.empty((max(self.info.nrefs, 1), 12), dtype=torch.float32, device=dev)
-        self.cell_box = torch.empty((self.info.ncells + self.info.nmacro, 8), dtype=torch.float32,
-                                    device=dev)
+        # float boxes of cells and macro blocks, then the 8-byte quantised macro table
+        nbox = 8 * (self.info.ncells + self.info.nmacro) + 2 * self.info.nmacro
+        self.cell_box = torch.empty(nbox, dtype=torch.float32, device=dev)
         check(lib.lem_grid_fill(ptr(self.tris), T, ctypes.byref(self.info), ptr(self.cell_start),
                                 ptr(self.tri_ids), ptr(self.recs), ptr(self.cell_box),
                                 ptr(self.ws), wsb, self.cells_per_tri, stream_ptr()),
