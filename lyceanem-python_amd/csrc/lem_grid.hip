@@ -856,8 +856,10 @@ __global__ __launch_bounds__(GB) void k_visibility_grid(
 // macro table (at most MACRO_LDS_MAX blocks) into LDS once, then walks its
 // share of the (outer, 256 inner) tasks, task = blockIdx.x + i * gridDim.x.
 // Dead-block tests read LDS instead of global memory, and no workgroup churn.
+constexpr int LDS_WG_PER_CU = 5;  // 5 x 32 KB of the 160 KB LDS of a CU
+
 template <bool ANY, bool SINK_MAJOR, int SEM>
-__global__ __launch_bounds__(GB) void k_visibility_grid_lds(
+__global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
     const float *__restrict__ src, int64_t S, const float *__restrict__ snk, int64_t K, GridDev G,
     const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
     const float4 *__restrict__ box, const uint2 *__restrict__ mq, int nmacro,
@@ -1247,16 +1249,19 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
     const uint2 *mq = reinterpret_cast<const uint2 *>(cell_box + 8 * (info->ncells + info->nmacro));
     const int64_t ntasks = NO * nbl;
     if (info->nmacro <= MACRO_LDS_MAX) {
-        // persistent launch: as many workgroups as fit on the device at once
-        int dev = 0, cus = 0, per_cu = 0;
+        // persistent launch: as many workgroups as are resident at once.  The
+        // registers are bounded for LDS_WG_PER_CU waves per SIMD (launch
+        // bounds), so the LDS per CU sets the count.
+        int dev = 0, cus = 0, lds = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, reinterpret_cast<const void *>(k_visibility_grid_lds<false, true, 0>), GB,
-                0) != hipSuccess) {
+            hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+                hipSuccess) {
             set_error("lem_visibility_grid: device query failed");
             return E_HIP;
         }
+        const int per_cu = std::max(1, std::min(LDS_WG_PER_CU,
+                                                lds / (int)(MACRO_LDS_MAX * sizeof(uint2))));
         const int64_t nwg = std::max<int64_t>(1, std::min<int64_t>(ntasks, (int64_t)cus * per_cu));
 #define LEM_VGL(A, M, SM)                                                                     \
     hipLaunchKernelGGL((k_visibility_grid_lds<A, M, SM>), dim3((unsigned)nwg), dim3(GB), 0, st,    \
