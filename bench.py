@@ -178,8 +178,8 @@ class C2:
         nat.check(lib.lem_visibility_grid(p(self.src), self.S, p(self.snk), self.K,
                                           nat.ctypes.byref(g.info), p(g.cell_start), p(g.recs),
                                           p(g.cell_box), nat.SEM_NUMBA, nat.VIS_SINK_MAJOR,
-                                          p(self.bits),
-                                          p(self.dist), nat.stream_ptr()), "visibility_grid")
+                                          p(self.bits), p(self.dist), p(g.vis_ws),
+                                          g.vis_ws.numel(), nat.stream_ptr()), "visibility_grid")
         if ev is not None:
             ev[1].record()
             ev[2].record()

@@ -161,11 +161,14 @@ int lem_grid_fill(const float *tris, int64_t T, const lem_grid_info *info, int32
  * 2e-7 + one float32 ulp of each other).  Replaces the tile traversal raycast_tiles / DDA
  * (raycasting_accelerated.cuh:106-268, 458-482) for source -> sink rays.
  * Flags: LEM_VIS_ANY_HIT, LEM_VIS_SINK_MAJOR, LEM_VIS_SKIP_SELF.
- * LEM_SEM_NATIVE is any-hit (no distance). */
+ * LEM_SEM_NATIVE is any-hit (no distance).  The workspace
+ * (lem_visibility_grid_workspace() bytes) holds the task counter of the
+ * persistent traversal: calls that overlap on one workspace race. */
+int64_t lem_visibility_grid_workspace(void);
 int lem_visibility_grid(const float *src, int64_t S, const float *snk, int64_t K,
                         const lem_grid_info *info, const int32_t *cell_start, const float *recs,
                         const float *cell_box, int semantics, int flags, uint64_t *hit_bits,
-                        float *dist, void *stream);
+                        float *dist, void *workspace, int64_t workspace_bytes, void *stream);
 
 /* Line-of-sight field fused with the grid traversal: lem_field_los (source
  * sum only) where each ray is cast any-hit through the grid instead of read

@@ -852,29 +852,39 @@ __global__ __launch_bounds__(GB) void k_visibility_grid(
         dist_out[outer * NI + inner] = dist;
 }
 
-// Persistent form of k_visibility_grid: each workgroup copies the quantised
-// macro table (at most MACRO_LDS_MAX blocks) into LDS once, then walks its
-// share of the (outer, 256 inner) tasks, task = blockIdx.x + i * gridDim.x.
-// Dead-block tests read LDS instead of global memory, and no workgroup churn.
 constexpr int LDS_WG_PER_CU = 5;  // 5 x 32 KB of the 160 KB LDS of a CU
 
+// Persistent form of k_visibility_grid: each workgroup copies the quantised
+// macro table (at most MACRO_LDS_MAX blocks) into LDS once; then each
+// wavefront repeatedly claims the next 64-ray word of output (one atomic on
+// *next_task), so waves never wait for one another and the load balances
+// whatever number of workgroups is resident.  Dead-block tests read LDS.
 template <bool ANY, bool SINK_MAJOR, int SEM>
 __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
     const float *__restrict__ src, int64_t S, const float *__restrict__ snk, int64_t K, GridDev G,
     const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
     const float4 *__restrict__ box, const uint2 *__restrict__ mq, int nmacro,
-    unsigned long long *__restrict__ bits, float *__restrict__ dist_out, int64_t nbl,
-    int64_t ntasks, int skip_self)
+    unsigned long long *__restrict__ bits, float *__restrict__ dist_out,
+    unsigned long long *__restrict__ next_task, int skip_self)
 {
     __shared__ uint2 smq[MACRO_LDS_MAX];
     for (int i = threadIdx.x; i < nmacro; i += GB)
         smq[i] = mq[i];
     __syncthreads();
-    const int64_t NI = SINK_MAJOR ? S : K;
+    const int64_t NI = SINK_MAJOR ? S : K, NO = SINK_MAJOR ? K : S;
     const int64_t W = (NI + 63) / 64;
-    for (int64_t task = blockIdx.x; task < ntasks; task += gridDim.x) {
-        const int64_t outer = task / nbl;
-        const int64_t inner = (task % nbl) * GB + threadIdx.x;
+    const unsigned long long ntasks = (unsigned long long)(NO * W);
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        unsigned long long task = 0;
+        if (lane == 0)
+            task = atomicAdd(next_task, 1ull);
+        task = __shfl(task, 0);
+        if (task >= ntasks)
+            break;
+        const int64_t outer = (int64_t)(task / W);
+        const int64_t word = (int64_t)(task % W);
+        const int64_t inner = word * 64 + lane;
         const bool active = inner < NI;
         const int64_t s = SINK_MAJOR ? (active ? inner : 0) : outer;
         const int64_t k = SINK_MAJOR ? outer : (active ? inner : 0);
@@ -885,15 +895,18 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
         if (active && !(skip_self && s == k))
             hit = dda_cast<ANY, SEM, true>(r, dist, G, cell_start, recs, box, nullptr, nullptr,
                                            nullptr, smq);
-        unsigned long long bw = __ballot(active && hit);
-        if ((threadIdx.x & 63) == 0) {
-            const int64_t word = inner >> 6;
-            if (word < W)
-                bits[outer * W + word] = bw;
-        }
+        const unsigned long long bw = __ballot(active && hit);
+        if (lane == 0)
+            bits[outer * W + word] = bw;
         if (dist_out && active)
             dist_out[outer * NI + inner] = dist;
     }
+}
+
+__global__ void k_zero_u64(unsigned long long *__restrict__ p)
+{
+    if (threadIdx.x == 0)
+        p[0] = 0ull;
 }
 
 // Instrumented traversal: counts the Moller-Trumbore tests and cell visits a
@@ -1219,10 +1232,13 @@ extern "C" int lem_grid_fill(const float *tris, int64_t T, const lem_grid_info *
     return check_launch("lem_grid_fill");
 }
 
+extern "C" int64_t lem_visibility_grid_workspace(void) { return 256; }
+
 extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk, int64_t K,
                                    const lem_grid_info *info, const int32_t *cell_start,
                                    const float *recs, const float *cell_box, int semantics,
-                                   int flags, uint64_t *hit_bits, float *dist, void *stream)
+                                   int flags, uint64_t *hit_bits, float *dist, void *workspace,
+                                   int64_t workspace_bytes, void *stream)
 {
     clear_error();
     LEM_REQUIRE(S >= 0 && K >= 0, "lem_visibility_grid: negative size");
@@ -1247,8 +1263,11 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
     const float4 *rc4 = reinterpret_cast<const float4 *>(recs);
     const float4 *bx4 = reinterpret_cast<const float4 *>(cell_box);
     const uint2 *mq = reinterpret_cast<const uint2 *>(cell_box + 8 * (info->ncells + info->nmacro));
-    const int64_t ntasks = NO * nbl;
     if (info->nmacro <= MACRO_LDS_MAX) {
+        if (!workspace || workspace_bytes < lem_visibility_grid_workspace()) {
+            set_error("lem_visibility_grid: workspace smaller than lem_visibility_grid_workspace()");
+            return E_WORKSPACE;
+        }
         // persistent launch: as many workgroups as are resident at once.  The
         // registers are bounded for LDS_WG_PER_CU waves per SIMD (launch
         // bounds), so the LDS per CU sets the count.
@@ -1262,11 +1281,15 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
         }
         const int per_cu = std::max(1, std::min(LDS_WG_PER_CU,
                                                 lds / (int)(MACRO_LDS_MAX * sizeof(uint2))));
-        const int64_t nwg = std::max<int64_t>(1, std::min<int64_t>(ntasks, (int64_t)cus * per_cu));
+        const int64_t words = NO * ceil_div(NI, 64);
+        const int64_t nwg =
+            std::max<int64_t>(1, std::min<int64_t>(ceil_div(words, 4), (int64_t)cus * per_cu));
+        unsigned long long *counter = reinterpret_cast<unsigned long long *>(workspace);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, counter);
 #define LEM_VGL(A, M, SM)                                                                     \
     hipLaunchKernelGGL((k_visibility_grid_lds<A, M, SM>), dim3((unsigned)nwg), dim3(GB), 0, st,    \
-                       src, S, snk, K, G, cell_start, rc4, bx4, mq, info->nmacro, hb, dist, nbl,   \
-                       ntasks, skip_self)
+                       src, S, snk, K, G, cell_start, rc4, bx4, mq, info->nmacro, hb, dist,        \
+                       counter, skip_self)
         if (semantics == LEM_SEM_NATIVE) {
             if (sink_major)
                 LEM_VGL(true, true, 1);

@@ -43,7 +43,9 @@ SIGNATURES = {
     "lem_grid_workspace": ([_I64, ctypes.c_float], _I64),
     "lem_grid_plan": ([_P, _I64, ctypes.c_float, _P, _P, _I64, _P], _INT),
     "lem_grid_fill": ([_P, _I64, _P, _P, _P, _P, _P, _P, _I64, ctypes.c_float, _P], _INT),
-    "lem_visibility_grid": ([_P, _I64, _P, _I64, _P, _P, _P, _P, _INT, _INT, _P, _P, _P], _INT),
+    "lem_visibility_grid_workspace": ([], _I64),
+    "lem_visibility_grid": ([_P, _I64, _P, _I64, _P, _P, _P, _P, _INT, _INT, _P, _P, _P, _I64,
+                             _P], _INT),
     "lem_visibility_grid_stats": ([_P, _I64, _P, _I64, _P, _P, _P, _P, _INT, _P, _P], _INT),
     "lem_beam_workspace": ([_I64, _I64], _I64),
     "lem_beam_power": ([_P, _I64, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _I64, _P], _INT),
@@ -455,6 +457,8 @@ class Grid:
         # float boxes of cells and macro blocks, then the 8-byte quantised macro table
         nbox = 8 * (self.info.ncells + self.info.nmacro) + 2 * self.info.nmacro
         self.cell_box = torch.empty(nbox, dtype=torch.float32, device=dev)
+        self.vis_ws = torch.empty(int(lib.lem_visibility_grid_workspace()), dtype=torch.uint8,
+                                  device=dev)
         check(lib.lem_grid_fill(ptr(self.tris), T, ctypes.byref(self.info), ptr(self.cell_start),
                                 ptr(self.tri_ids), ptr(self.recs), ptr(self.cell_box),
                                 ptr(self.ws), wsb, self.cells_per_tri, stream_ptr()),
@@ -477,7 +481,8 @@ class Grid:
             (VIS_SKIP_SELF if skip_self else 0)
         check(lib.lem_visibility_grid(ptr(src), S, ptr(snk), K, ctypes.byref(self.info),
                                       ptr(self.cell_start), ptr(self.recs), ptr(self.cell_box),
-                                      semantics, flags, ptr(bits), ptr(dist), stream_ptr()),
+                                      semantics, flags, ptr(bits), ptr(dist), ptr(self.vis_ws),
+                                      self.vis_ws.numel(), stream_ptr()),
               "lem_visibility_grid")
         return bits, dist
 
