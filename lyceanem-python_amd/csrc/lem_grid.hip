@@ -422,14 +422,15 @@ LEM_DEV void slab(bool par, float o, float lo, float hi, float id, float &tn, fl
 }
 
 // True when the segment t in [t_lo, t_hi] of the ray meets the box (slab test
-// with a margin of m in t).
+// with a margin of m in t).  PAR = false: no direction component is zero.
+template <bool PAR = true>
 LEM_DEV bool seg_box(const Ray &r, float idx, float idy, float idz, float4 lo, float4 hi,
                      float t_lo, float t_hi, float m)
 {
     float tn = t_lo, tf = t_hi;
-    slab(r.dx == 0.0f, r.ox, lo.x, hi.x, idx, tn, tf);
-    slab(r.dy == 0.0f, r.oy, lo.y, hi.y, idy, tn, tf);
-    slab(r.dz == 0.0f, r.oz, lo.z, hi.z, idz, tn, tf);
+    slab(PAR && r.dx == 0.0f, r.ox, lo.x, hi.x, idx, tn, tf);
+    slab(PAR && r.dy == 0.0f, r.oy, lo.y, hi.y, idy, tn, tf);
+    slab(PAR && r.dz == 0.0f, r.oz, lo.z, hi.z, idz, tn, tf);
     return tn <= tf + m;
 }
 
@@ -578,7 +579,7 @@ LEM_DEV float replay_window(const Ray &r, double tstar, const GridDev &G,
 // or corner only shifts the walk within the eps inflation of the boxes).
 // LDSM: macro boxes come from the quantised table `smq` (LDS) instead of the
 // float boxes in global memory.
-template <bool ANY, int SEM = 0, bool LDSM = false>
+template <bool ANY, int SEM = 0, bool LDSM = false, bool PAR = true>
 LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
                       const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
                       const float4 *__restrict__ box, unsigned int *n_tests = nullptr,
@@ -595,6 +596,16 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
     const float margin = 2.0f * G.eps;
     const int ms = G.mshift, B = 1 << ms;
     const float4 *__restrict__ mbox = box + 2 * G.ncells;  // macro-block boxes
+    // Quantised boxes, rays with no zero direction component: the slab bound
+    // of face q of block j on an axis is t = A + C j + S q, with
+    // A = (lo - o) / d, C = cell * 2^mshift / d, S = step / d (the same
+    // quantities as the decoded box, reassociated; the 2 eps margin in t
+    // covers the rounding difference).
+    const float Axx = (G.lox - r.ox) * idx, Ayy = (G.loy - r.oy) * idy,
+                Azz = (G.loz - r.oz) * idz;
+    const float Cxx = G.cx * (float)B * idx, Cyy = G.cy * (float)B * idy,
+                Czz = G.cz * (float)B * idz;
+    const float Sxx = G.qx * idx, Syy = G.qy * idy, Szz = G.qz * idz;
     // last cell of block j along an axis in the direction of travel
     auto last = [&](int j, int s, int n) { return s > 0 ? min((j << ms) + B - 1, n - 1) : (j << ms); };
     int jx = ix >> ms, jy = iy >> ms, jz = iz >> ms;
@@ -613,22 +624,40 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
     // never move back, so the walk ends; the caps only guard the impossible.
     for (int guard = G.mx + G.my + G.mz + 4; guard > 0; --guard) {
         const int m = (jz * G.my + jy) * G.mx + jx;
-        float4 mlo, mhi;
-        if (LDSM) {
+        bool live;
+        if (LDSM && !PAR) {
             const uint2 qv = smq[m];
-            const float ox = macro_origin(G.lox, G.cx, jx, ms), oy = macro_origin(G.loy, G.cy, jy, ms),
-                        oz = macro_origin(G.loz, G.cz, jz, ms);
-            mlo = make_float4(ox + (float)(qv.x & 255u) * G.qx, oy + (float)((qv.x >> 8) & 255u) * G.qy,
-                              oz + (float)((qv.x >> 16) & 255u) * G.qz, 0.0f);
-            mhi = make_float4(ox + (float)(qv.x >> 24) * G.qx, oy + (float)(qv.y & 255u) * G.qy,
-                              oz + (float)((qv.y >> 8) & 255u) * G.qz, 0.0f);
+            const float bx = Axx + Cxx * (float)jx, by = Ayy + Cyy * (float)jy,
+                        bz = Azz + Czz * (float)jz;
+            const float x0 = bx + Sxx * (float)(qv.x & 255u), x1 = bx + Sxx * (float)(qv.x >> 24);
+            const float y0 = by + Syy * (float)((qv.x >> 8) & 255u),
+                        y1 = by + Syy * (float)(qv.y & 255u);
+            const float z0 = bz + Szz * (float)((qv.x >> 16) & 255u),
+                        z1 = bz + Szz * (float)((qv.y >> 8) & 255u);
+            const float tn = fmaxf(fmaxf(-margin, fminf(x0, x1)), fmaxf(fminf(y0, y1), fminf(z0, z1)));
+            const float tf = fminf(fminf(dist + margin, fmaxf(x0, x1)),
+                                   fminf(fmaxf(y0, y1), fmaxf(z0, z1)));
+            live = ((qv.x & 255u) <= (qv.x >> 24)) & (tn <= tf + margin);
         } else {
-            // both halves of the box are loaded before either is tested
-            mlo = mbox[2 * m];
-            mhi = mbox[2 * m + 1];
+            float4 mlo, mhi;
+            if (LDSM) {
+                const uint2 qv = smq[m];
+                const float ox = macro_origin(G.lox, G.cx, jx, ms),
+                            oy = macro_origin(G.loy, G.cy, jy, ms),
+                            oz = macro_origin(G.loz, G.cz, jz, ms);
+                mlo = make_float4(ox + (float)(qv.x & 255u) * G.qx,
+                                  oy + (float)((qv.x >> 8) & 255u) * G.qy,
+                                  oz + (float)((qv.x >> 16) & 255u) * G.qz, 0.0f);
+                mhi = make_float4(ox + (float)(qv.x >> 24) * G.qx, oy + (float)(qv.y & 255u) * G.qy,
+                                  oz + (float)((qv.y >> 8) & 255u) * G.qz, 0.0f);
+            } else {
+                // both halves of the box are loaded before either is tested
+                mlo = mbox[2 * m];
+                mhi = mbox[2 * m + 1];
+            }
+            live = (mlo.x <= mhi.x) &
+                   seg_box<PAR>(r, idx, idy, idz, mlo, mhi, -margin, dist + margin, margin);
         }
-        const bool live = (mlo.x <= mhi.x) &
-                          seg_box(r, idx, idy, idz, mlo, mhi, -margin, dist + margin, margin);
         if (n_macro)
             *n_macro += 1;
         if (!live) {
@@ -685,7 +714,7 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
                 e = cell_start[c + 1];
                 if (n_cells)
                     *n_cells += 1;
-                if (e > b && !seg_box(r, idx, idy, idz, box[2 * c], box[2 * c + 1], -margin,
+                if (e > b && !seg_box<PAR>(r, idx, idy, idz, box[2 * c], box[2 * c + 1], -margin,
                                       dist + margin, margin))
                     e = b;
             }
@@ -892,9 +921,15 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
                        snk[3 * k + 2]);
         float dist = r.len;
         bool hit = false;
-        if (active && !(skip_self && s == k))
-            hit = dda_cast<ANY, SEM, true>(r, dist, G, cell_start, recs, box, nullptr, nullptr,
-                                           nullptr, smq);
+        if (active && !(skip_self && s == k)) {
+            // rays along a grid plane take the general slab test
+            if (r.dx == 0.0f || r.dy == 0.0f || r.dz == 0.0f)
+                hit = dda_cast<ANY, SEM, true, true>(r, dist, G, cell_start, recs, box, nullptr,
+                                                     nullptr, nullptr, smq);
+            else
+                hit = dda_cast<ANY, SEM, true, false>(r, dist, G, cell_start, recs, box, nullptr,
+                                                      nullptr, nullptr, smq);
+        }
         const unsigned long long bw = __ballot(active && hit);
         if (lane == 0)
             bits[outer * W + word] = bw;
@@ -932,8 +967,12 @@ __global__ __launch_bounds__(GB) void k_grid_stats(
         Ray r = charge(src[3 * s], src[3 * s + 1], src[3 * s + 2], snk[3 * k], snk[3 * k + 1],
                        snk[3 * k + 2]);
         float dist = r.len;
-        hit = lds ? dda_cast<ANY, 0, true>(r, dist, G, cell_start, recs, box, &nt, &nc, &nm, smq)
-                  : dda_cast<ANY>(r, dist, G, cell_start, recs, box, &nt, &nc, &nm);
+        const bool par = r.dx == 0.0f || r.dy == 0.0f || r.dz == 0.0f;
+        hit = !lds ? dda_cast<ANY>(r, dist, G, cell_start, recs, box, &nt, &nc, &nm)
+              : par ? dda_cast<ANY, 0, true, true>(r, dist, G, cell_start, recs, box, &nt, &nc,
+                                                   &nm, smq)
+                    : dda_cast<ANY, 0, true, false>(r, dist, G, cell_start, recs, box, &nt, &nc,
+                                                    &nm, smq);
         nw = conservative_refs(r, dist, G, cell_start);
     }
     // wave sums, one atomic per wave
