@@ -710,12 +710,15 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
             int32_t b, e;
             {
                 const int c = (iz * G.ny + iy) * G.nx + ix;
+                // the range and the box depend on the cell alone: all four
+                // loads go out together
                 b = cell_start[c];
                 e = cell_start[c + 1];
+                const float4 clo = box[2 * c], chi = box[2 * c + 1];
                 if (n_cells)
                     *n_cells += 1;
-                if (e > b && !seg_box<PAR>(r, idx, idy, idz, box[2 * c], box[2 * c + 1], -margin,
-                                      dist + margin, margin))
+                if (e > b && !seg_box<PAR>(r, idx, idy, idz, clo, chi, -margin, dist + margin,
+                                           margin))
                     e = b;
             }
             if (n_tests)
