@@ -35,6 +35,19 @@ for _p in (ROOT, PKG):
 import numpy as np  # noqa: E402
 
 MT_FLOPS = 46          # FP32 flops of one Moller-Trumbore test (+1 divide)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_visibility_grid_lds_c2_terrain_1M_traffic.md")
+
+
+def _traffic_per_ray():
+    """HBM bytes per ray of the grid traversal, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated) kept with the profiles, or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_ray:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 PAIR_FLOPS = 92        # FP32 flops of one Green's-function pair
 PEAK_FP32 = 157.3e12   # MI355X FP32 vector peak (packed)
 PEAK_HBM = 8.0e12
@@ -195,11 +208,16 @@ class C2:
         work_all = conservative * scale
         achieved = work_all * MT_FLOPS / (vis * 1e-3)
         bytes_out = self.bits.numel() * 8 + self.dist.numel() * 4
+        per_ray = _traffic_per_ray()
+        traffic = None if per_ray is None else per_ray * self.rays()
         return {
-            "kernel": "k_visibility_grid<closest-hit, sink-major>",
+            "kernel": "k_visibility_grid_lds<closest-hit, sink-major>",
             "kernel_ms": vis,
             "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None},
+                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic,
+                         "traffic_source": None if traffic is None else
+                         "profiles/" + os.path.basename(TRAFFIC_PROFILE) + " (bytes per ray x "
+                         "rays of this launch)"},
             "extra": {
                 "grid_build_s": self.build_s,
                 "grid_cells": [int(x) for x in self.grid.info.n],
