@@ -290,10 +290,14 @@ class C3:
         nat, t, g, b = self.nat, self.t, self.grid, self.sc["beta"]
         if ev is not None:
             ev[0].record()
-        H = nat.field_los_grid(g, t["src"], t["w"], t["scat"], t["scat_n"], snk_w=t["scat_w"],
-                               beta=b, sink_tangent=True)
+        # leg 1 (array -> scatterers): sink-major any-hit mask, 64 neighbouring
+        # elements to one scatterer per wave, then the masked field
+        bits, _ = g.visibility(t["src"], t["scat"], any_hit=True, sink_major=True)
         if ev is not None:
             ev[1].record()
+        H = nat.field_los(t["src"], t["w"], t["scat"], t["scat_n"], hit_bits=bits,
+                          snk_w=t["scat_w"], beta=b, sink_tangent=True, bits_sink_major=True)
+        del bits
         E = nat.field_los_grid(g, t["scat"], H, t["snk"], t["snk_n"], beta=b)
         E += nat.field_los_grid(g, t["src"], t["w"], t["snk"], t["snk_n"], beta=b,
                                 sink_tangent=True)
@@ -301,21 +305,32 @@ class C3:
             ev[2].record()
         self.E = E
 
+    stats_stride = 100
+
     def record(self, times_ms):
-        leg1 = float(np.mean([a for a, _ in times_ms]))
+        vis = float(np.mean([a for a, _ in times_ms]))
         rest = float(np.mean([b for _, b in times_ms]))
         pairs1 = self.S * self.P
+        sub = self.t["scat"][:: self.stats_stride].contiguous()
+        tests, cells, blocked, macros, conservative = self.nat.grid_stats(
+            self.grid, self.t["src"], sub, any_hit=True)
+        n_sub = self.S * sub.shape[0]
+        achieved = conservative * (self.P / sub.shape[0]) * MT_FLOPS / (vis * 1e-3)
         return {
-            "kernel": "k_field_los_grid_partial (leg 1: sources -> scatterers)",
-            "kernel_ms": leg1,
-            "roofline": {"bound": "valu", "achieved": pairs1 * PAIR_FLOPS / (leg1 * 1e-3) / 1e12,
-                         "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                         "frac": pairs1 * PAIR_FLOPS / (leg1 * 1e-3) / PEAK_FP32,
-                         "traffic": None, "note": "pair flops only; the DDA cast of every "
-                                                  "pair runs inside the same kernel"},
-            "extra": {"gpairs_per_s": self.rays() / ((leg1 + rest) * 1e-3) / 1e9,
-                      "leg1_gpairs_per_s": pairs1 / (leg1 * 1e-3) / 1e9,
-                      "legs2_los_ms": rest, "grid_cells": [int(x) for x in self.grid.info.n]},
+            "kernel": "k_visibility_grid_lds<any-hit, sink-major> (leg 1 mask: array -> "
+                      "scatterers)",
+            "kernel_ms": vis,
+            "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None},
+            "extra": {"gpairs_per_s": self.rays() / ((vis + rest) * 1e-3) / 1e9,
+                      "leg1_mask_grays_per_s": pairs1 / (vis * 1e-3) / 1e9,
+                      "leg1_field_legs2_los_ms": rest,
+                      "grid_cells": [int(x) for x in self.grid.info.n],
+                      "conservative_mt_tests_per_ray": conservative / n_sub,
+                      "mt_tests_per_ray": tests / n_sub, "cells_per_ray": cells / n_sub,
+                      "blocked_fraction": blocked / n_sub,
+                      "work_counted_on": f"every {self.stats_stride}th scatterer "
+                                         f"({sub.shape[0]}), scaled"},
         }
 
     def cpu_baseline(self, budget_s):
@@ -521,6 +536,9 @@ class C5:
 # --------------------------------------------------------------------------
 
 TD_PATH_FLOPS = 75  # float64 work of one line-of-sight path in k_time_paths
+# placement bytes per path: arrival read by the wake and bin passes (2 x 8 B),
+# the start sample written and read (2 x 8 B), the amplitude read (24 B)
+TD_PLACE_BYTES = 56
 
 
 class TD:
@@ -576,16 +594,23 @@ class TD:
         tp = float(np.mean([a for a, _ in times_ms]))
         rest = float(np.mean([b for _, b in times_ms]))
         flops = self.V * TD_PATH_FLOPS
-        return {
-            "kernel": "k_time_paths",
-            "kernel_ms": tp,
-            "roofline": {"bound": "valu-fp64", "achieved": flops / (tp * 1e-3) / 1e12,
-                         "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                         "frac": flops / (tp * 1e-3) / PEAK_FP64, "traffic": None},
-            "extra": {"paths": self.V, "samples": self.N, "excitation_samples":
-                      int(self.exc.shape[0]), "wake_bins_trace_ms": rest,
-                      "time_paths_gpaths_per_s": self.V / (tp * 1e-3) / 1e9},
-        }
+        place = self.V * TD_PLACE_BYTES + self.N * 24 + self.exc.shape[0] * 8
+        extra = {"paths": self.V, "samples": self.N, "excitation_samples":
+                 int(self.exc.shape[0]), "time_paths_ms": tp, "placement_ms": rest,
+                 "time_paths_fp64_frac": flops / (tp * 1e-3) / PEAK_FP64,
+                 "placement_hbm_frac": place / (rest * 1e-3) / PEAK_HBM}
+        if tp >= rest:
+            return {"kernel": "k_time_paths", "kernel_ms": tp, "extra": extra,
+                    "roofline": {"bound": "valu-fp64", "achieved": flops / (tp * 1e-3) / 1e12,
+                                 "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                                 "frac": flops / (tp * 1e-3) / PEAK_FP64, "traffic": None}}
+        return {"kernel": "placement: k_time_wake + k_time_bins + k_trace_keys + radix sort + "
+                          "k_trace_starts/hist/conv", "kernel_ms": rest, "extra": extra,
+                "roofline": {"bound": "hbm", "achieved": place / (rest * 1e-3) / 1e9,
+                             "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                             "frac": place / (rest * 1e-3) / PEAK_HBM, "traffic": None,
+                             "note": f"algorithmic bytes: {TD_PLACE_BYTES} B per path + the "
+                                     f"trace and excitation"}}
 
     def cpu_baseline(self, budget_s):
         from oracle import oracle as orc
@@ -607,7 +632,7 @@ class TD:
                           f"{dt:.1f} s"}
 
     def config(self):
-        return {"workload": "td: 64x64 array at 3 GHz -> 256 sinks 50-500 m, 1,048,576 "
+        return {"workload": "td: 64x64 array at 3 GHz -> 4096 sinks 50-500 m, 16,777,216 "
                             "line-of-sight paths, 256-sample pulse at 40 GS/s into a "
                             "65,536-sample trace", "paths": self.V, "samples": self.N}
 

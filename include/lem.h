@@ -61,6 +61,10 @@ extern "C" {
                                     of a 3-column index (empropagation.py:1150-1161) */
 #define LEM_FIELD_ELEMENTS 2     /* write the S x K x 3 network instead of the
                                     source sum (frequency_domain.py:703-716) */
+#define LEM_FIELD_BITS_SINK_MAJOR 4 /* hit_bits is K x ceil(S/64) words (bit s%64 of
+                                       word s/64 of row k), the layout
+                                       lem_visibility_grid writes with
+                                       LEM_VIS_SINK_MAJOR */
 
 int lem_abi_version(void);
 const char *lem_last_error(void);
@@ -96,7 +100,8 @@ int lem_raycast_pairs(const float *points, int64_t n_points, const int32_t *pair
  * `np.dot(ones(S), scatter_map[:, :, c])` (frequency_domain.py:647-649).
  * src: S x 3, src_w: S x 3 complex64; snk, snk_n: K x 3; snk_w: K x 3 complex64
  * or NULL (weight 1, frequency_domain.py:398-400); hit_bits as produced by
- * lem_visibility_dense, or NULL (all visible).
+ * lem_visibility_dense (S x ceil(K/64)), or K x ceil(S/64) with
+ * LEM_FIELD_BITS_SINK_MAJOR, or NULL (all visible).
  * out: K x 3 complex64 (source sum), or S x K x 3 complex64 with
  * LEM_FIELD_ELEMENTS.  workspace: lem_field_los_workspace() bytes. */
 int64_t lem_field_los_workspace(int64_t S, int64_t K, int flags);

@@ -73,6 +73,14 @@ LEM_DEV bool blocked(const unsigned long long *__restrict__ hit_bits, int64_t W6
     return (w >> (k & 63)) & 1ull;
 }
 
+// Mask bit of ray s -> k in either layout: source-major (S x W64 words over
+// sinks) or sink-major (K x W64 words over sources).
+LEM_DEV bool blocked_any(const unsigned long long *__restrict__ hit_bits, int64_t W64,
+                         bool sink_major, int64_t s, int64_t k)
+{
+    return sink_major ? blocked(hit_bits, W64, k, s) : blocked(hit_bits, W64, s, k);
+}
+
 LEM_DEV void stage_sources(SrcLds *s_src, const float *__restrict__ src,
                            const float *__restrict__ src_w, int64_t base, int64_t s_end)
 {

@@ -26,8 +26,8 @@ namespace lem {
 __global__ __launch_bounds__(FB) void k_field_los_partial(
     const float *__restrict__ src, const float *__restrict__ src_w, int64_t S,
     const float *__restrict__ snk, const float *__restrict__ snk_n, int64_t K,
-    const unsigned long long *__restrict__ hit_bits, int64_t W64, float alpha, double beta,
-    int64_t CH, int64_t nbk, float *__restrict__ ws_sum, float *__restrict__ ws_cmp)
+    const unsigned long long *__restrict__ hit_bits, int64_t W64, int sink_major, float alpha,
+    double beta, int64_t CH, int64_t nbk, float *__restrict__ ws_sum, float *__restrict__ ws_cmp)
 {
     __shared__ SrcLds s_src[FS_TILE];
     const int64_t c = (int64_t)blockIdx.x / nbk;
@@ -40,6 +40,10 @@ __global__ __launch_bounds__(FB) void k_field_los_partial(
     const int64_t s1 = (s0 + CH) < S ? (s0 + CH) : S;
     KahanC3 acc;
     acc.zero();
+    // sink-major mask: one word covers 64 consecutive sources of this lane's
+    // row; every lane walks the same sources, so the reload is wave-uniform
+    unsigned long long word = 0;
+    int64_t word_at = -1;
     for (int64_t base = s0; base < s1; base += FS_TILE) {
         __syncthreads();
         stage_sources(s_src, src, src_w, base, s1);
@@ -47,7 +51,18 @@ __global__ __launch_bounds__(FB) void k_field_los_partial(
         int n = (int)((s1 - base) < FS_TILE ? (s1 - base) : FS_TILE);
         if (active) {
             for (int j = 0; j < n; ++j) {
-                if (blocked(hit_bits, W64, base + j, k))
+                const int64_t s = base + j;
+                bool bl;
+                if (sink_major) {
+                    if ((s >> 6) != word_at) {
+                        word_at = s >> 6;
+                        word = hit_bits ? hit_bits[k * W64 + word_at] : 0ull;
+                    }
+                    bl = (word >> (s & 63)) & 1ull;
+                } else {
+                    bl = blocked(hit_bits, W64, s, k);
+                }
+                if (bl)
                     continue;
                 const SrcLds &L = s_src[j];
                 C3 w;
@@ -102,15 +117,15 @@ __global__ __launch_bounds__(FB) void k_field_los_finish(
 __global__ __launch_bounds__(FB) void k_field_los_elements(
     const float *__restrict__ src, const float *__restrict__ src_w, const float *__restrict__ snk,
     const float *__restrict__ snk_n, const float *__restrict__ snk_w, int64_t K,
-    const unsigned long long *__restrict__ hit_bits, int64_t W64, float alpha, double beta,
-    int sink_tangent, int64_t nbk, float *__restrict__ out)
+    const unsigned long long *__restrict__ hit_bits, int64_t W64, int sink_major, float alpha,
+    double beta, int sink_tangent, int64_t nbk, float *__restrict__ out)
 {
     const int64_t s = (int64_t)blockIdx.x / nbk;
     const int64_t k = ((int64_t)blockIdx.x % nbk) * FB + threadIdx.x;
     if (k >= K)
         return;
     C3 e;
-    if (blocked(hit_bits, W64, s, k)) {
+    if (blocked_any(hit_bits, W64, sink_major != 0, s, k)) {
         e.xr = e.xi = e.yr = e.yi = e.zr = e.zi = 0.0f;
     } else {
         const float nkx = snk_n[3 * k], nky = snk_n[3 * k + 1], nkz = snk_n[3 * k + 2];
@@ -279,7 +294,8 @@ extern "C" int lem_field_los(const float *src, const float *src_w, int64_t S, co
         return OK;
     LEM_REQUIRE(out && snk && snk_n, "lem_field_los: null pointer");
     LEM_REQUIRE(S == 0 || (src && src_w), "lem_field_los: null source arrays");
-    const int64_t w64 = ceil_div(K, 64);
+    const int sm = (flags & LEM_FIELD_BITS_SINK_MAJOR) ? 1 : 0;
+    const int64_t w64 = sm ? ceil_div(S, 64) : ceil_div(K, 64);
     const int tangent = (flags & LEM_FIELD_SINK_TANGENT) ? 1 : 0;
     hipStream_t st = as_stream(stream);
     const unsigned long long *hb = reinterpret_cast<const unsigned long long *>(hit_bits);
@@ -289,8 +305,8 @@ extern "C" int lem_field_los(const float *src, const float *src_w, int64_t S, co
             return OK;
         LEM_REQUIRE(S * nbk <= 2147483647LL, "lem_field_los: S*K too large for one launch");
         hipLaunchKernelGGL(k_field_los_elements, dim3((unsigned)(S * nbk)), dim3(FB), 0, st, src,
-                           src_w, snk, snk_n, snk_w, K, hb, w64, (float)alpha, beta, tangent, nbk,
-                           out);
+                           src_w, snk, snk_n, snk_w, K, hb, w64, sm, (float)alpha, beta, tangent,
+                           nbk, out);
         return check_launch("k_field_los_elements");
     }
     if (S == 0) {
@@ -309,7 +325,8 @@ extern "C" int lem_field_los(const float *src, const float *src_w, int64_t S, co
     float *ws_cmp = ws_sum + NC * 6 * K;
     LEM_REQUIRE(NC * nbk <= 2147483647LL, "lem_field_los: grid too large");
     hipLaunchKernelGGL(k_field_los_partial, dim3((unsigned)(NC * nbk)), dim3(FB), 0, st, src, src_w,
-                       S, snk, snk_n, K, hb, w64, (float)alpha, beta, CH, nbk, ws_sum, ws_cmp);
+                       S, snk, snk_n, K, hb, w64, sm, (float)alpha, beta, CH, nbk, ws_sum,
+                       ws_cmp);
     int rc = check_launch("k_field_los_partial");
     if (rc)
         return rc;

@@ -23,6 +23,7 @@ VIS_SKIP_SELF = 2
 VIS_SINK_MAJOR = 4
 FIELD_SINK_TANGENT = 1
 FIELD_ELEMENTS = 2
+FIELD_BITS_SINK_MAJOR = 4
 
 # Entry points and their ctypes signatures (mirrors include/lem.h).
 _P = ctypes.c_void_p
@@ -214,8 +215,10 @@ def raycast_pairs(points, pairs, tris, semantics=SEM_NUMBA, any_hit=False, want_
 
 
 def field_los(src, src_w, snk, snk_n, hit_bits=None, snk_w=None, alpha=0.0, beta=2 * np.pi,
-              sink_tangent=False, elements=False):
-    """Fused line-of-sight field: (K,3) complex64 source sum or (S,K,3) network."""
+              sink_tangent=False, elements=False, bits_sink_major=False):
+    """Fused line-of-sight field: (K,3) complex64 source sum or (S,K,3) network.
+    hit_bits: S x W64(K) words, or K x W64(S) with bits_sink_major (the layout
+    of Grid.visibility(sink_major=True))."""
     torch = _torch()
     lib = load_library()
     src = to_dev(src, torch.float32).reshape(-1, 3)
@@ -225,15 +228,17 @@ def field_los(src, src_w, snk, snk_n, hit_bits=None, snk_w=None, alpha=0.0, beta
     src_w = to_dev(src_w, torch.complex64).reshape(S, 3)
     if snk_w is not None:
         snk_w = to_dev(snk_w, torch.complex64).reshape(K, 3)
-    flags = (FIELD_SINK_TANGENT if sink_tangent else 0) | (FIELD_ELEMENTS if elements else 0)
+    flags = (FIELD_SINK_TANGENT if sink_tangent else 0) | (FIELD_ELEMENTS if elements else 0) | \
+        (FIELD_BITS_SINK_MAJOR if bits_sink_major else 0)
     out = torch.empty((S, K, 3) if elements else (K, 3), dtype=torch.complex64,
                       device=src.device)
     wsb = int(lib.lem_field_los_workspace(S, K, flags))
     ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=src.device)
     if hit_bits is not None:
         hit_bits = to_dev(hit_bits, torch.int64)
-        if hit_bits.shape != (S, (K + 63) // 64):
-            raise LemError(f"hit_bits shape {tuple(hit_bits.shape)} != {(S, (K + 63) // 64)}")
+        want = (K, (S + 63) // 64) if bits_sink_major else (S, (K + 63) // 64)
+        if tuple(hit_bits.shape) != want:
+            raise LemError(f"hit_bits shape {tuple(hit_bits.shape)} != {want}")
     check(lib.lem_field_los(ptr(src), ptr(src_w), S, ptr(snk), ptr(snk_n), ptr(snk_w), K,
                             ptr(hit_bits), float(alpha), float(beta), flags, ptr(out), ptr(ws),
                             wsb, stream_ptr()), "lem_field_los")
@@ -510,6 +515,40 @@ def field_los_grid(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, beta=2 *
                                  ctypes.byref(grid.info), ptr(grid.cell_start), ptr(grid.recs),
                                  ptr(grid.cell_box), float(alpha), float(beta), flags, ptr(out),
                                  ptr(ws), wsb, stream_ptr()), "lem_field_los_grid")
+    return out
+
+
+MASK_CHUNK_BYTES = 1 << 31
+
+
+def field_los_grid_sink_major(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0,
+                              beta=2 * np.pi, sink_tangent=False, chunk_bytes=MASK_CHUNK_BYTES):
+    """field_los_grid's result, computed as a sink-major any-hit mask from the
+    persistent grid traversal (each wave casts 64 consecutive sources to one
+    sink) followed by the masked field kernel.  This suits sources that are
+    close together (array elements) and sinks that are not (scatterers spread
+    over a scene), where field_los_grid's waves -- one source to 64 sinks --
+    diverge.  The sinks are processed in chunks whose mask stays within
+    chunk_bytes.  Returns (K,3) complex64."""
+    torch = _torch()
+    src = to_dev(src, torch.float32).reshape(-1, 3)
+    snk = to_dev(snk, torch.float32).reshape(-1, 3)
+    snk_n = to_dev(snk_n, torch.float32).reshape(-1, 3)
+    S, K = src.shape[0], snk.shape[0]
+    out = torch.zeros((K, 3), dtype=torch.complex64, device=snk.device)
+    if S == 0 or K == 0:
+        return out
+    src_w = to_dev(src_w, torch.complex64).reshape(S, 3)
+    if snk_w is not None:
+        snk_w = to_dev(snk_w, torch.complex64).reshape(K, 3)
+    row_bytes = 8 * ((S + 63) // 64)
+    kc = max(1, int(chunk_bytes) // row_bytes)
+    for k0 in range(0, K, kc):
+        k1 = min(K, k0 + kc)
+        bits, _ = grid.visibility(src, snk[k0:k1], any_hit=True, sink_major=True)
+        out[k0:k1] = field_los(src, src_w, snk[k0:k1], snk_n[k0:k1], hit_bits=bits,
+                               snk_w=None if snk_w is None else snk_w[k0:k1], alpha=alpha,
+                               beta=beta, sink_tangent=sink_tangent, bits_sink_major=True)
     return out
 
 

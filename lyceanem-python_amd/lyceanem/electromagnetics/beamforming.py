@@ -127,6 +127,22 @@ def command_weights(Et, Ep, points, wavelength, az_range, elev_range, az_index, 
     return W, pstar
 
 
+class _ZeroMapKernel:
+    """GPUBeamformingMap (beamforming.py:1553-1562) is unfinished in the
+    reference: launched as GPUBeamformingMap[blocks, threads](...), every
+    thread inside (az_range, el_range) writes DirectivityMap[az, el] = 0.  The
+    same launch form and result are kept; there is no computation to port."""
+
+    def __getitem__(self, launch):
+        return self
+
+    def __call__(self, Etheta, Ephi, DirectivityMap, az_range, el_range, wavelength):
+        DirectivityMap[:len(az_range), :len(el_range)] = 0
+
+
+GPUBeamformingMap = _ZeroMapKernel()
+
+
 def MaximumDirectivityMap(Etheta, Ephi, source_coords, wavelength, az_range=None,
                           elev_range=None, az_index=None, elev_index=None, forming="Total",
                           total_solid_angle=(4 * np.pi), phase_resolution=24):

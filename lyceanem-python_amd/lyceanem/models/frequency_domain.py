@@ -165,8 +165,10 @@ def one_bounce_fields(src, src_w, snk, snk_n, scat, scat_n, tris9, scat_w=None, 
     if P == 0:
         return np.zeros((K, 3), np.complex128)
     if grid is not None:
-        H = _native.field_los_grid(grid, src, src_w, scat, scat_n, snk_w=scat_w, alpha=alpha,
-                                   beta=beta, sink_tangent=True)
+        # sources are the (compact) aperture, scatterers are spread over the
+        # scene: cast 64 neighbouring sources to one scatterer per wave
+        H = _native.field_los_grid_sink_major(grid, src, src_w, scat, scat_n, snk_w=scat_w,
+                                              alpha=alpha, beta=beta, sink_tangent=True)
     else:
         bits = _visibility_bits(src, scat, tris9, None)
         H = _native.field_los(src, src_w, scat, scat_n, hit_bits=bits, snk_w=scat_w,

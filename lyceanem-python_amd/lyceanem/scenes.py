@@ -243,7 +243,7 @@ def batched_c5(n_src_side=128, n_el=128, n_az=128, n_freq=32, f_lo=2e9, f_hi=4e9
                 wavenumbers=kf, W=W, directions=u)
 
 
-def time_domain_td(n_src_side=64, n_snk=256, freq=3e9, fs=40e9, n_samples=65536, n_exc=256,
+def time_domain_td(n_src_side=64, n_snk=4096, freq=3e9, fs=40e9, n_samples=65536, n_exc=256,
                    r_min=50.0, r_max=500.0, seed=13):
     """Time-domain workload: an n x n array at lambda/2 in z=0 (normals +z,
     x-polarised, weight = Area = (lambda/2)^2), n_snk sinks 50-500 m away in the

@@ -133,3 +133,41 @@ def test_fused_grid_field_matches_masked_field(native):
                                   sink_tangent=tangent).cpu().numpy()
         # identical masks and per-pair arithmetic, same summation order
         assert np.array_equal(a, b)
+
+
+def test_sink_major_mask_field_matches_fused(native):
+    """The field read through a sink-major mask (K x W64(S)), whole or in sink
+    chunks, equals the fused grid field and the source-major masked field."""
+    src, snk, tris = _small_tri_scene(8, S=150, K=230)
+    rng = np.random.default_rng(8)
+    w = (rng.normal(size=(len(src), 3)) + 1j * rng.normal(size=(len(src), 3))).astype(np.complex64)
+    nrm = rng.normal(size=(len(snk), 3)).astype(np.float32)
+    wk = (rng.normal(size=(len(snk), 3)) + 1j * rng.normal(size=(len(snk), 3))).astype(np.complex64)
+    g = native.Grid(tris)
+    beta = 2 * np.pi / 0.3
+    bits_sm, _ = g.visibility(src, snk, any_hit=True, sink_major=True)
+    bits, _ = g.visibility(src, snk, any_hit=True)
+    m = native.unpack_bits(bits, len(snk))
+    assert np.array_equal(native.unpack_bits(bits_sm, len(src)).T, m)
+    assert 0.05 < m.mean() < 0.95
+    for tangent in (False, True):
+        ref = native.field_los_grid(g, src, w, snk, nrm, snk_w=wk, beta=beta,
+                                    sink_tangent=tangent).cpu().numpy()
+        a = native.field_los(src, w, snk, nrm, hit_bits=bits_sm, snk_w=wk, beta=beta,
+                             sink_tangent=tangent, bits_sink_major=True).cpu().numpy()
+        assert np.array_equal(a, ref)
+        # 5 sink chunks of 50 rows (one row is 3 words here)
+        for cb in (1 << 31, 24 * 50):
+            b = native.field_los_grid_sink_major(g, src, w, snk, nrm, snk_w=wk, beta=beta,
+                                                 sink_tangent=tangent,
+                                                 chunk_bytes=cb).cpu().numpy()
+            assert np.array_equal(b, ref)
+    # per-element network from either layout
+    e1 = native.field_los(src, w, snk, nrm, hit_bits=bits, snk_w=wk, beta=beta,
+                          elements=True).cpu().numpy()
+    e2 = native.field_los(src, w, snk, nrm, hit_bits=bits_sm, snk_w=wk, beta=beta,
+                          elements=True, bits_sink_major=True).cpu().numpy()
+    assert np.array_equal(e1, e2)
+    assert np.array_equal(e1[m.astype(bool)], np.zeros_like(e1[m.astype(bool)]))
+    with pytest.raises(native.LemError, match="hit_bits shape"):
+        native.field_los(src, w, snk, nrm, hit_bits=bits, bits_sink_major=True)

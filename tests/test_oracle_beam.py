@@ -89,3 +89,14 @@ def test_command_weights_match_per_command_builders():
                                        atol=1e-6)
             np.testing.assert_allclose(W[a, e, 2], BF.WavefrontWeights(pts, sv, 0.1), atol=1e-6)
             assert (pstar[a, e] == e * 13 + a).all()
+
+
+def test_gpu_beamforming_map_launch_form():
+    """The reference's GPUBeamformingMap is an unfinished kernel that zeroes
+    the map inside (az_range, el_range); the launch form [blocks, threads] and
+    that result are kept."""
+    from lyceanem.electromagnetics.beamforming import GPUBeamformingMap
+
+    m = np.ones((5, 4))
+    GPUBeamformingMap[(1, 1), (16, 16)](None, None, m, np.zeros(3), np.zeros(2), 0.1)
+    assert np.all(m[:3, :2] == 0) and m[3:].all() and m[:, 2:].all()
