@@ -158,10 +158,11 @@ class C2:
     name = "c2"
     stats_stride = 8  # MT tests counted on every 8th sink, scaled
 
-    def __init__(self, torch, n_sinks=16384, cells_per_tri=2.0):
+    def __init__(self, torch, n_sinks=16384, cells_per_tri=2.0, input_order=False):
         from lyceanem import _native, scenes
 
         self.torch, self.nat = torch, _native
+        self.input_order = input_order
         self.lib = _native.load_library()
         t0 = time.perf_counter()
         sc = scenes.terrain_c2(n_snk=n_sinks)
@@ -186,13 +187,15 @@ class C2:
 
     def step(self, ev=None):
         nat, lib, p, g = self.nat, self.lib, self.nat.ptr, self.grid
+        flags = nat.VIS_SINK_MAJOR | (nat.VIS_INPUT_ORDER if self.input_order else 0)
+        ws = g.vis_workspace(self.S, self.K, flags)
         if ev is not None:
             ev[0].record()
         nat.check(lib.lem_visibility_grid(p(self.src), self.S, p(self.snk), self.K,
                                           nat.ctypes.byref(g.info), p(g.cell_start), p(g.recs),
-                                          p(g.cell_box), nat.SEM_NUMBA, nat.VIS_SINK_MAJOR,
-                                          p(self.bits), p(self.dist), p(g.vis_ws),
-                                          g.vis_ws.numel(), nat.stream_ptr()), "visibility_grid")
+                                          p(g.cell_box), nat.SEM_NUMBA, flags, p(self.bits),
+                                          p(self.dist), p(ws), ws.numel(), nat.stream_ptr()),
+                  "visibility_grid")
         if ev is not None:
             ev[1].record()
             ev[2].record()
@@ -220,6 +223,7 @@ class C2:
                          "rays of this launch)"},
             "extra": {
                 "grid_build_s": self.build_s,
+                "lane_order": "input" if self.input_order else "morton",
                 "grid_cells": [int(x) for x in self.grid.info.n],
                 "grid_refs": int(self.grid.info.nrefs),
                 "conservative_mt_tests_per_ray": conservative / (self.S * sub.shape[0]),
@@ -269,10 +273,11 @@ class C2:
 class C3:
     name = "c3"
 
-    def __init__(self, torch):
+    def __init__(self, torch, from_sink=True):
         from lyceanem import _native, scenes
 
         self.torch, self.nat = torch, _native
+        self.from_sink = from_sink
         sc = scenes.clutter_c3()
         self.sc = sc
         dev = _native.device()
@@ -291,8 +296,10 @@ class C3:
         if ev is not None:
             ev[0].record()
         # leg 1 (array -> scatterers): sink-major any-hit mask, 64 neighbouring
-        # elements to one scatterer per wave, then the masked field
-        bits, _ = g.visibility(t["src"], t["scat"], any_hit=True, sink_major=True)
+        # elements to one scatterer per wave, each ray walked from the
+        # scatterer end, then the masked field
+        bits, _ = g.visibility(t["src"], t["scat"], any_hit=True, sink_major=True,
+                               from_sink=self.from_sink)
         if ev is not None:
             ev[1].record()
         H = nat.field_los(t["src"], t["w"], t["scat"], t["scat_n"], hit_bits=bits,
@@ -323,6 +330,7 @@ class C3:
             "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                          "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None},
             "extra": {"gpairs_per_s": self.rays() / ((vis + rest) * 1e-3) / 1e9,
+                      "leg1_walk": "from scatterer" if self.from_sink else "from array",
                       "leg1_mask_grays_per_s": pairs1 / (vis * 1e-3) / 1e9,
                       "leg1_field_legs2_los_ms": rest,
                       "grid_cells": [int(x) for x in self.grid.info.n],
@@ -746,6 +754,12 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--c2-sinks", type=int, default=16384, help="c2 sink count (probe runs)")
     ap.add_argument("--cells-per-tri", type=float, default=2.0, help="grid cell budget")
+    ap.add_argument("--c3-walk-from-source", action="store_true",
+                    help="c3: walk the leg-1 rays from the array end (default: from the "
+                         "scatterer end)")
+    ap.add_argument("--input-order", action="store_true",
+                    help="c2: wavefronts take sources in input order (64-source rows) instead "
+                         "of the default Morton order")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB, 3 times (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -771,8 +785,12 @@ def main(argv=None) -> int:
         print("bench.py needs a HIP device", file=sys.stderr)
         return 2
     torch.cuda.set_device(0)
-    cfg = C2(torch, a.c2_sinks, a.cells_per_tri) if a.config == "c2" else \
-        CONFIGS[a.config](torch)
+    if a.config == "c2":
+        cfg = C2(torch, a.c2_sinks, a.cells_per_tri, a.input_order)
+    elif a.config == "c3":
+        cfg = C3(torch, from_sink=not a.c3_walk_from_source)
+    else:
+        cfg = CONFIGS[a.config](torch)
     print(f"[bench] config {a.config}: setup done", file=sys.stderr, flush=True)
     for _ in range(a.warmup):
         cfg.step()

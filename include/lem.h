@@ -54,6 +54,15 @@ extern "C" {
                                raycasting_accelerated.cuh:72) */
 #define LEM_VIS_SINK_MAJOR 4 /* grid traversal only: lanes run over sources, bits
                                 are K x ceil(S/64) words and dist is K x S */
+#define LEM_VIS_INPUT_ORDER 8 /* grid traversal only: wavefronts take the lane-side
+                                 points (sources when sink-major, else sinks) in
+                                 input order instead of a spatial (Morton) order;
+                                 for inputs already ordered in compact groups of 64 */
+#define LEM_VIS_FROM_SINK 16 /* grid traversal, any-hit only: walk each ray's cells from
+                                its sink end (the same segment; every triangle is
+                                still tested with the source -> sink ray, so the
+                                mask is unchanged); finds blockers that sit near the
+                                sinks -- e.g. a scatterer's own surface -- first */
 
 /* Field flags. */
 #define LEM_FIELD_SINK_TANGENT 1 /* project onto the sink tangent plane, as
@@ -166,10 +175,14 @@ int lem_grid_fill(const float *tris, int64_t T, const lem_grid_info *info, int32
  * 2e-7 + one float32 ulp of each other).  Replaces the tile traversal raycast_tiles / DDA
  * (raycasting_accelerated.cuh:106-268, 458-482) for source -> sink rays.
  * Flags: LEM_VIS_ANY_HIT, LEM_VIS_SINK_MAJOR, LEM_VIS_SKIP_SELF.
- * LEM_SEM_NATIVE is any-hit (no distance).  The workspace
- * (lem_visibility_grid_workspace() bytes) holds the task counter of the
- * persistent traversal: calls that overlap on one workspace race. */
-int64_t lem_visibility_grid_workspace(void);
+ * LEM_SEM_NATIVE is any-hit (no distance).  Unless LEM_VIS_INPUT_ORDER is
+ * set, the lane-side points are first sorted into a 3-D Morton order so that
+ * each wavefront's 64 rays are spatially compact; results are written at the
+ * original indices (hit_bits is cleared and bits are set atomically).  The
+ * workspace (lem_visibility_grid_workspace(S, K, flags) bytes) holds the task
+ * counter of the persistent traversal and the sort: calls that overlap on one
+ * workspace race. */
+int64_t lem_visibility_grid_workspace(int64_t S, int64_t K, int flags);
 int lem_visibility_grid(const float *src, int64_t S, const float *snk, int64_t K,
                         const lem_grid_info *info, const int32_t *cell_start, const float *recs,
                         const float *cell_box, int semantics, int flags, uint64_t *hit_bits,

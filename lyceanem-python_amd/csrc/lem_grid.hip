@@ -30,6 +30,8 @@
 #include <algorithm>
 #include <cstring>
 
+#include <hipcub/hipcub.hpp>
+
 namespace lem {
 
 constexpr int GB = 256;
@@ -579,13 +581,18 @@ LEM_DEV float replay_window(const Ray &r, double tstar, const GridDev &G,
 // or corner only shifts the walk within the eps inflation of the boxes).
 // LDSM: macro boxes come from the quantised table `smq` (LDS) instead of the
 // float boxes in global memory.
+// rt (any-hit only): the ray whose predicate decides; the walk then covers
+// the same segment with r (e.g. charged from the other end, to reach hits
+// near the sink first) and every triangle is tested with *rt.  The mask of
+// any-hit mode does not depend on the order of the tests.
 template <bool ANY, int SEM = 0, bool LDSM = false, bool PAR = true>
 LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
                       const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
                       const float4 *__restrict__ box, unsigned int *n_tests = nullptr,
                       unsigned int *n_cells = nullptr, unsigned int *n_macro = nullptr,
-                      const uint2 *smq = nullptr)
+                      const uint2 *smq = nullptr, const Ray *rt = nullptr)
 {
+    const Ray &rq = rt ? *rt : r;  // the predicate's ray
     Walk w;
     if (!walk_begin(r, G, w))
         return false;
@@ -726,13 +733,13 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
             for (int32_t j = b; j < e; ++j) {
                 Tri t = load_rec(recs, j);
                 if (SEM == 1) {
-                    if (hit_native(r, t))
+                    if (hit_native(rq, t))
                         return true;
                     continue;
                 }
                 double tt;
                 if (ANY) {
-                    if (hit_numba(r, dist, t, tt)) {
+                    if (hit_numba(rq, rt ? rq.len : dist, t, tt)) {
                         dist = (float)tt;
                         return true;
                     }
@@ -853,32 +860,48 @@ LEM_DEV int64_t xcd_block(int64_t b, int64_t nb)
     return (b % 8) * per + b / 8;
 }
 
-template <bool ANY, bool SINK_MAJOR, int SEM>
+// REV (any-hit): each ray is walked from its sink end (see dda_cast's rt).
+template <bool ANY, bool SINK_MAJOR, int SEM, bool REV = false>
 __global__ __launch_bounds__(GB) void k_visibility_grid(
     const float *__restrict__ src, int64_t S, const float *__restrict__ snk, int64_t K, GridDev G,
     const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
-    const float4 *__restrict__ box, unsigned long long *__restrict__ bits,
-    float *__restrict__ dist_out, int64_t nbl, int skip_self)
+    const float4 *__restrict__ box, const int32_t *__restrict__ order,
+    unsigned long long *__restrict__ bits, float *__restrict__ dist_out, int64_t nbl, int skip_self)
 {
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t outer = blk / nbl;
-    const int64_t inner = (blk % nbl) * GB + threadIdx.x;
+    const int64_t slot = (blk % nbl) * GB + threadIdx.x;
     const int64_t NI = SINK_MAJOR ? S : K;
-    const bool active = inner < NI;
+    const bool active = slot < NI;
+    const int64_t inner = (order && active) ? (int64_t)order[slot] : slot;
     const int64_t s = SINK_MAJOR ? (active ? inner : 0) : outer;
     const int64_t k = SINK_MAJOR ? outer : (active ? inner : 0);
     Ray r = charge(src[3 * s], src[3 * s + 1], src[3 * s + 2], snk[3 * k], snk[3 * k + 1],
                    snk[3 * k + 2]);
     float dist = r.len;
     bool hit = false;
-    if (active && !(skip_self && s == k))
-        hit = dda_cast<ANY, SEM>(r, dist, G, cell_start, recs, box);
-    unsigned long long bw = __ballot(active && hit);
+    if (active && !(skip_self && s == k)) {
+        if (REV) {
+            const Ray rb = charge(snk[3 * k], snk[3 * k + 1], snk[3 * k + 2], src[3 * s],
+                                  src[3 * s + 1], src[3 * s + 2]);
+            float db = rb.len;
+            hit = dda_cast<ANY, SEM>(rb, db, G, cell_start, recs, box, nullptr, nullptr, nullptr,
+                                     nullptr, &r);
+        } else {
+            hit = dda_cast<ANY, SEM>(r, dist, G, cell_start, recs, box);
+        }
+    }
     const int64_t W = (NI + 63) / 64;
-    if ((threadIdx.x & 63) == 0) {
-        int64_t word = inner >> 6;
-        if (word < W)
-            bits[outer * W + word] = bw;
+    if (order) {
+        if (active && hit)
+            atomicOr(&bits[outer * W + (inner >> 6)], 1ull << (inner & 63));
+    } else {
+        unsigned long long bw = __ballot(active && hit);
+        if ((threadIdx.x & 63) == 0) {
+            int64_t word = slot >> 6;
+            if (word < W)
+                bits[outer * W + word] = bw;
+        }
     }
     if (dist_out && active)
         dist_out[outer * NI + inner] = dist;
@@ -891,13 +914,13 @@ constexpr int LDS_WG_PER_CU = 5;  // 5 x 32 KB of the 160 KB LDS of a CU
 // wavefront repeatedly claims the next 64-ray word of output (one atomic on
 // *next_task), so waves never wait for one another and the load balances
 // whatever number of workgroups is resident.  Dead-block tests read LDS.
-template <bool ANY, bool SINK_MAJOR, int SEM>
+template <bool ANY, bool SINK_MAJOR, int SEM, bool REV = false>
 __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
     const float *__restrict__ src, int64_t S, const float *__restrict__ snk, int64_t K, GridDev G,
     const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
     const float4 *__restrict__ box, const uint2 *__restrict__ mq, int nmacro,
-    unsigned long long *__restrict__ bits, float *__restrict__ dist_out,
-    unsigned long long *__restrict__ next_task, int skip_self)
+    const int32_t *__restrict__ order, unsigned long long *__restrict__ bits,
+    float *__restrict__ dist_out, unsigned long long *__restrict__ next_task, int skip_self)
 {
     __shared__ uint2 smq[MACRO_LDS_MAX];
     for (int i = threadIdx.x; i < nmacro; i += GB)
@@ -916,8 +939,10 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
             break;
         const int64_t outer = (int64_t)(task / W);
         const int64_t word = (int64_t)(task % W);
-        const int64_t inner = word * 64 + lane;
-        const bool active = inner < NI;
+        const int64_t slot = word * 64 + lane;
+        const bool active = slot < NI;
+        // with an order, lane slot -> the slot-th point of the spatial order
+        const int64_t inner = (order && active) ? (int64_t)order[slot] : slot;
         const int64_t s = SINK_MAJOR ? (active ? inner : 0) : outer;
         const int64_t k = SINK_MAJOR ? outer : (active ? inner : 0);
         Ray r = charge(src[3 * s], src[3 * s + 1], src[3 * s + 2], snk[3 * k], snk[3 * k + 1],
@@ -925,17 +950,30 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
         float dist = r.len;
         bool hit = false;
         if (active && !(skip_self && s == k)) {
+            // REV: walk from the sink end, test with the charged ray r
+            const Ray rw = REV ? charge(snk[3 * k], snk[3 * k + 1], snk[3 * k + 2], src[3 * s],
+                                        src[3 * s + 1], src[3 * s + 2])
+                               : r;
+            float dw = rw.len;
+            float &dd = REV ? dw : dist;
+            const Ray *rt = REV ? &r : nullptr;
             // rays along a grid plane take the general slab test
-            if (r.dx == 0.0f || r.dy == 0.0f || r.dz == 0.0f)
-                hit = dda_cast<ANY, SEM, true, true>(r, dist, G, cell_start, recs, box, nullptr,
-                                                     nullptr, nullptr, smq);
+            if (rw.dx == 0.0f || rw.dy == 0.0f || rw.dz == 0.0f)
+                hit = dda_cast<ANY, SEM, true, true>(rw, dd, G, cell_start, recs, box, nullptr,
+                                                     nullptr, nullptr, smq, rt);
             else
-                hit = dda_cast<ANY, SEM, true, false>(r, dist, G, cell_start, recs, box, nullptr,
-                                                      nullptr, nullptr, smq);
+                hit = dda_cast<ANY, SEM, true, false>(rw, dd, G, cell_start, recs, box, nullptr,
+                                                      nullptr, nullptr, smq, rt);
         }
-        const unsigned long long bw = __ballot(active && hit);
-        if (lane == 0)
-            bits[outer * W + word] = bw;
+        if (order) {
+            // the lanes' bits belong to several words (zeroed beforehand)
+            if (active && hit)
+                atomicOr(&bits[outer * W + (inner >> 6)], 1ull << (inner & 63));
+        } else {
+            const unsigned long long bw = __ballot(active && hit);
+            if (lane == 0)
+                bits[outer * W + word] = bw;
+        }
         if (dist_out && active)
             dist_out[outer * NI + inner] = dist;
     }
@@ -945,6 +983,108 @@ __global__ void k_zero_u64(unsigned long long *__restrict__ p)
 {
     if (threadIdx.x == 0)
         p[0] = 0ull;
+}
+
+__global__ __launch_bounds__(GB) void k_zero_words(unsigned long long *__restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x; i < n; i += (int64_t)gridDim.x * GB)
+        p[i] = 0ull;
+}
+
+// Spatial order of the points a wavefront's lanes run over: 3-D Morton keys
+// on a cubic lattice over the points' bounding box (1024 cells along its
+// longest side), sorted stably.  64 consecutive points of this order form a
+// compact cluster (an 8 x 8 patch of a planar array) instead of, say, one
+// 64-point row, so a wave's rays walk nearly the same cells and their loops
+// end together.
+LEM_DEV uint32_t spread3(uint32_t v)
+{
+    // 10 bits -> every third bit of 30
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+LEM_DEV float fkey_dec(unsigned int k)
+{
+    const unsigned int u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __builtin_bit_cast(float, u);
+}
+
+__global__ __launch_bounds__(GB) void k_morton_keys(const float *__restrict__ p, int64_t n,
+                                                    const unsigned int *__restrict__ b,
+                                                    uint32_t *__restrict__ keys,
+                                                    int32_t *__restrict__ vals)
+{
+    const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (i >= n)
+        return;
+    float lo[3], ext = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = fkey_dec(b[a]);
+        ext = fmaxf(ext, fkey_dec(b[3 + a]) - lo[a]);
+    }
+    const float inv = ext > 0.0f ? 1023.0f / ext : 0.0f;
+    uint32_t key = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float f = (p[3 * i + a] - lo[a]) * inv;
+        const uint32_t q = f >= 1023.0f ? 1023u : (f > 0.0f ? (uint32_t)f : 0u);  // NaN -> 0
+        key |= spread3(q) << a;
+    }
+    keys[i] = key;
+    vals[i] = (int32_t)i;
+}
+
+// Workspace of lem_visibility_grid: the task counter and point bounds, then
+// (unless the input order is kept) keys, values and order of the lane-side
+// points and the radix sort's temporary storage.
+struct VisWs {
+    unsigned long long *counter;
+    unsigned int *bounds;
+    uint32_t *k0, *k1;
+    int32_t *v0, *order;
+    void *cub;
+    size_t cub_bytes;
+};
+
+constexpr int MORTON_BITS = 30;
+
+inline int64_t vis_sort_bytes(int64_t n)
+{
+    size_t bytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t *)nullptr,
+                                           (uint32_t *)nullptr, (const int32_t *)nullptr,
+                                           (int32_t *)nullptr, (int)n, 0,
+                                           MORTON_BITS) != hipSuccess)
+        return -1;
+    return (int64_t)bytes;
+}
+
+inline int64_t vis_layout(int64_t n, size_t cub_bytes, char *base, VisWs *L)
+{
+    auto al = [](int64_t b) { return ((b + 255) / 256) * 256; };
+    int64_t off = 256;
+    char *p[4];
+    for (int i = 0; i < 4; ++i) {
+        p[i] = base ? base + off : nullptr;
+        off += al(4 * n);
+    }
+    if (L) {
+        L->counter = reinterpret_cast<unsigned long long *>(base);
+        L->bounds = reinterpret_cast<unsigned int *>(base + 64);
+        L->k0 = reinterpret_cast<uint32_t *>(p[0]);
+        L->k1 = reinterpret_cast<uint32_t *>(p[1]);
+        L->v0 = reinterpret_cast<int32_t *>(p[2]);
+        L->order = reinterpret_cast<int32_t *>(p[3]);
+        L->cub = base + off;
+        L->cub_bytes = cub_bytes;
+    }
+    return off + (int64_t)cub_bytes;
 }
 
 // Instrumented traversal: counts the Moller-Trumbore tests and cell visits a
@@ -1274,7 +1414,20 @@ extern "C" int lem_grid_fill(const float *tris, int64_t T, const lem_grid_info *
     return check_launch("lem_grid_fill");
 }
 
-extern "C" int64_t lem_visibility_grid_workspace(void) { return 256; }
+extern "C" int64_t lem_visibility_grid_workspace(int64_t S, int64_t K, int flags)
+{
+    if (S < 0 || K < 0)
+        return -1;
+    const int64_t NI = (flags & LEM_VIS_SINK_MAJOR) ? S : K;
+    if ((flags & LEM_VIS_INPUT_ORDER) || NI == 0)
+        return 256;
+    if (NI > 2147483647LL)
+        return -1;
+    const int64_t cub = vis_sort_bytes(NI);
+    if (cub < 0)
+        return -1;
+    return vis_layout(NI, (size_t)cub, nullptr, nullptr);
+}
 
 extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk, int64_t K,
                                    const lem_grid_info *info, const int32_t *cell_start,
@@ -1294,22 +1447,57 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
     const int skip_self = (flags & LEM_VIS_SKIP_SELF) ? 1 : 0;
     const bool any = (flags & LEM_VIS_ANY_HIT) != 0 || semantics == LEM_SEM_NATIVE;
     const bool sink_major = (flags & LEM_VIS_SINK_MAJOR) != 0;
+    const bool rev = (flags & LEM_VIS_FROM_SINK) != 0;
     LEM_REQUIRE(!(any && dist), "lem_visibility_grid: dist needs the closest-hit mode");
+    LEM_REQUIRE(!rev || any, "lem_visibility_grid: LEM_VIS_FROM_SINK needs the any-hit mode");
     const int64_t NI = sink_major ? S : K, NO = sink_major ? K : S;
+    LEM_REQUIRE(NI <= 2147483647LL, "lem_visibility_grid: more than 2^31-1 lane-side points");
     const int64_t nbl = ceil_div(NI, GB);
-    LEM_REQUIRE(NO * nbl <= 2147483647LL, "lem_visibility_grid: S*K too large for one launch");
     GridDev G = to_dev(*info);
     hipStream_t st = as_stream(stream);
-    dim3 grid((unsigned)(NO * nbl));
+    const int64_t need = lem_visibility_grid_workspace(S, K, flags);
+    if (need < 0) {
+        set_error("lem_visibility_grid: workspace size query failed");
+        return E_HIP;
+    }
+    if (!workspace || workspace_bytes < need) {
+        set_error("lem_visibility_grid: workspace smaller than lem_visibility_grid_workspace()");
+        return E_WORKSPACE;
+    }
     unsigned long long *hb = reinterpret_cast<unsigned long long *>(hit_bits);
     const float4 *rc4 = reinterpret_cast<const float4 *>(recs);
     const float4 *bx4 = reinterpret_cast<const float4 *>(cell_box);
     const uint2 *mq = reinterpret_cast<const uint2 *>(cell_box + 8 * (info->ncells + info->nmacro));
-    if (info->nmacro <= MACRO_LDS_MAX) {
-        if (!workspace || workspace_bytes < lem_visibility_grid_workspace()) {
-            set_error("lem_visibility_grid: workspace smaller than lem_visibility_grid_workspace()");
-            return E_WORKSPACE;
+    VisWs L;
+    const int64_t cub = (flags & LEM_VIS_INPUT_ORDER) ? 0 : vis_sort_bytes(NI);
+    vis_layout(NI, (size_t)std::max<int64_t>(cub, 0), reinterpret_cast<char *>(workspace), &L);
+    const int32_t *order = nullptr;
+    if (!(flags & LEM_VIS_INPUT_ORDER)) {
+        // spatial order of the lane-side points; bits are then set per lane
+        const float *pts = sink_major ? src : snk;
+        hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, st, L.bounds);
+        hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(ceil_div(NI, GB), 1024)),
+                           dim3(GB), 0, st, pts, 3 * NI, L.bounds);
+        hipLaunchKernelGGL(k_morton_keys, dim3((unsigned)ceil_div(NI, GB)), dim3(GB), 0, st, pts,
+                           NI, L.bounds, L.k0, L.v0);
+        int rc = check_launch("k_morton_keys");
+        if (rc)
+            return rc;
+        size_t tb = L.cub_bytes;
+        if (hipcub::DeviceRadixSort::SortPairs(L.cub, tb, L.k0, L.k1, L.v0, L.order, (int)NI, 0,
+                                               MORTON_BITS, st) != hipSuccess) {
+            set_error("lem_visibility_grid: radix sort failed");
+            return E_HIP;
         }
+        const int64_t nw = NO * ceil_div(NI, 64);
+        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<int64_t>(ceil_div(nw, GB), 65536)),
+                           dim3(GB), 0, st, hb, nw);
+        rc = check_launch("k_zero_words");
+        if (rc)
+            return rc;
+        order = L.order;
+    }
+    if (info->nmacro <= MACRO_LDS_MAX) {
         // persistent launch: as many workgroups as are resident at once.  The
         // registers are bounded for LDS_WG_PER_CU waves per SIMD (launch
         // bounds), so the LDS per CU sets the count.
@@ -1326,50 +1514,61 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
         const int64_t words = NO * ceil_div(NI, 64);
         const int64_t nwg =
             std::max<int64_t>(1, std::min<int64_t>(ceil_div(words, 4), (int64_t)cus * per_cu));
-        unsigned long long *counter = reinterpret_cast<unsigned long long *>(workspace);
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, counter);
-#define LEM_VGL(A, M, SM)                                                                     \
-    hipLaunchKernelGGL((k_visibility_grid_lds<A, M, SM>), dim3((unsigned)nwg), dim3(GB), 0, st,    \
-                       src, S, snk, K, G, cell_start, rc4, bx4, mq, info->nmacro, hb, dist,        \
-                       counter, skip_self)
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, L.counter);
+#define LEM_VGL(A, M, SM, R)                                                                  \
+    hipLaunchKernelGGL((k_visibility_grid_lds<A, M, SM, R>), dim3((unsigned)nwg), dim3(GB), 0, st, \
+                       src, S, snk, K, G, cell_start, rc4, bx4, mq, info->nmacro, order, hb, dist, \
+                       L.counter, skip_self)
+#define LEM_VGL2(A, SM, R)                                                                    \
+    do {                                                                                      \
+        if (sink_major)                                                                       \
+            LEM_VGL(A, true, SM, R);                                                          \
+        else                                                                                  \
+            LEM_VGL(A, false, SM, R);                                                         \
+    } while (0)
         if (semantics == LEM_SEM_NATIVE) {
-            if (sink_major)
-                LEM_VGL(true, true, 1);
+            if (rev)
+                LEM_VGL2(true, 1, true);
             else
-                LEM_VGL(true, false, 1);
+                LEM_VGL2(true, 1, false);
         } else if (any) {
-            if (sink_major)
-                LEM_VGL(true, true, 0);
+            if (rev)
+                LEM_VGL2(true, 0, true);
             else
-                LEM_VGL(true, false, 0);
+                LEM_VGL2(true, 0, false);
         } else {
-            if (sink_major)
-                LEM_VGL(false, true, 0);
-            else
-                LEM_VGL(false, false, 0);
+            LEM_VGL2(false, 0, false);
         }
+#undef LEM_VGL2
 #undef LEM_VGL
         return check_launch("k_visibility_grid_lds");
     }
-#define LEM_VG(A, M, SM)                                                                      \
-    hipLaunchKernelGGL((k_visibility_grid<A, M, SM>), grid, dim3(GB), 0, st, src, S, snk, K, G,   \
-                       cell_start, rc4, bx4, hb, dist, nbl, skip_self)
+    LEM_REQUIRE(NO * nbl <= 2147483647LL, "lem_visibility_grid: S*K too large for one launch");
+    dim3 grid((unsigned)(NO * nbl));
+#define LEM_VG(A, M, SM, R)                                                                   \
+    hipLaunchKernelGGL((k_visibility_grid<A, M, SM, R>), grid, dim3(GB), 0, st, src, S, snk, K,   \
+                       G, cell_start, rc4, bx4, order, hb, dist, nbl, skip_self)
+#define LEM_VG2(A, SM, R)                                                                     \
+    do {                                                                                      \
+        if (sink_major)                                                                       \
+            LEM_VG(A, true, SM, R);                                                           \
+        else                                                                                  \
+            LEM_VG(A, false, SM, R);                                                          \
+    } while (0)
     if (semantics == LEM_SEM_NATIVE) {
-        if (sink_major)
-            LEM_VG(true, true, 1);
+        if (rev)
+            LEM_VG2(true, 1, true);
         else
-            LEM_VG(true, false, 1);
+            LEM_VG2(true, 1, false);
     } else if (any) {
-        if (sink_major)
-            LEM_VG(true, true, 0);
+        if (rev)
+            LEM_VG2(true, 0, true);
         else
-            LEM_VG(true, false, 0);
+            LEM_VG2(true, 0, false);
     } else {
-        if (sink_major)
-            LEM_VG(false, true, 0);
-        else
-            LEM_VG(false, false, 0);
+        LEM_VG2(false, 0, false);
     }
+#undef LEM_VG2
 #undef LEM_VG
     return check_launch("k_visibility_grid");
 }

@@ -21,6 +21,8 @@ SEM_NATIVE = 1
 VIS_ANY_HIT = 1
 VIS_SKIP_SELF = 2
 VIS_SINK_MAJOR = 4
+VIS_INPUT_ORDER = 8
+VIS_FROM_SINK = 16
 FIELD_SINK_TANGENT = 1
 FIELD_ELEMENTS = 2
 FIELD_BITS_SINK_MAJOR = 4
@@ -44,7 +46,7 @@ SIGNATURES = {
     "lem_grid_workspace": ([_I64, ctypes.c_float], _I64),
     "lem_grid_plan": ([_P, _I64, ctypes.c_float, _P, _P, _I64, _P], _INT),
     "lem_grid_fill": ([_P, _I64, _P, _P, _P, _P, _P, _P, _I64, ctypes.c_float, _P], _INT),
-    "lem_visibility_grid_workspace": ([], _I64),
+    "lem_visibility_grid_workspace": ([_I64, _I64, _INT], _I64),
     "lem_visibility_grid": ([_P, _I64, _P, _I64, _P, _P, _P, _P, _INT, _INT, _P, _P, _P, _I64,
                              _P], _INT),
     "lem_visibility_grid_stats": ([_P, _I64, _P, _I64, _P, _P, _P, _P, _INT, _P, _P], _INT),
@@ -462,17 +464,27 @@ class Grid:
         # float boxes of cells and macro blocks, then the 8-byte quantised macro table
         nbox = 8 * (self.info.ncells + self.info.nmacro) + 2 * self.info.nmacro
         self.cell_box = torch.empty(nbox, dtype=torch.float32, device=dev)
-        self.vis_ws = torch.empty(int(lib.lem_visibility_grid_workspace()), dtype=torch.uint8,
-                                  device=dev)
+        self.vis_ws = torch.empty(0, dtype=torch.uint8, device=dev)
         check(lib.lem_grid_fill(ptr(self.tris), T, ctypes.byref(self.info), ptr(self.cell_start),
                                 ptr(self.tri_ids), ptr(self.recs), ptr(self.cell_box),
                                 ptr(self.ws), wsb, self.cells_per_tri, stream_ptr()),
               "lem_grid_fill")
         self.ws = None
 
+    def vis_workspace(self, S, K, flags):
+        """lem_visibility_grid's workspace for (S, K, flags), kept and grown."""
+        wsb = int(load_library().lem_visibility_grid_workspace(S, K, flags))
+        if wsb < 0:
+            raise LemError("lem_visibility_grid_workspace: size query failed")
+        if self.vis_ws.numel() < wsb:
+            self.vis_ws = _torch().empty(wsb, dtype=_torch().uint8, device=self.tris.device)
+        return self.vis_ws
+
     def visibility(self, src, snk, any_hit=True, want_dist=False, sink_major=False,
-                   semantics=SEM_NUMBA, skip_self=False):
-        """Hit bits (S x W64(K), or K x W64(S) sink-major) and optional dist."""
+                   semantics=SEM_NUMBA, skip_self=False, input_order=False, from_sink=False):
+        """Hit bits (S x W64(K), or K x W64(S) sink-major) and optional dist.
+        input_order: lanes take the lane-side points in input order instead of
+        a Morton order.  from_sink (any-hit): walk each ray from its sink end."""
         torch = _torch()
         lib = load_library()
         src = to_dev(src, torch.float32).reshape(-1, 3)
@@ -483,11 +495,13 @@ class Grid:
         dist = torch.empty((rows, cols), dtype=torch.float32, device=src.device) \
             if want_dist else None
         flags = (VIS_ANY_HIT if any_hit else 0) | (VIS_SINK_MAJOR if sink_major else 0) | \
-            (VIS_SKIP_SELF if skip_self else 0)
+            (VIS_SKIP_SELF if skip_self else 0) | (VIS_INPUT_ORDER if input_order else 0) | \
+            (VIS_FROM_SINK if from_sink else 0)
+        ws = self.vis_workspace(S, K, flags)
         check(lib.lem_visibility_grid(ptr(src), S, ptr(snk), K, ctypes.byref(self.info),
                                       ptr(self.cell_start), ptr(self.recs), ptr(self.cell_box),
-                                      semantics, flags, ptr(bits), ptr(dist), ptr(self.vis_ws),
-                                      self.vis_ws.numel(), stream_ptr()),
+                                      semantics, flags, ptr(bits), ptr(dist), ptr(ws),
+                                      ws.numel(), stream_ptr()),
               "lem_visibility_grid")
         return bits, dist
 
@@ -522,14 +536,17 @@ MASK_CHUNK_BYTES = 1 << 31
 
 
 def field_los_grid_sink_major(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0,
-                              beta=2 * np.pi, sink_tangent=False, chunk_bytes=MASK_CHUNK_BYTES):
+                              beta=2 * np.pi, sink_tangent=False, chunk_bytes=MASK_CHUNK_BYTES,
+                              from_sink=True):
     """field_los_grid's result, computed as a sink-major any-hit mask from the
     persistent grid traversal (each wave casts 64 consecutive sources to one
     sink) followed by the masked field kernel.  This suits sources that are
     close together (array elements) and sinks that are not (scatterers spread
     over a scene), where field_los_grid's waves -- one source to 64 sinks --
-    diverge.  The sinks are processed in chunks whose mask stays within
-    chunk_bytes.  Returns (K,3) complex64."""
+    diverge.  from_sink: walk each ray from its sink end, which reaches first
+    the blockers that sit next to sinks on surfaces (a scatterer behind its own
+    face); the mask is the same either way.  The sinks are processed in chunks
+    whose mask stays within chunk_bytes.  Returns (K,3) complex64."""
     torch = _torch()
     src = to_dev(src, torch.float32).reshape(-1, 3)
     snk = to_dev(snk, torch.float32).reshape(-1, 3)
@@ -545,7 +562,8 @@ def field_los_grid_sink_major(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.
     kc = max(1, int(chunk_bytes) // row_bytes)
     for k0 in range(0, K, kc):
         k1 = min(K, k0 + kc)
-        bits, _ = grid.visibility(src, snk[k0:k1], any_hit=True, sink_major=True)
+        bits, _ = grid.visibility(src, snk[k0:k1], any_hit=True, sink_major=True,
+                                  from_sink=from_sink)
         out[k0:k1] = field_los(src, src_w, snk[k0:k1], snk_n[k0:k1], hit_bits=bits,
                                snk_w=None if snk_w is None else snk_w[k0:k1], alpha=alpha,
                                beta=beta, sink_tangent=sink_tangent, bits_sink_major=True)

@@ -37,6 +37,75 @@ def test_grid_matches_brute_force(native, seed):
     assert np.array_equal(native.unpack_bits(bits_a, K), mb)
 
 
+@pytest.mark.parametrize("sink_major", [False, True])
+def test_grid_spatial_order_matches_input_order(native, sink_major):
+    """The default traversal (lanes over a Morton order of the lane-side
+    points, bits set atomically) gives the same masks and distances as lanes in
+    input order, including ragged sizes, skip_self and a bits buffer that
+    holds stale words."""
+    src, snk, tris = _small_tri_scene(4, S=333, K=201)
+    pts = np.concatenate([src[:150], snk[:100]])  # shared rows for skip_self
+    g = native.Grid(tris)
+    for any_hit in (False, True):
+        for skip_self in (False, True):
+            print(f"order sm={sink_major} any={any_hit} skip={skip_self}", flush=True)
+            a_src, a_snk = (pts, pts[::-1].copy()) if skip_self else (src, snk)
+            ref = g.visibility(a_src, a_snk, any_hit=any_hit, want_dist=not any_hit,
+                               sink_major=sink_major, skip_self=skip_self, input_order=True)
+            out = g.visibility(a_src, a_snk, any_hit=any_hit, want_dist=not any_hit,
+                               sink_major=sink_major, skip_self=skip_self)
+            assert np.array_equal(ref[0].cpu().numpy(), out[0].cpu().numpy())
+            if not any_hit:
+                assert np.array_equal(ref[1].cpu().numpy(), out[1].cpu().numpy())
+            else:
+                # walked from the sink end: the same mask
+                for io in (False, True):
+                    rev = g.visibility(a_src, a_snk, any_hit=True, sink_major=sink_major,
+                                       skip_self=skip_self, input_order=io, from_sink=True)
+                    assert np.array_equal(ref[0].cpu().numpy(), rev[0].cpu().numpy())
+    # stale words in the caller's bits buffer are cleared by the call
+    import torch
+    lib = native.load_library()
+    S, K = len(src), len(snk)
+    rows, cols = (K, S) if sink_major else (S, K)
+    flags = native.VIS_ANY_HIT | (native.VIS_SINK_MAJOR if sink_major else 0)
+    bits = torch.full((rows, (cols + 63) // 64), -1, dtype=torch.int64, device="cuda")
+    ws = g.vis_workspace(S, K, flags)
+    s_t, k_t = native.to_dev(src, torch.float32), native.to_dev(snk, torch.float32)
+    native.check(lib.lem_visibility_grid(native.ptr(s_t), S, native.ptr(k_t), K,
+                                         native.ctypes.byref(g.info), native.ptr(g.cell_start),
+                                         native.ptr(g.recs), native.ptr(g.cell_box),
+                                         native.SEM_NUMBA, flags, native.ptr(bits), None,
+                                         native.ptr(ws), ws.numel(), native.stream_ptr()), "vis")
+    ref, _ = g.visibility(src, snk, any_hit=True, sink_major=sink_major, input_order=True)
+    assert np.array_equal(bits.cpu().numpy(), ref.cpu().numpy())
+    with pytest.raises(native.LemError, match="any-hit"):
+        g.visibility(src, snk, any_hit=False, from_sink=True)
+
+
+def test_grid_from_sink_matches_brute_force(native):
+    """Rays ending just behind a triangle (a scatterer lifted off the far side
+    of its face): the walk from the sink end finds the same masks as brute
+    force, native semantics included."""
+    src, snk, tris = _small_tri_scene(9, S=260, K=10)
+    t = tris.reshape(-1, 3, 3).astype(np.float64)
+    n = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    c = t.mean(1)
+    pick = np.arange(0, len(t), 15)
+    scat = np.concatenate([c[pick] + 1e-4 * n[pick], c[pick] - 1e-4 * n[pick]]).astype(np.float32)
+    g = native.Grid(tris)
+    for sem in (native.SEM_NUMBA, native.SEM_NATIVE):
+        ref, _ = native.visibility_dense(src, scat, tris, semantics=sem, any_hit=True)
+        m = native.unpack_bits(ref, len(scat))
+        assert 0.05 < m.mean() < 0.95
+        for sm in (False, True):
+            b, _ = g.visibility(src, scat, any_hit=True, sink_major=sm, semantics=sem,
+                                from_sink=True)
+            mb = native.unpack_bits(b, len(src)).T if sm else native.unpack_bits(b, len(scat))
+            assert np.array_equal(mb, m), f"sem={sem} sm={sm}: {(mb != m).sum()} mismatches"
+
+
 def test_grid_sink_major_layout(native):
     src, snk, tris = _small_tri_scene(3, S=130, K=70)
     g = native.Grid(tris)
