@@ -74,6 +74,8 @@ extern "C" {
                                        word s/64 of row k), the layout
                                        lem_visibility_grid writes with
                                        LEM_VIS_SINK_MAJOR */
+#define LEM_FIELD_INPUT_ORDER 8     /* lem_field_los_grid: lanes take the sinks in input
+                                       order instead of a spatial (Morton) order */
 
 int lem_abi_version(void);
 const char *lem_last_error(void);
@@ -190,8 +192,11 @@ int lem_visibility_grid(const float *src, int64_t S, const float *snk, int64_t K
 
 /* Line-of-sight field fused with the grid traversal: lem_field_los (source
  * sum only) where each ray is cast any-hit through the grid instead of read
- * from a precomputed mask -- the S x K mask is never stored.  Same workspace
- * size as lem_field_los_workspace(S, K, 0). */
+ * from a precomputed mask -- the S x K mask is never stored.  Lanes run over
+ * the sinks in a Morton order unless LEM_FIELD_INPUT_ORDER is set (each sink's
+ * sum, and so the result, is the same).  workspace:
+ * lem_field_los_grid_workspace(S, K, flags) bytes. */
+int64_t lem_field_los_grid_workspace(int64_t S, int64_t K, int flags);
 int lem_field_los_grid(const float *src, const float *src_w, int64_t S, const float *snk,
                        const float *snk_n, const float *snk_w, int64_t K,
                        const lem_grid_info *info, const int32_t *cell_start, const float *recs,

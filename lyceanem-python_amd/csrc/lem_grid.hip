@@ -668,24 +668,27 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
         if (n_macro)
             *n_macro += 1;
         if (!live) {
-            const int a = (Tx <= Ty && Tx <= Tz) ? 0 : (Ty <= Tz ? 1 : 2);
-            const float tex = a == 0 ? Tx : (a == 1 ? Ty : Tz);
+            // leave the block across its nearest face; the axis is selected,
+            // not branched on, so lanes crossing different axes stay together
+            const bool ax = Tx <= Ty && Tx <= Tz;
+            const bool ay = !ax && Ty <= Tz;
+            const float tex = ax ? Tx : (ay ? Ty : Tz);
             if (tex >= t1 + margin || (hit && dist + margin <= tex))
                 break;
-            if (a == 0) {
-                jx += sx;
-                if (jx < 0 || jx >= G.mx) break;
-                Tx = tnext(last(jx, sx, G.nx), sx, G.lox, G.cx, r.ox, idx);
-            } else if (a == 1) {
-                jy += sy;
-                if (jy < 0 || jy >= G.my) break;
-                Ty = tnext(last(jy, sy, G.ny), sy, G.loy, G.cy, r.oy, idy);
-            } else {
-                jz += sz;
-                if (jz < 0 || jz >= G.mz) break;
-                Tz = tnext(last(jz, sz, G.nz), sz, G.loz, G.cz, r.oz, idz);
-            }
-            entry_axis = a;
+            const int sa = ax ? sx : (ay ? sy : sz);
+            const int ja = (ax ? jx : (ay ? jy : jz)) + sa;
+            if (ja < 0 || ja >= (ax ? G.mx : (ay ? G.my : G.mz)))
+                break;
+            const float Ta = tnext(last(ja, sa, ax ? G.nx : (ay ? G.ny : G.nz)), sa,
+                                   ax ? G.lox : (ay ? G.loy : G.loz), ax ? G.cx : (ay ? G.cy : G.cz),
+                                   ax ? r.ox : (ay ? r.oy : r.oz), ax ? idx : (ay ? idy : idz));
+            jx = ax ? ja : jx;
+            jy = ay ? ja : jy;
+            jz = (!ax && !ay) ? ja : jz;
+            Tx = ax ? Ta : Tx;
+            Ty = ay ? Ta : Ty;
+            Tz = (!ax && !ay) ? Ta : Tz;
+            entry_axis = ax ? 0 : (ay ? 1 : 2);
             t_entry = tex;
             fine = false;
             continue;
@@ -766,33 +769,39 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
                 done = true;
                 break;
             }
-            if (tx <= ty && tx <= tz) {
-                ix += sx;
-                if (ix < 0 || ix >= G.nx) { done = true; break; }
-                tx = tnext(ix, sx, G.lox, G.cx, r.ox, idx);
-                if ((ix >> ms) != jx) {
-                    jx = ix >> ms;
-                    Tx = tnext(last(jx, sx, G.nx), sx, G.lox, G.cx, r.ox, idx);
-                    break;
-                }
-            } else if (ty <= tz) {
-                iy += sy;
-                if (iy < 0 || iy >= G.ny) { done = true; break; }
-                ty = tnext(iy, sy, G.loy, G.cy, r.oy, idy);
-                if ((iy >> ms) != jy) {
-                    jy = iy >> ms;
-                    Ty = tnext(last(jy, sy, G.ny), sy, G.loy, G.cy, r.oy, idy);
-                    break;
-                }
-            } else {
-                iz += sz;
-                if (iz < 0 || iz >= G.nz) { done = true; break; }
-                tz = tnext(iz, sz, G.loz, G.cz, r.oz, idz);
-                if ((iz >> ms) != jz) {
-                    jz = iz >> ms;
-                    Tz = tnext(last(jz, sz, G.nz), sz, G.loz, G.cz, r.oz, idz);
-                    break;
-                }
+            // next cell across the nearest face (axis selected, as above)
+            const bool ax = tx <= ty && tx <= tz;
+            const bool ay = !ax && ty <= tz;
+            const bool az = !ax && !ay;
+            const int sa = ax ? sx : (ay ? sy : sz);
+            const int ia = (ax ? ix : (ay ? iy : iz)) + sa;
+            const int na = ax ? G.nx : (ay ? G.ny : G.nz);
+            if (ia < 0 || ia >= na) {
+                done = true;
+                break;
+            }
+            const float loa = ax ? G.lox : (ay ? G.loy : G.loz);
+            const float ca = ax ? G.cx : (ay ? G.cy : G.cz);
+            const float oa = ax ? r.ox : (ay ? r.oy : r.oz);
+            const float ida = ax ? idx : (ay ? idy : idz);
+            const float ta = tnext(ia, sa, loa, ca, oa, ida);
+            ix = ax ? ia : ix;
+            iy = ay ? ia : iy;
+            iz = az ? ia : iz;
+            tx = ax ? ta : tx;
+            ty = ay ? ta : ty;
+            tz = az ? ta : tz;
+            const int ja = ia >> ms;
+            if (ja != (ax ? jx : (ay ? jy : jz))) {
+                // into the next block: its far boundary on this axis
+                const float Ta = tnext(last(ja, sa, na), sa, loa, ca, oa, ida);
+                jx = ax ? ja : jx;
+                jy = ay ? ja : jy;
+                jz = az ? ja : jz;
+                Tx = ax ? Ta : Tx;
+                Ty = ay ? Ta : Ty;
+                Tz = az ? Ta : Tz;
+                break;
             }
         }
         if (done)
@@ -1087,6 +1096,26 @@ inline int64_t vis_layout(int64_t n, size_t cub_bytes, char *base, VisWs *L)
     return off + (int64_t)cub_bytes;
 }
 
+// L.order = the Morton order of n points (the workspace laid out by vis_layout).
+int morton_order(const float *pts, int64_t n, const VisWs &L, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, st, L.bounds);
+    hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(ceil_div(n, GB), 1024)),
+                       dim3(GB), 0, st, pts, 3 * n, L.bounds);
+    hipLaunchKernelGGL(k_morton_keys, dim3((unsigned)ceil_div(n, GB)), dim3(GB), 0, st, pts, n,
+                       L.bounds, L.k0, L.v0);
+    int rc = check_launch("k_morton_keys");
+    if (rc)
+        return rc;
+    size_t tb = L.cub_bytes;
+    if (hipcub::DeviceRadixSort::SortPairs(L.cub, tb, L.k0, L.k1, L.v0, L.order, (int)n, 0,
+                                           MORTON_BITS, st) != hipSuccess) {
+        set_error("radix sort of the Morton keys failed");
+        return E_HIP;
+    }
+    return OK;
+}
+
 // Instrumented traversal: counts the Moller-Trumbore tests and cell visits a
 // traversal performs (the algorithmic work of the roofline), no outputs.
 template <bool ANY>
@@ -1144,15 +1173,16 @@ __global__ __launch_bounds__(FB) void k_field_los_grid_partial(
     const float *__restrict__ src, const float *__restrict__ src_w, int64_t S,
     const float *__restrict__ snk, const float *__restrict__ snk_n, int64_t K, GridDev G,
     const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
-    const float4 *__restrict__ box, float alpha, double beta, int64_t CH, int64_t nbk,
-    float *__restrict__ ws_sum,
-    float *__restrict__ ws_cmp)
+    const float4 *__restrict__ box, const int32_t *__restrict__ order, float alpha, double beta,
+    int64_t CH, int64_t nbk, float *__restrict__ ws_sum, float *__restrict__ ws_cmp)
 {
     __shared__ SrcLds s_src[FS_TILE];
     const int64_t c = (int64_t)blockIdx.x / nbk;
-    const int64_t k = ((int64_t)blockIdx.x % nbk) * FB + threadIdx.x;
-    const bool active = k < K;
-    const int64_t kk = active ? k : 0;
+    const int64_t slot = ((int64_t)blockIdx.x % nbk) * FB + threadIdx.x;
+    const bool active = slot < K;
+    // lanes over a spatial order of the sinks (each sink's sum is unchanged)
+    const int64_t k = active ? (order ? (int64_t)order[slot] : slot) : 0;
+    const int64_t kk = k;
     const float xkf = snk[3 * kk], ykf = snk[3 * kk + 1], zkf = snk[3 * kk + 2];
     const double xk = xkf, yk = ykf, zk = zkf;
     const float nkx = snk_n[3 * kk], nky = snk_n[3 * kk + 1], nkz = snk_n[3 * kk + 2];
@@ -1474,21 +1504,9 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
     const int32_t *order = nullptr;
     if (!(flags & LEM_VIS_INPUT_ORDER)) {
         // spatial order of the lane-side points; bits are then set per lane
-        const float *pts = sink_major ? src : snk;
-        hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, st, L.bounds);
-        hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(ceil_div(NI, GB), 1024)),
-                           dim3(GB), 0, st, pts, 3 * NI, L.bounds);
-        hipLaunchKernelGGL(k_morton_keys, dim3((unsigned)ceil_div(NI, GB)), dim3(GB), 0, st, pts,
-                           NI, L.bounds, L.k0, L.v0);
-        int rc = check_launch("k_morton_keys");
+        int rc = morton_order(sink_major ? src : snk, NI, L, st);
         if (rc)
             return rc;
-        size_t tb = L.cub_bytes;
-        if (hipcub::DeviceRadixSort::SortPairs(L.cub, tb, L.k0, L.k1, L.v0, L.order, (int)NI, 0,
-                                               MORTON_BITS, st) != hipSuccess) {
-            set_error("lem_visibility_grid: radix sort failed");
-            return E_HIP;
-        }
         const int64_t nw = NO * ceil_div(NI, 64);
         hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<int64_t>(ceil_div(nw, GB), 65536)),
                            dim3(GB), 0, st, hb, nw);
@@ -1599,6 +1617,24 @@ extern "C" int lem_visibility_grid_stats(const float *src, int64_t S, const floa
     return check_launch("k_grid_stats");
 }
 
+extern "C" int64_t lem_field_los_grid_workspace(int64_t S, int64_t K, int flags)
+{
+    if (S < 0 || K < 0)
+        return -1;
+    if (S == 0 || K == 0)
+        return 0;
+    const int64_t part = ((2 * los_chunk_count(S, K) * 6 * K * (int64_t)sizeof(float) + 255) / 256) *
+                         256;
+    if (flags & LEM_FIELD_INPUT_ORDER)
+        return part;
+    if (K > 2147483647LL)
+        return -1;
+    const int64_t cub = vis_sort_bytes(K);
+    if (cub < 0)
+        return -1;
+    return part + vis_layout(K, (size_t)cub, nullptr, nullptr);
+}
+
 extern "C" int lem_field_los_grid(const float *src, const float *src_w, int64_t S,
                                   const float *snk, const float *snk_n, const float *snk_w,
                                   int64_t K, const lem_grid_info *info, const int32_t *cell_start,
@@ -1613,9 +1649,13 @@ extern "C" int lem_field_los_grid(const float *src, const float *src_w, int64_t 
                 "lem_field_los_grid: null pointer");
     LEM_REQUIRE(!(flags & LEM_FIELD_ELEMENTS), "lem_field_los_grid: source sum only");
     const int64_t NC = los_chunk_count(S, K);
-    const int64_t need = 2 * NC * 6 * K * (int64_t)sizeof(float);
+    const int64_t need = lem_field_los_grid_workspace(S, K, flags);
+    if (need < 0) {
+        set_error("lem_field_los_grid: workspace size query failed");
+        return E_HIP;
+    }
     if (!workspace || workspace_bytes < need) {
-        set_error("lem_field_los_grid: workspace smaller than lem_field_los_workspace()");
+        set_error("lem_field_los_grid: workspace smaller than lem_field_los_grid_workspace()");
         return E_WORKSPACE;
     }
     const int64_t CH = ceil_div(S, NC);
@@ -1624,11 +1664,21 @@ extern "C" int lem_field_los_grid(const float *src, const float *src_w, int64_t 
     float *ws_sum = reinterpret_cast<float *>(workspace);
     float *ws_cmp = ws_sum + NC * 6 * K;
     hipStream_t st = as_stream(stream);
+    const int32_t *order = nullptr;
+    if (!(flags & LEM_FIELD_INPUT_ORDER)) {
+        const int64_t part = ((2 * NC * 6 * K * (int64_t)sizeof(float) + 255) / 256) * 256;
+        VisWs L;
+        vis_layout(K, (size_t)vis_sort_bytes(K), reinterpret_cast<char *>(workspace) + part, &L);
+        int rc = morton_order(snk, K, L, st);
+        if (rc)
+            return rc;
+        order = L.order;
+    }
     hipLaunchKernelGGL(k_field_los_grid_partial, dim3((unsigned)(NC * nbk)), dim3(FB), 0, st, src,
                        src_w, S, snk, snk_n, K, to_dev(*info), cell_start,
                        reinterpret_cast<const float4 *>(recs),
-                       reinterpret_cast<const float4 *>(cell_box), (float)alpha, beta, CH, nbk,
-                       ws_sum, ws_cmp);
+                       reinterpret_cast<const float4 *>(cell_box), order, (float)alpha, beta, CH,
+                       nbk, ws_sum, ws_cmp);
     int rc = check_launch("k_field_los_grid_partial");
     if (rc)
         return rc;

@@ -26,6 +26,7 @@ VIS_FROM_SINK = 16
 FIELD_SINK_TANGENT = 1
 FIELD_ELEMENTS = 2
 FIELD_BITS_SINK_MAJOR = 4
+FIELD_INPUT_ORDER = 8
 
 # Entry points and their ctypes signatures (mirrors include/lem.h).
 _P = ctypes.c_void_p
@@ -59,6 +60,7 @@ SIGNATURES = {
     "lem_time_map": ([_P, _P, _P, _P, _P, _I64, _P, _I64, _I64, _P, _P], _INT),
     "lem_time_trace_workspace": ([_I64, _I64], _I64),
     "lem_time_trace": ([_P, _P, _I64, _P, _I64, _I64, _P, _P, _I64, _P], _INT),
+    "lem_field_los_grid_workspace": ([_I64, _I64, _INT], _I64),
     "lem_field_los_grid": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _P, _P, _P, _D, _D, _INT, _P,
                             _P, _I64, _P], _INT),
     "lem_gw_mfma": ([_P, _I64, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P, _I64, _P,
@@ -507,9 +509,10 @@ class Grid:
 
 
 def field_los_grid(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, beta=2 * np.pi,
-                   sink_tangent=False):
+                   sink_tangent=False, input_order=False):
     """Source-summed line-of-sight field with visibility cast through `grid`
-    inside the field kernel: (K,3) complex64."""
+    inside the field kernel: (K,3) complex64.  input_order: lanes take the
+    sinks in input order instead of a Morton order (same result)."""
     torch = _torch()
     lib = load_library()
     src = to_dev(src, torch.float32).reshape(-1, 3)
@@ -522,9 +525,11 @@ def field_los_grid(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, beta=2 *
     src_w = to_dev(src_w, torch.complex64).reshape(S, 3)
     if snk_w is not None:
         snk_w = to_dev(snk_w, torch.complex64).reshape(K, 3)
-    wsb = int(lib.lem_field_los_workspace(S, K, 0))
+    flags = (FIELD_SINK_TANGENT if sink_tangent else 0) | (FIELD_INPUT_ORDER if input_order else 0)
+    wsb = int(lib.lem_field_los_grid_workspace(S, K, flags))
+    if wsb < 0:
+        raise LemError("lem_field_los_grid_workspace: size query failed")
     ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=snk.device)
-    flags = FIELD_SINK_TANGENT if sink_tangent else 0
     check(lib.lem_field_los_grid(ptr(src), ptr(src_w), S, ptr(snk), ptr(snk_n), ptr(snk_w), K,
                                  ctypes.byref(grid.info), ptr(grid.cell_start), ptr(grid.recs),
                                  ptr(grid.cell_box), float(alpha), float(beta), flags, ptr(out),

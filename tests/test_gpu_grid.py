@@ -87,7 +87,7 @@ def test_grid_from_sink_matches_brute_force(native):
     """Rays ending just behind a triangle (a scatterer lifted off the far side
     of its face): the walk from the sink end finds the same masks as brute
     force, native semantics included."""
-    src, snk, tris = _small_tri_scene(9, S=260, K=10)
+    src, _, tris = _small_tri_scene(9, n_tris=400, S=260, K=1)
     t = tris.reshape(-1, 3, 3).astype(np.float64)
     n = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
     n /= np.linalg.norm(n, axis=1, keepdims=True)
@@ -202,6 +202,10 @@ def test_fused_grid_field_matches_masked_field(native):
                                   sink_tangent=tangent).cpu().numpy()
         # identical masks and per-pair arithmetic, same summation order
         assert np.array_equal(a, b)
+        # lanes over the sinks in input order instead of the Morton order
+        c = native.field_los_grid(g, src, w, snk, nrm, snk_w=wk, beta=2 * np.pi / 0.3,
+                                  sink_tangent=tangent, input_order=True).cpu().numpy()
+        assert np.array_equal(a, c)
 
 
 def test_sink_major_mask_field_matches_fused(native):
