@@ -158,8 +158,10 @@ typedef struct {
  *    recs (nrefs x 12 float32: v0, e1, e2, the triangle id as int32 bits,
  *    padding; cell by cell, ids ascending within a cell) and
  *    cell_box ((ncells + nmacro) x 8 + nmacro x 2 float32: the tight box of
- *    each cell's contents, lo.xyz, 0, hi.xyz, 0, then the same for each macro
- *    block, then each macro block's box quantised to 8 bytes), and
+ *    each cell's contents, lo.xyz, start, hi.xyz, end -- start/end being
+ *    cell_start[c] and cell_start[c + 1] as int32 bits --, then each macro
+ *    block's box (w lanes 0), then each macro block's box quantised to 8
+ *    bytes), and
  * 3) lem_grid_fill writes them; the same workspace must be passed to both. */
 int64_t lem_grid_workspace(int64_t T, float cells_per_tri);
 int lem_grid_plan(const float *tris, int64_t T, float cells_per_tri, lem_grid_info *info,
@@ -182,8 +184,9 @@ int lem_grid_fill(const float *tris, int64_t T, const lem_grid_info *info, int32
  * each wavefront's 64 rays are spatially compact; results are written at the
  * original indices (hit_bits is cleared and bits are set atomically).  The
  * workspace (lem_visibility_grid_workspace(S, K, flags) bytes) holds the task
- * counter of the persistent traversal and the sort: calls that overlap on one
- * workspace race. */
+ * counters of the persistent traversal (one per XCD: each XCD walks a
+ * contiguous eighth of every row of rays, so its L2 keeps one region's cells)
+ * and the sort: calls that overlap on one workspace race. */
 int64_t lem_visibility_grid_workspace(int64_t S, int64_t K, int flags);
 int lem_visibility_grid(const float *src, int64_t S, const float *snk, int64_t K,
                         const lem_grid_info *info, const int32_t *cell_start, const float *recs,

@@ -321,8 +321,10 @@ __global__ __launch_bounds__(GB) void k_cell_boxes(const float *__restrict__ tri
             hi[a] = fminf(hi[a], ch[a]);
         }
     }
-    box[2 * c] = make_float4(lo[0], lo[1], lo[2], 0.0f);
-    box[2 * c + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+    // the cell's record range rides in the w lanes, so one 32-byte read gives
+    // the walk both the box and the range
+    box[2 * c] = make_float4(lo[0], lo[1], lo[2], __builtin_bit_cast(float, cell_start[c]));
+    box[2 * c + 1] = make_float4(hi[0], hi[1], hi[2], __builtin_bit_cast(float, cell_start[c + 1]));
 }
 
 // Box of each macro block: the union of its cells' boxes (inverted if empty),
@@ -720,11 +722,10 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
             int32_t b, e;
             {
                 const int c = (iz * G.ny + iy) * G.nx + ix;
-                // the range and the box depend on the cell alone: all four
-                // loads go out together
-                b = cell_start[c];
-                e = cell_start[c + 1];
+                // box and record range of the cell in one 32-byte read
                 const float4 clo = box[2 * c], chi = box[2 * c + 1];
+                b = __builtin_bit_cast(int32_t, clo.w);
+                e = __builtin_bit_cast(int32_t, chi.w);
                 if (n_cells)
                     *n_cells += 1;
                 if (e > b && !seg_box<PAR>(r, idx, idy, idz, clo, chi, -margin, dist + margin,
@@ -920,9 +921,10 @@ constexpr int LDS_WG_PER_CU = 5;  // 5 x 32 KB of the 160 KB LDS of a CU
 
 // Persistent form of k_visibility_grid: each workgroup copies the quantised
 // macro table (at most MACRO_LDS_MAX blocks) into LDS once; then each
-// wavefront repeatedly claims the next 64-ray word of output (one atomic on
-// *next_task), so waves never wait for one another and the load balances
-// whatever number of workgroups is resident.  Dead-block tests read LDS.
+// wavefront repeatedly claims a 64-ray word of output (one atomic on one of
+// the 8 counters next_task[0..7]), so waves never wait for one another and
+// the load balances whatever number of workgroups is resident.  Dead-block
+// tests read LDS.
 template <bool ANY, bool SINK_MAJOR, int SEM, bool REV = false>
 __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
     const float *__restrict__ src, int64_t S, const float *__restrict__ snk, int64_t K, GridDev G,
@@ -937,17 +939,29 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
     __syncthreads();
     const int64_t NI = SINK_MAJOR ? S : K, NO = SINK_MAJOR ? K : S;
     const int64_t W = (NI + 63) / 64;
-    const unsigned long long ntasks = (unsigned long long)(NO * W);
     const int lane = threadIdx.x & 63;
+    // Workgroup b runs on XCD b % 8, each XCD with its own L2.  XCD x owns
+    // words [x*CH, (x+1)*CH) of every outer row -- a contiguous eighth of the
+    // (spatially ordered) lane-side points -- and claims them with its own
+    // counter, so its L2 holds the cells of one region; an XCD whose share is
+    // done claims from the next counters.
+    const int64_t CH = (W + 7) / 8;
+    int owner = (int)(blockIdx.x & 7), tried = 0;
     for (;;) {
-        unsigned long long task = 0;
+        unsigned long long t = 0;
         if (lane == 0)
-            task = atomicAdd(next_task, 1ull);
-        task = __shfl(task, 0);
-        if (task >= ntasks)
-            break;
-        const int64_t outer = (int64_t)(task / W);
-        const int64_t word = (int64_t)(task % W);
+            t = atomicAdd(&next_task[owner], 1ull);
+        t = __shfl(t, 0);
+        const int64_t outer = (int64_t)(t / (unsigned long long)CH);
+        const int64_t word = owner * CH + (int64_t)(t % (unsigned long long)CH);
+        if (outer >= NO) {
+            if (++tried == 8)
+                break;
+            owner = (owner + 1) & 7;
+            continue;
+        }
+        if (word >= W)
+            continue;
         const int64_t slot = word * 64 + lane;
         const bool active = slot < NI;
         // with an order, lane slot -> the slot-th point of the spatial order
@@ -988,10 +1002,11 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
     }
 }
 
+// clears the 8 task counters of the persistent traversal
 __global__ void k_zero_u64(unsigned long long *__restrict__ p)
 {
-    if (threadIdx.x == 0)
-        p[0] = 0ull;
+    if (threadIdx.x < 8)
+        p[threadIdx.x] = 0ull;
 }
 
 __global__ __launch_bounds__(GB) void k_zero_words(unsigned long long *__restrict__ p, int64_t n)
