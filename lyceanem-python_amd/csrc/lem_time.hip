@@ -295,25 +295,38 @@ __global__ __launch_bounds__(TB) void k_trace_starts(const int32_t *__restrict__
     start[b] = (int32_t)lo;
 }
 
-// H[b] = sum of the amplitudes of the paths starting at sample b, in path order
+// H[b] = sum of the amplitudes of the paths starting at sample b.  One
+// wavefront per sample: lane l sums the sample's paths l, l + 64, ... in path
+// order, then the 64 partial sums meet in a fixed butterfly -- the order is
+// fixed, so the result is deterministic.
 __global__ __launch_bounds__(TB) void k_trace_hist(const double *__restrict__ amp,
                                                    const int32_t *__restrict__ val,
                                                    const int32_t *__restrict__ start, int64_t N,
                                                    double *__restrict__ H)
 {
-    const int64_t b = (int64_t)blockIdx.x * TB + threadIdx.x;
+    const int64_t b = (int64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (b >= N)
-        return;
+        return;  // wave-uniform
     double h0 = 0.0, h1 = 0.0, h2 = 0.0;
-    for (int32_t i = start[b]; i < start[b + 1]; ++i) {
+    const int32_t e = start[b + 1];
+    for (int32_t i = start[b] + lane; i < e; i += 64) {
         const int64_t p = val[i];
         h0 += amp[3 * p];
         h1 += amp[3 * p + 1];
         h2 += amp[3 * p + 2];
     }
-    H[3 * b] = h0;
-    H[3 * b + 1] = h1;
-    H[3 * b + 2] = h2;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        h0 += __shfl_xor(h0, off);
+        h1 += __shfl_xor(h1, off);
+        h2 += __shfl_xor(h2, off);
+    }
+    if (lane == 0) {
+        H[3 * b] = h0;
+        H[3 * b + 1] = h1;
+        H[3 * b + 2] = h2;
+    }
 }
 
 // out[n] = sum_{j <= min(E - 1, n)} exc[j] H[n - j]
@@ -517,8 +530,9 @@ extern "C" int lem_time_trace(const double *amp, const int64_t *bin, int64_t V,
     }
     hipLaunchKernelGGL(k_trace_starts, dim3((unsigned)ceil_div(N + 1, TB)), dim3(TB), 0, st,
                        L.k1, V, N, L.start);
-    hipLaunchKernelGGL(k_trace_hist, dim3((unsigned)ceil_div(N, TB)), dim3(TB), 0, st, amp, L.v1,
-                       L.start, N, L.H);
+    LEM_REQUIRE(ceil_div(N, TB / 64) <= 2147483647LL, "lem_time_trace: too many samples");
+    hipLaunchKernelGGL(k_trace_hist, dim3((unsigned)ceil_div(N, TB / 64)), dim3(TB), 0, st, amp,
+                       L.v1, L.start, N, L.H);
     hipLaunchKernelGGL(k_trace_conv, dim3((unsigned)ceil_div(N, TB)), dim3(TB), 0, st, L.H,
                        excitation, E, N, out);
     return check_launch("k_trace_conv");
