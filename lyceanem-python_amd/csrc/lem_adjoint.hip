@@ -61,12 +61,16 @@ LEM_DEV float2 cmulf(float ar, float ai, float br, float bi)
 }
 
 // Partials: block b covers sources [(b % nbs)*256, +256), sink chunk c = b / nbs.
+template <bool LOSSY>
 __global__ __launch_bounds__(AB) void k_adjoint_partial(
     const float *__restrict__ src, const float *__restrict__ src_w, int64_t S,
     const float *__restrict__ snk, const float *__restrict__ snk_n, const float *__restrict__ g,
     int64_t K, const unsigned long long *__restrict__ hit_bits, int64_t W64, float alpha,
     double beta, int64_t CH, int64_t nbs, float *__restrict__ ws_sum, float *__restrict__ ws_cmp)
 {
+    // a * b + c may be fused in the gradient arithmetic (the Kahan updates
+    // have no products, so they are not contracted)
+#pragma clang fp contract(fast)
     __shared__ SnkLds s_k[AK_TILE];
     const int64_t c = (int64_t)blockIdx.x / nbs;
     const int64_t s = ((int64_t)blockIdx.x % nbs) * AB + threadIdx.x;
@@ -105,6 +109,7 @@ __global__ __launch_bounds__(AB) void k_adjoint_partial(
         if (!active)
             continue;
         unsigned long long word = 0;
+#pragma unroll 2
         for (int j = 0; j < n; ++j) {
             const int64_t k = base + j;
             if (hit_bits) {
@@ -121,7 +126,7 @@ __global__ __launch_bounds__(AB) void k_adjoint_partial(
             const float cs = ux * L.nx + uy * L.ny + uz * L.nz;
             // G = e^{-alpha r}/r e^{-j beta r}
             float amp = ir;
-            if (alpha != 0.0f)
+            if (LOSSY)
                 amp *= expf(-alpha * (float)p.r);
             const float Gr = amp * p.c, Gi = amp * p.s;
             // F = FRONT (-g - 1/r) G ; F' = FRONT (g^2 + 2g/r + 2/r^2) G
@@ -285,7 +290,12 @@ extern "C" int lem_field_los_adjoint(const float *src, const float *src_w, int64
     hipStream_t st = as_stream(stream);
     hipLaunchKernelGGL(k_adjoint_prep, dim3((unsigned)ceil_div(K, AB)), dim3(AB), 0, st, grad_out,
                        snk_n, snk_w, (flags & LEM_FIELD_SINK_TANGENT) ? 1 : 0, K, g);
-    hipLaunchKernelGGL(k_adjoint_partial, dim3((unsigned)(NCe * nbs)), dim3(AB), 0, st, src, src_w,
+    if (alpha != 0.0)
+        hipLaunchKernelGGL(k_adjoint_partial<true>, dim3((unsigned)(NCe * nbs)), dim3(AB), 0, st, src, src_w,
+                       S, snk, snk_n, g, K, reinterpret_cast<const unsigned long long *>(hit_bits),
+                       ceil_div(K, 64), (float)alpha, beta, CH, nbs, ws_sum, ws_cmp);
+    else
+        hipLaunchKernelGGL(k_adjoint_partial<false>, dim3((unsigned)(NCe * nbs)), dim3(AB), 0, st, src, src_w,
                        S, snk, snk_n, g, K, reinterpret_cast<const unsigned long long *>(hit_bits),
                        ceil_div(K, 64), (float)alpha, beta, CH, nbs, ws_sum, ws_cmp);
     int rc = check_launch("k_adjoint_partial");

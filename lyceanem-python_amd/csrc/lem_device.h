@@ -225,20 +225,13 @@ LEM_DEV Phase phase_of(double dx, double dy, double dz, double beta)
     xr = __builtin_fma(-q, PIO2_LO, xr);
     float s, c;
     sincos_reduced((float)xr, s, c);
-    int qi = ((int)(long long)q) & 3;
-    float cs = c, sn = s;
-    if (qi == 1) {
-        cs = -s;
-        sn = c;
-    } else if (qi == 2) {
-        cs = -c;
-        sn = -s;
-    } else if (qi == 3) {
-        cs = s;
-        sn = -c;
-    }
-    p.c = cs;
-    p.s = sn;
+    // quadrant q mod 4, exact for |q| < 2^53 (q / 4 and its floor are exact)
+    const int qi = (int)__builtin_fma(-4.0, __builtin_floor(q * 0.25), q);
+    // (cos, sin) of x + qi pi/2: quadrants 0..3 give (c, s), (-s, c), (-c, -s),
+    // (s, -c) -- selected, not branched on
+    const float cs = (qi & 1) ? s : c, sn = (qi & 1) ? c : s;
+    p.c = ((qi + 1) & 2) ? -cs : cs;
+    p.s = (qi & 2) ? -sn : sn;
     return p;
 }
 
@@ -246,11 +239,15 @@ LEM_DEV Phase phase_of(double dx, double dy, double dz, double beta)
 // em_wave (em.cuh:64-120):
 //   loss = -(1/2pi) * (-(alpha + j beta) - 1/r) * e^{-(alpha + j beta) r}/r * cos
 // with cos = d . n_end (raw end normal, no clipping).  Returned as (re, im).
+// LOSSY = false: alpha is 0 (no attenuation term is evaluated).  The field
+// arithmetic may contract a * b + c into one fused operation.
+template <bool LOSSY = true>
 LEM_DEV void leg_factor(const Phase &p, float alpha, float beta_f, float cosang,
                         float &lr, float &li)
 {
+#pragma clang fp contract(fast)
     float amp = p.inv_r;
-    if (alpha != 0.0f)
+    if (LOSSY)
         amp *= expf(-alpha * (float)p.r);
     float Gr = amp * p.c, Gi = amp * p.s;
     float ar = -alpha - p.inv_r, ai = -beta_f;
@@ -267,6 +264,7 @@ LEM_DEV void leg_factor(const Phase &p, float alpha, float beta_f, float cosang,
 // (em.cuh:18-61), (E.u)u + (E.v)v with u, v, d orthonormal.
 LEM_DEV C3 project(const C3 &w, float dx, float dy, float dz)
 {
+#pragma clang fp contract(fast)
     float pr = w.xr * dx + w.yr * dy + w.zr * dz;
     float pi = w.xi * dx + w.yi * dy + w.zi * dz;
     C3 o;

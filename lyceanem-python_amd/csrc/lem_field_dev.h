@@ -26,6 +26,7 @@ LEM_DEV C3 load_w(const float *__restrict__ w, int64_t i)
 
 LEM_DEV C3 cmul_comp(const C3 &a, const C3 &b)
 {
+#pragma clang fp contract(fast)
     C3 o;
     o.xr = a.xr * b.xr - a.xi * b.xi; o.xi = a.xr * b.xi + a.xi * b.xr;
     o.yr = a.yr * b.yr - a.yi * b.yi; o.yi = a.yr * b.yi + a.yi * b.yr;
@@ -35,6 +36,7 @@ LEM_DEV C3 cmul_comp(const C3 &a, const C3 &b)
 
 LEM_DEV C3 scale(const C3 &a, float lr, float li)
 {
+#pragma clang fp contract(fast)
     C3 o;
     o.xr = a.xr * lr - a.xi * li; o.xi = a.xr * li + a.xi * lr;
     o.yr = a.yr * lr - a.yi * li; o.yi = a.yr * li + a.yi * lr;
@@ -51,15 +53,17 @@ LEM_DEV C3 project_normal(const C3 &e, float nx, float ny, float nz)
 }
 
 // Contribution of one source->sink leg before the sink weight / tangent step.
+template <bool LOSSY = true>
 LEM_DEV C3 los_term(double xs, double ys, double zs, const C3 &ws, double xk, double yk,
                     double zk, float nkx, float nky, float nkz, float alpha, double beta)
 {
+#pragma clang fp contract(fast)
     double dx = xk - xs, dy = yk - ys, dz = zk - zs;
     Phase p = phase_of(dx, dy, dz, beta);
     float ux = (float)dx * p.inv_r, uy = (float)dy * p.inv_r, uz = (float)dz * p.inv_r;
     float cosang = ux * nkx + uy * nky + uz * nkz;
     float lr, li;
-    leg_factor(p, alpha, (float)beta, cosang, lr, li);
+    leg_factor<LOSSY>(p, alpha, (float)beta, cosang, lr, li);
     C3 e = project(ws, ux, uy, uz);
     return scale(e, lr, li);
 }

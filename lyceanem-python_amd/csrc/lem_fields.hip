@@ -23,6 +23,7 @@ namespace lem {
 
 // Source-summed partials: block b covers sinks [(b % nbk)*256, +256) and
 // source chunk c = b / nbk ([c*CH, min(S, (c+1)*CH))).
+template <bool LOSSY>
 __global__ __launch_bounds__(FB) void k_field_los_partial(
     const float *__restrict__ src, const float *__restrict__ src_w, int64_t S,
     const float *__restrict__ snk, const float *__restrict__ snk_n, int64_t K,
@@ -30,14 +31,20 @@ __global__ __launch_bounds__(FB) void k_field_los_partial(
     double beta, int64_t CH, int64_t nbk, float *__restrict__ ws_sum, float *__restrict__ ws_cmp)
 {
     __shared__ SrcLds s_src[FS_TILE];
+    // source-major mask: the tile's words for this block's 256 sinks (4 per
+    // source), read from LDS in the loop instead of one global load per source
+    __shared__ unsigned long long s_mask[FS_TILE][FB / 64];
     const int64_t c = (int64_t)blockIdx.x / nbk;
-    const int64_t k = ((int64_t)blockIdx.x % nbk) * FB + threadIdx.x;
+    const int64_t k0 = ((int64_t)blockIdx.x % nbk) * FB;
+    const int64_t k = k0 + threadIdx.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const bool active = k < K;
     const int64_t kk = active ? k : 0;
     const double xk = snk[3 * kk], yk = snk[3 * kk + 1], zk = snk[3 * kk + 2];
     const float nkx = snk_n[3 * kk], nky = snk_n[3 * kk + 1], nkz = snk_n[3 * kk + 2];
     const int64_t s0 = c * CH;
     const int64_t s1 = (s0 + CH) < S ? (s0 + CH) : S;
+    const bool src_major_bits = hit_bits && !sink_major;
     KahanC3 acc;
     acc.zero();
     // sink-major mask: one word covers 64 consecutive sources of this lane's
@@ -47,20 +54,29 @@ __global__ __launch_bounds__(FB) void k_field_los_partial(
     for (int64_t base = s0; base < s1; base += FS_TILE) {
         __syncthreads();
         stage_sources(s_src, src, src_w, base, s1);
+        if (src_major_bits) {
+            const int64_t sj = base + threadIdx.x;
+#pragma unroll
+            for (int i = 0; i < FB / 64; ++i) {
+                const int64_t wi = (k0 >> 6) + i;
+                s_mask[threadIdx.x][i] = (sj < s1 && wi < W64) ? hit_bits[sj * W64 + wi] : 0ull;
+            }
+        }
         __syncthreads();
         int n = (int)((s1 - base) < FS_TILE ? (s1 - base) : FS_TILE);
         if (active) {
+#pragma unroll 2
             for (int j = 0; j < n; ++j) {
                 const int64_t s = base + j;
-                bool bl;
+                bool bl = false;
                 if (sink_major) {
                     if ((s >> 6) != word_at) {
                         word_at = s >> 6;
                         word = hit_bits ? hit_bits[k * W64 + word_at] : 0ull;
                     }
                     bl = (word >> (s & 63)) & 1ull;
-                } else {
-                    bl = blocked(hit_bits, W64, s, k);
+                } else if (src_major_bits) {
+                    bl = (s_mask[j][wave] >> lane) & 1ull;
                 }
                 if (bl)
                     continue;
@@ -68,7 +84,7 @@ __global__ __launch_bounds__(FB) void k_field_los_partial(
                 C3 w;
                 w.xr = L.w[0]; w.xi = L.w[1]; w.yr = L.w[2]; w.yi = L.w[3]; w.zr = L.w[4];
                 w.zi = L.w[5];
-                C3 t = los_term(L.x, L.y, L.z, w, xk, yk, zk, nkx, nky, nkz, alpha, beta);
+                C3 t = los_term<LOSSY>(L.x, L.y, L.z, w, xk, yk, zk, nkx, nky, nkz, alpha, beta);
                 float v[6] = {t.xr, t.xi, t.yr, t.yi, t.zr, t.zi};
                 acc.add(v);
             }
@@ -114,6 +130,7 @@ __global__ __launch_bounds__(FB) void k_field_los_finish(
 
 // Per-element network (calculate_scattering(elements=True)): out is
 // S x K x 3 complex64; block b covers source s = b / nbk, 256 sinks.
+template <bool LOSSY>
 __global__ __launch_bounds__(FB) void k_field_los_elements(
     const float *__restrict__ src, const float *__restrict__ src_w, const float *__restrict__ snk,
     const float *__restrict__ snk_n, const float *__restrict__ snk_w, int64_t K,
@@ -129,7 +146,7 @@ __global__ __launch_bounds__(FB) void k_field_los_elements(
         e.xr = e.xi = e.yr = e.yi = e.zr = e.zi = 0.0f;
     } else {
         const float nkx = snk_n[3 * k], nky = snk_n[3 * k + 1], nkz = snk_n[3 * k + 2];
-        e = los_term(src[3 * s], src[3 * s + 1], src[3 * s + 2], load_w(src_w, s), snk[3 * k],
+        e = los_term<LOSSY>(src[3 * s], src[3 * s + 1], src[3 * s + 2], load_w(src_w, s), snk[3 * k],
                      snk[3 * k + 1], snk[3 * k + 2], nkx, nky, nkz, alpha, beta);
         if (snk_w)
             e = cmul_comp(e, load_w(snk_w, k));
@@ -145,6 +162,7 @@ __global__ __launch_bounds__(FB) void k_field_los_elements(
 // Generic path list (EMGPUFreqDomain's input): one thread per (source, sink)
 // slot, summing that slot's paths in index order.  index rows are 1-based,
 // 0 = absent point; `order` lists path rows grouped by slot.
+template <bool LOSSY>
 __global__ __launch_bounds__(FB) void k_field_paths(
     const float *__restrict__ pos, const float *__restrict__ nrm, const float *__restrict__ w,
     int64_t N, const int32_t *__restrict__ index, int ncol, const int64_t *__restrict__ order,
@@ -185,7 +203,7 @@ __global__ __launch_bounds__(FB) void k_field_paths(
                       uz = (float)dz * p.inv_r;
                 float cosang = ux * nrm[3 * bb] + uy * nrm[3 * bb + 1] + uz * nrm[3 * bb + 2];
                 float l2r, l2i;
-                leg_factor(p, alpha, (float)beta, cosang, l2r, l2i);
+                leg_factor<LOSSY>(p, alpha, (float)beta, cosang, l2r, l2i);
                 float nr = lr * l2r - li * l2i;
                 float ni = lr * l2i + li * l2r;
                 lr = nr;
@@ -304,9 +322,14 @@ extern "C" int lem_field_los(const float *src, const float *src_w, int64_t S, co
         if (S == 0)
             return OK;
         LEM_REQUIRE(S * nbk <= 2147483647LL, "lem_field_los: S*K too large for one launch");
-        hipLaunchKernelGGL(k_field_los_elements, dim3((unsigned)(S * nbk)), dim3(FB), 0, st, src,
-                           src_w, snk, snk_n, snk_w, K, hb, w64, sm, (float)alpha, beta, tangent,
-                           nbk, out);
+        if (alpha != 0.0)
+            hipLaunchKernelGGL(k_field_los_elements<true>, dim3((unsigned)(S * nbk)), dim3(FB), 0,
+                               st, src, src_w, snk, snk_n, snk_w, K, hb, w64, sm, (float)alpha,
+                               beta, tangent, nbk, out);
+        else
+            hipLaunchKernelGGL(k_field_los_elements<false>, dim3((unsigned)(S * nbk)), dim3(FB), 0,
+                               st, src, src_w, snk, snk_n, snk_w, K, hb, w64, sm, (float)alpha,
+                               beta, tangent, nbk, out);
         return check_launch("k_field_los_elements");
     }
     if (S == 0) {
@@ -324,9 +347,14 @@ extern "C" int lem_field_los(const float *src, const float *src_w, int64_t S, co
     float *ws_sum = reinterpret_cast<float *>(workspace);
     float *ws_cmp = ws_sum + NC * 6 * K;
     LEM_REQUIRE(NC * nbk <= 2147483647LL, "lem_field_los: grid too large");
-    hipLaunchKernelGGL(k_field_los_partial, dim3((unsigned)(NC * nbk)), dim3(FB), 0, st, src, src_w,
-                       S, snk, snk_n, K, hb, w64, sm, (float)alpha, beta, CH, nbk, ws_sum,
-                       ws_cmp);
+    if (alpha != 0.0)
+        hipLaunchKernelGGL(k_field_los_partial<true>, dim3((unsigned)(NC * nbk)), dim3(FB), 0, st,
+                           src, src_w, S, snk, snk_n, K, hb, w64, sm, (float)alpha, beta, CH, nbk,
+                           ws_sum, ws_cmp);
+    else
+        hipLaunchKernelGGL(k_field_los_partial<false>, dim3((unsigned)(NC * nbk)), dim3(FB), 0, st,
+                           src, src_w, S, snk, snk_n, K, hb, w64, sm, (float)alpha, beta, CH, nbk,
+                           ws_sum, ws_cmp);
     int rc = check_launch("k_field_los_partial");
     if (rc)
         return rc;
@@ -349,7 +377,12 @@ extern "C" int lem_field_paths(const float *pos, const float *nrm, const float *
     LEM_REQUIRE(pos && nrm && w && index && order && seg_start && seg_slot && out && err_flag,
                 "lem_field_paths: null pointer");
     LEM_REQUIRE(ceil_div(nseg, FB) <= 2147483647LL, "lem_field_paths: too many segments");
-    hipLaunchKernelGGL(k_field_paths, dim3((unsigned)ceil_div(nseg, FB)), dim3(FB), 0,
+    if (alpha != 0.0)
+        hipLaunchKernelGGL(k_field_paths<true>, dim3((unsigned)ceil_div(nseg, FB)), dim3(FB), 0,
+                       as_stream(stream), pos, nrm, w, N, index, ncol, order, seg_start, seg_slot,
+                       nseg, (float)alpha, beta, out, err_flag);
+    else
+        hipLaunchKernelGGL(k_field_paths<false>, dim3((unsigned)ceil_div(nseg, FB)), dim3(FB), 0,
                        as_stream(stream), pos, nrm, w, N, index, ncol, order, seg_start, seg_slot,
                        nseg, (float)alpha, beta, out, err_flag);
     return check_launch("k_field_paths");

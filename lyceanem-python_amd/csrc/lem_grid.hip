@@ -1184,6 +1184,7 @@ __global__ __launch_bounds__(GB) void k_grid_stats(
 // cast any-hit through the grid and, when unblocked, its Green's-function term
 // is accumulated -- the mask is never stored.  Same partial layout as
 // k_field_los_partial (lem_fields.hip); merged by the same finish kernel.
+template <bool LOSSY>
 __global__ __launch_bounds__(FB) void k_field_los_grid_partial(
     const float *__restrict__ src, const float *__restrict__ src_w, int64_t S,
     const float *__restrict__ snk, const float *__restrict__ snk_n, int64_t K, GridDev G,
@@ -1220,7 +1221,7 @@ __global__ __launch_bounds__(FB) void k_field_los_grid_partial(
                 C3 w;
                 w.xr = L.w[0]; w.xi = L.w[1]; w.yr = L.w[2]; w.yi = L.w[3]; w.zr = L.w[4];
                 w.zi = L.w[5];
-                C3 t = los_term(L.x, L.y, L.z, w, xk, yk, zk, nkx, nky, nkz, alpha, beta);
+                C3 t = los_term<LOSSY>(L.x, L.y, L.z, w, xk, yk, zk, nkx, nky, nkz, alpha, beta);
                 float v[6] = {t.xr, t.xi, t.yr, t.yi, t.zr, t.zi};
                 acc.add(v);
             }
@@ -1689,7 +1690,14 @@ extern "C" int lem_field_los_grid(const float *src, const float *src_w, int64_t 
             return rc;
         order = L.order;
     }
-    hipLaunchKernelGGL(k_field_los_grid_partial, dim3((unsigned)(NC * nbk)), dim3(FB), 0, st, src,
+    if (alpha != 0.0)
+        hipLaunchKernelGGL(k_field_los_grid_partial<true>, dim3((unsigned)(NC * nbk)), dim3(FB), 0, st, src,
+                       src_w, S, snk, snk_n, K, to_dev(*info), cell_start,
+                       reinterpret_cast<const float4 *>(recs),
+                       reinterpret_cast<const float4 *>(cell_box), order, (float)alpha, beta, CH,
+                       nbk, ws_sum, ws_cmp);
+    else
+        hipLaunchKernelGGL(k_field_los_grid_partial<false>, dim3((unsigned)(NC * nbk)), dim3(FB), 0, st, src,
                        src_w, S, snk, snk_n, K, to_dev(*info), cell_start,
                        reinterpret_cast<const float4 *>(recs),
                        reinterpret_cast<const float4 *>(cell_box), order, (float)alpha, beta, CH,

@@ -29,6 +29,7 @@ constexpr int GW_M = 32;   // sinks per workgroup
 constexpr int GW_N = 64;   // weight sets (columns)
 constexpr int GW_KC = 32;  // sources per chunk
 
+template <bool LOSSY>
 __global__ __launch_bounds__(256) void k_gw_mfma(
     const float *__restrict__ src, int64_t S, const float *__restrict__ snk,
     const float *__restrict__ snk_n, int64_t K, float ex, float ey, float ez,
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(256) void k_gw_mfma(
             const float ux = (float)dx * ir, uy = (float)dy * ir, uz = (float)dz * ir;
             const float cs = ux * nx + uy * ny + uz * nz;
             float amp = ir;
-            if (alpha != 0.0f)
+            if (LOSSY)
                 amp *= expf(-alpha * (float)p.r);
             const float Gr = amp * p.c, Gi = amp * p.s;
             const float ar = -alpha - ir, ai = -kw_f;
@@ -149,7 +150,12 @@ extern "C" int lem_gw_mfma(const float *src, int64_t S, float pol_x, float pol_y
     LEM_REQUIRE(K % GW_M == 0, "lem_gw_mfma: K must be a multiple of 32 (pad the sinks)");
     LEM_REQUIRE(F <= 65535 && K / GW_M <= 2147483647LL, "lem_gw_mfma: grid too large");
     dim3 grid((unsigned)(K / GW_M), (unsigned)F);
-    hipLaunchKernelGGL(k_gw_mfma, grid, dim3(256), 0, as_stream(stream), src, S, snk, snk_n, K,
+    if (alpha != 0.0)
+        hipLaunchKernelGGL(k_gw_mfma<true>, grid, dim3(256), 0, as_stream(stream), src, S, snk, snk_n, K,
+                       pol_x, pol_y, pol_z, wavenumbers, (float)alpha,
+                       reinterpret_cast<const float2 *>(W), out);
+    else
+        hipLaunchKernelGGL(k_gw_mfma<false>, grid, dim3(256), 0, as_stream(stream), src, S, snk, snk_n, K,
                        pol_x, pol_y, pol_z, wavenumbers, (float)alpha,
                        reinterpret_cast<const float2 *>(W), out);
     return check_launch("k_gw_mfma");
