@@ -273,7 +273,7 @@ class C2:
 class C3:
     name = "c3"
 
-    def __init__(self, torch, from_sink=True):
+    def __init__(self, torch, from_sink=True, cells_per_tri=2.0):
         from lyceanem import _native, scenes
 
         self.torch, self.nat = torch, _native
@@ -284,7 +284,7 @@ class C3:
         t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev)
              for k, v in sc.items() if isinstance(v, np.ndarray)}
         self.t = t
-        self.grid = _native.Grid(t["tris"])
+        self.grid = _native.Grid(t["tris"], cells_per_tri=cells_per_tri)
         self.S, self.P, self.K = len(sc["src"]), len(sc["scat"]), len(sc["snk"])
         self.T = len(sc["tris"])
 
@@ -788,7 +788,7 @@ def main(argv=None) -> int:
     if a.config == "c2":
         cfg = C2(torch, a.c2_sinks, a.cells_per_tri, a.input_order)
     elif a.config == "c3":
-        cfg = C3(torch, from_sink=not a.c3_walk_from_source)
+        cfg = C3(torch, from_sink=not a.c3_walk_from_source, cells_per_tri=a.cells_per_tri)
     else:
         cfg = CONFIGS[a.config](torch)
     print(f"[bench] config {a.config}: setup done", file=sys.stderr, flush=True)

@@ -918,6 +918,10 @@ __global__ __launch_bounds__(GB) void k_visibility_grid(
 }
 
 constexpr int LDS_WG_PER_CU = 5;  // 5 x 32 KB of the 160 KB LDS of a CU
+// The 8 task counters sit 4 KB apart: device-scope atomics on one cache line
+// serialise where the line lives, which capped claiming near 82 M per second.
+constexpr int COUNTER_STRIDE = 512;  // uint64 elements
+constexpr int CLAIM_WORDS = 8;       // 64-ray words per claim
 
 // Persistent form of k_visibility_grid: each workgroup copies the quantised
 // macro table (at most MACRO_LDS_MAX blocks) into LDS once; then each
@@ -946,59 +950,63 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
     // counter, so its L2 holds the cells of one region; an XCD whose share is
     // done claims from the next counters.
     const int64_t CH = (W + 7) / 8;
+    // a claim takes CLAIM_WORDS consecutive words of the owner's eighth
+    const int64_t UPR = (CH + CLAIM_WORDS - 1) / CLAIM_WORDS;
     int owner = (int)(blockIdx.x & 7), tried = 0;
     for (;;) {
         unsigned long long t = 0;
         if (lane == 0)
-            t = atomicAdd(&next_task[owner], 1ull);
+            t = atomicAdd(&next_task[owner * COUNTER_STRIDE], 1ull);
         t = __shfl(t, 0);
-        const int64_t outer = (int64_t)(t / (unsigned long long)CH);
-        const int64_t word = owner * CH + (int64_t)(t % (unsigned long long)CH);
+        const int64_t outer = (int64_t)(t / (unsigned long long)UPR);
         if (outer >= NO) {
             if (++tried == 8)
                 break;
             owner = (owner + 1) & 7;
             continue;
         }
-        if (word >= W)
-            continue;
-        const int64_t slot = word * 64 + lane;
-        const bool active = slot < NI;
-        // with an order, lane slot -> the slot-th point of the spatial order
-        const int64_t inner = (order && active) ? (int64_t)order[slot] : slot;
-        const int64_t s = SINK_MAJOR ? (active ? inner : 0) : outer;
-        const int64_t k = SINK_MAJOR ? outer : (active ? inner : 0);
-        Ray r = charge(src[3 * s], src[3 * s + 1], src[3 * s + 2], snk[3 * k], snk[3 * k + 1],
-                       snk[3 * k + 2]);
-        float dist = r.len;
-        bool hit = false;
-        if (active && !(skip_self && s == k)) {
-            // REV: walk from the sink end, test with the charged ray r
-            const Ray rw = REV ? charge(snk[3 * k], snk[3 * k + 1], snk[3 * k + 2], src[3 * s],
-                                        src[3 * s + 1], src[3 * s + 2])
-                               : r;
-            float dw = rw.len;
-            float &dd = REV ? dw : dist;
-            const Ray *rt = REV ? &r : nullptr;
-            // rays along a grid plane take the general slab test
-            if (rw.dx == 0.0f || rw.dy == 0.0f || rw.dz == 0.0f)
-                hit = dda_cast<ANY, SEM, true, true>(rw, dd, G, cell_start, recs, box, nullptr,
-                                                     nullptr, nullptr, smq, rt);
-            else
-                hit = dda_cast<ANY, SEM, true, false>(rw, dd, G, cell_start, recs, box, nullptr,
-                                                      nullptr, nullptr, smq, rt);
+        const int64_t w0 = owner * CH + (int64_t)(t % (unsigned long long)UPR) * CLAIM_WORDS;
+        const int64_t end = (int64_t)(owner + 1) * CH < W ? (int64_t)(owner + 1) * CH : W;
+        const int64_t w1 = w0 + CLAIM_WORDS < end ? w0 + CLAIM_WORDS : end;
+        for (int64_t word = w0; word < w1; ++word) {
+            const int64_t slot = word * 64 + lane;
+            const bool active = slot < NI;
+            // with an order, lane slot -> the slot-th point of the spatial order
+            const int64_t inner = (order && active) ? (int64_t)order[slot] : slot;
+            const int64_t s = SINK_MAJOR ? (active ? inner : 0) : outer;
+            const int64_t k = SINK_MAJOR ? outer : (active ? inner : 0);
+            Ray r = charge(src[3 * s], src[3 * s + 1], src[3 * s + 2], snk[3 * k],
+                           snk[3 * k + 1], snk[3 * k + 2]);
+            float dist = r.len;
+            bool hit = false;
+            if (active && !(skip_self && s == k)) {
+                // REV: walk from the sink end, test with the charged ray r
+                const Ray rw = REV ? charge(snk[3 * k], snk[3 * k + 1], snk[3 * k + 2],
+                                            src[3 * s], src[3 * s + 1], src[3 * s + 2])
+                                   : r;
+                float dw = rw.len;
+                float &dd = REV ? dw : dist;
+                const Ray *rt = REV ? &r : nullptr;
+                // rays along a grid plane take the general slab test
+                if (rw.dx == 0.0f || rw.dy == 0.0f || rw.dz == 0.0f)
+                    hit = dda_cast<ANY, SEM, true, true>(rw, dd, G, cell_start, recs, box,
+                                                         nullptr, nullptr, nullptr, smq, rt);
+                else
+                    hit = dda_cast<ANY, SEM, true, false>(rw, dd, G, cell_start, recs, box,
+                                                          nullptr, nullptr, nullptr, smq, rt);
+            }
+            if (order) {
+                // the lanes' bits belong to several words (zeroed beforehand)
+                if (active && hit)
+                    atomicOr(&bits[outer * W + (inner >> 6)], 1ull << (inner & 63));
+            } else {
+                const unsigned long long bw = __ballot(active && hit);
+                if (lane == 0)
+                    bits[outer * W + word] = bw;
+            }
+            if (dist_out && active)
+                dist_out[outer * NI + inner] = dist;
         }
-        if (order) {
-            // the lanes' bits belong to several words (zeroed beforehand)
-            if (active && hit)
-                atomicOr(&bits[outer * W + (inner >> 6)], 1ull << (inner & 63));
-        } else {
-            const unsigned long long bw = __ballot(active && hit);
-            if (lane == 0)
-                bits[outer * W + word] = bw;
-        }
-        if (dist_out && active)
-            dist_out[outer * NI + inner] = dist;
     }
 }
 
@@ -1006,7 +1014,7 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
 __global__ void k_zero_u64(unsigned long long *__restrict__ p)
 {
     if (threadIdx.x < 8)
-        p[threadIdx.x] = 0ull;
+        p[threadIdx.x * COUNTER_STRIDE] = 0ull;
 }
 
 __global__ __launch_bounds__(GB) void k_zero_words(unsigned long long *__restrict__ p, int64_t n)
@@ -1077,6 +1085,8 @@ struct VisWs {
 };
 
 constexpr int MORTON_BITS = 30;
+// workspace head: the 8 task counters (4 KB apart), then the point bounds
+constexpr int64_t VIS_HEAD = 8 * 8 * COUNTER_STRIDE + 256;
 
 inline int64_t vis_sort_bytes(int64_t n)
 {
@@ -1092,7 +1102,7 @@ inline int64_t vis_sort_bytes(int64_t n)
 inline int64_t vis_layout(int64_t n, size_t cub_bytes, char *base, VisWs *L)
 {
     auto al = [](int64_t b) { return ((b + 255) / 256) * 256; };
-    int64_t off = 256;
+    int64_t off = VIS_HEAD;
     char *p[4];
     for (int i = 0; i < 4; ++i) {
         p[i] = base ? base + off : nullptr;
@@ -1100,7 +1110,7 @@ inline int64_t vis_layout(int64_t n, size_t cub_bytes, char *base, VisWs *L)
     }
     if (L) {
         L->counter = reinterpret_cast<unsigned long long *>(base);
-        L->bounds = reinterpret_cast<unsigned int *>(base + 64);
+        L->bounds = reinterpret_cast<unsigned int *>(base + 8 * 8 * COUNTER_STRIDE);
         L->k0 = reinterpret_cast<uint32_t *>(p[0]);
         L->k1 = reinterpret_cast<uint32_t *>(p[1]);
         L->v0 = reinterpret_cast<int32_t *>(p[2]);
@@ -1466,7 +1476,7 @@ extern "C" int64_t lem_visibility_grid_workspace(int64_t S, int64_t K, int flags
         return -1;
     const int64_t NI = (flags & LEM_VIS_SINK_MAJOR) ? S : K;
     if ((flags & LEM_VIS_INPUT_ORDER) || NI == 0)
-        return 256;
+        return VIS_HEAD;
     if (NI > 2147483647LL)
         return -1;
     const int64_t cub = vis_sort_bytes(NI);
