@@ -42,7 +42,7 @@ constexpr int GB = 256;
 #ifndef LEM_GRID_MSHIFT
 #define LEM_GRID_MSHIFT 2
 #endif
-constexpr int MACRO_LDS_MAX = 4096;  // blocks (32 KB of LDS)
+constexpr int MACRO_LDS_MAX = 3264;  // blocks (25.5 KB of LDS: 6 workgroups per CU)
 
 // Order-preserving float <-> uint key (for atomic min / max of floats).
 LEM_DEV unsigned int fkey(float f)
@@ -917,7 +917,7 @@ __global__ __launch_bounds__(GB) void k_visibility_grid(
         dist_out[outer * NI + inner] = dist;
 }
 
-constexpr int LDS_WG_PER_CU = 5;  // 5 x 32 KB of the 160 KB LDS of a CU
+constexpr int LDS_WG_PER_CU = 6;  // 6 x 25.5 KB of the 160 KB LDS of a CU
 // The 8 task counters sit 4 KB apart: device-scope atomics on one cache line
 // serialise where the line lives, which capped claiming near 82 M per second.
 constexpr int COUNTER_STRIDE = 512;  // uint64 elements
