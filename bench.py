@@ -305,9 +305,11 @@ class C3:
         H = nat.field_los(t["src"], t["w"], t["scat"], t["scat_n"], hit_bits=bits,
                           snk_w=t["scat_w"], beta=b, sink_tangent=True, bits_sink_major=True)
         del bits
-        E = nat.field_los_grid(g, t["scat"], H, t["snk"], t["snk_n"], beta=b)
-        E += nat.field_los_grid(g, t["src"], t["w"], t["snk"], t["snk_n"], beta=b,
-                                sink_tangent=True)
+        # leg 2 and line of sight through the same persistent traversal
+        E = nat.field_los_grid_masked(g, t["scat"], H, t["snk"], t["snk_n"], beta=b,
+                                      sink_major=False)
+        E += nat.field_los_grid_masked(g, t["src"], t["w"], t["snk"], t["snk_n"], beta=b,
+                                       sink_tangent=True, sink_major=True)
         if ev is not None:
             ev[2].record()
         self.E = E

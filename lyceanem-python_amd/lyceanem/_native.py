@@ -540,18 +540,20 @@ def field_los_grid(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, beta=2 *
 MASK_CHUNK_BYTES = 1 << 31
 
 
-def field_los_grid_sink_major(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0,
-                              beta=2 * np.pi, sink_tangent=False, chunk_bytes=MASK_CHUNK_BYTES,
-                              from_sink=True):
-    """field_los_grid's result, computed as a sink-major any-hit mask from the
-    persistent grid traversal (each wave casts 64 consecutive sources to one
-    sink) followed by the masked field kernel.  This suits sources that are
-    close together (array elements) and sinks that are not (scatterers spread
-    over a scene), where field_los_grid's waves -- one source to 64 sinks --
-    diverge.  from_sink: walk each ray from its sink end, which reaches first
-    the blockers that sit next to sinks on surfaces (a scatterer behind its own
-    face); the mask is the same either way.  The sinks are processed in chunks
-    whose mask stays within chunk_bytes.  Returns (K,3) complex64."""
+def field_los_grid_masked(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, beta=2 * np.pi,
+                          sink_tangent=False, sink_major=True, from_sink=False,
+                          chunk_bytes=MASK_CHUNK_BYTES):
+    """field_los_grid's result, computed as an any-hit mask from the persistent
+    grid traversal followed by the masked field kernel.
+
+    sink_major=True: each wave casts 64 neighbouring sources to one sink -- for
+    compact sources (array elements) and spread-out sinks (scatterers over a
+    scene).  sink_major=False: each wave casts one source to 64 neighbouring
+    sinks -- for spread-out sources and compact sinks.  from_sink: walk each
+    ray from its sink end, which reaches first the blockers that sit next to
+    sinks on surfaces (a scatterer behind its own face); the mask is the same
+    either way.  The mask is built in chunks (of sinks, or of sources) that stay
+    within chunk_bytes.  Returns (K,3) complex64."""
     torch = _torch()
     src = to_dev(src, torch.float32).reshape(-1, 3)
     snk = to_dev(snk, torch.float32).reshape(-1, 3)
@@ -563,16 +565,34 @@ def field_los_grid_sink_major(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.
     src_w = to_dev(src_w, torch.complex64).reshape(S, 3)
     if snk_w is not None:
         snk_w = to_dev(snk_w, torch.complex64).reshape(K, 3)
-    row_bytes = 8 * ((S + 63) // 64)
-    kc = max(1, int(chunk_bytes) // row_bytes)
-    for k0 in range(0, K, kc):
-        k1 = min(K, k0 + kc)
-        bits, _ = grid.visibility(src, snk[k0:k1], any_hit=True, sink_major=True,
-                                  from_sink=from_sink)
-        out[k0:k1] = field_los(src, src_w, snk[k0:k1], snk_n[k0:k1], hit_bits=bits,
-                               snk_w=None if snk_w is None else snk_w[k0:k1], alpha=alpha,
-                               beta=beta, sink_tangent=sink_tangent, bits_sink_major=True)
+    if sink_major:
+        kc = max(1, int(chunk_bytes) // (8 * ((S + 63) // 64)))
+        for k0 in range(0, K, kc):
+            k1 = min(K, k0 + kc)
+            bits, _ = grid.visibility(src, snk[k0:k1], any_hit=True, sink_major=True,
+                                      from_sink=from_sink)
+            out[k0:k1] = field_los(src, src_w, snk[k0:k1], snk_n[k0:k1], hit_bits=bits,
+                                   snk_w=None if snk_w is None else snk_w[k0:k1], alpha=alpha,
+                                   beta=beta, sink_tangent=sink_tangent, bits_sink_major=True)
+        return out
+    sc = max(1, int(chunk_bytes) // (8 * ((K + 63) // 64)))
+    for s0 in range(0, S, sc):
+        s1 = min(S, s0 + sc)
+        bits, _ = grid.visibility(src[s0:s1], snk, any_hit=True, from_sink=from_sink)
+        part = field_los(src[s0:s1], src_w[s0:s1], snk, snk_n, hit_bits=bits, snk_w=snk_w,
+                         alpha=alpha, beta=beta, sink_tangent=sink_tangent)
+        out = part if s0 == 0 else out + part
     return out
+
+
+def field_los_grid_sink_major(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0,
+                              beta=2 * np.pi, sink_tangent=False, chunk_bytes=MASK_CHUNK_BYTES,
+                              from_sink=True):
+    """field_los_grid_masked with a sink-major mask walked from the sink end
+    (the one-bounce leg from an aperture to scatterers)."""
+    return field_los_grid_masked(grid, src, src_w, snk, snk_n, snk_w=snk_w, alpha=alpha,
+                                 beta=beta, sink_tangent=sink_tangent, sink_major=True,
+                                 from_sink=from_sink, chunk_bytes=chunk_bytes)
 
 
 def grid_stats(grid, src, snk, any_hit=False):

@@ -176,10 +176,13 @@ def one_bounce_fields(src, src_w, snk, snk_n, scat, scat_n, tris9, scat_w=None, 
     if not bool(H.abs().sum().item() > 0):
         return np.zeros((K, 3), np.complex128)
     if grid is not None:
-        E = _native.field_los_grid(grid, scat, H, snk, snk_n, alpha=alpha, beta=beta)
+        # leg 2: one scatterer to 64 neighbouring sinks per wave; line of sight:
+        # 64 neighbouring aperture points to one sink per wave
+        E = _native.field_los_grid_masked(grid, scat, H, snk, snk_n, alpha=alpha, beta=beta,
+                                          sink_major=False)
         if los:
-            E = E + _native.field_los_grid(grid, src, src_w, snk, snk_n, alpha=alpha, beta=beta,
-                                           sink_tangent=True)
+            E = E + _native.field_los_grid_masked(grid, src, src_w, snk, snk_n, alpha=alpha,
+                                                  beta=beta, sink_tangent=True, sink_major=True)
     else:
         bits = _visibility_bits(scat, snk, tris9, None)
         E = _native.field_los(scat, H, snk, snk_n, hit_bits=bits, alpha=alpha, beta=beta)

@@ -235,6 +235,18 @@ def test_sink_major_mask_field_matches_fused(native):
                                                  sink_tangent=tangent,
                                                  chunk_bytes=cb).cpu().numpy()
             assert np.array_equal(b, ref)
+    # source-major mask from the persistent traversal: whole (identical sums)
+    # and in source chunks (chunk sums added: float32 rounding apart)
+    for tangent in (False, True):
+        ref = native.field_los_grid(g, src, w, snk, nrm, snk_w=wk, beta=beta,
+                                    sink_tangent=tangent).cpu().numpy()
+        a = native.field_los_grid_masked(g, src, w, snk, nrm, snk_w=wk, beta=beta,
+                                         sink_tangent=tangent, sink_major=False).cpu().numpy()
+        assert np.array_equal(a, ref)
+        b = native.field_los_grid_masked(g, src, w, snk, nrm, snk_w=wk, beta=beta,
+                                         sink_tangent=tangent, sink_major=False,
+                                         chunk_bytes=32 * 40).cpu().numpy()
+        np.testing.assert_allclose(b, ref, rtol=1e-5, atol=1e-6 * np.abs(ref).max())
     # per-element network from either layout
     e1 = native.field_los(src, w, snk, nrm, hit_bits=bits, snk_w=wk, beta=beta,
                           elements=True).cpu().numpy()
