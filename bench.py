@@ -1,7 +1,8 @@
 #!/usr/bin/env python
 """Benchmark of the LyceanEM raycast + frequency-domain hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c3|c4|c5|td|bf]
+                    [--no-cpu]
 
 One "step" is one pass of the hot path over the configuration's synthetic scene
 with every input already resident in HBM:
@@ -12,6 +13,13 @@ with every input already resident in HBM:
   c1  32x32 lambda/2 array (3 GHz), 2,000 random triangles, 91 x 181 far-field
       sinks: brute-force visibility with kernel1Dv3 semantics (closest hit,
       every ray tests every triangle) + the fused masked field (source sum).
+  c3  one bounce + line of sight: 256 x 256 array, 50k triangles, 200k
+      scatterers, 64 x 64 sinks (grid masks and masked fields, leg by leg).
+  c4  64 x 64 conformal array: masked far field forward and its adjoint.
+  c5  per-element network times 64 excitation sets on FP32 MFMA.
+  td  time domain: 16.8M line-of-sight paths placed into a 65,536-sample trace.
+  bf  MaximumDirectivityMap: every grid point as a command angle, three weight
+      schemes, beamformed power on FP32 MFMA.
 
 The dominant kernel's average duration comes from HIP events recorded on the
 stream the kernels run on (torch's current stream).  The CPU baseline is the

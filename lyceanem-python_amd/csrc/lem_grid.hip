@@ -42,7 +42,16 @@ constexpr int GB = 256;
 #ifndef LEM_GRID_MSHIFT
 #define LEM_GRID_MSHIFT 2
 #endif
-constexpr int MACRO_LDS_MAX = 3264;  // blocks (25.5 KB of LDS: 6 workgroups per CU)
+constexpr int MACRO_LDS_MAX = 4352;  // blocks (6 B each, 25.5 KB of LDS: 6 workgroups per CU)
+
+// The quantised macro table as the walk reads it from LDS: per block the
+// 32-bit word (lo.x, lo.y, lo.z, hi.x) and the 16-bit word (hi.y, hi.z) of
+// the 8-byte table entry, kept in two arrays so a block takes 6 bytes.
+struct MacroTab {
+    const uint32_t *a;
+    const uint16_t *b;
+    LEM_DEV uint2 operator[](int m) const { return make_uint2(a[m], (uint32_t)b[m]); }
+};
 
 // Order-preserving float <-> uint key (for atomic min / max of floats).
 LEM_DEV unsigned int fkey(float f)
@@ -592,7 +601,7 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
                       const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
                       const float4 *__restrict__ box, unsigned int *n_tests = nullptr,
                       unsigned int *n_cells = nullptr, unsigned int *n_macro = nullptr,
-                      const uint2 *smq = nullptr, const Ray *rt = nullptr)
+                      MacroTab smq = MacroTab{nullptr, nullptr}, const Ray *rt = nullptr)
 {
     const Ray &rq = rt ? *rt : r;  // the predicate's ray
     Walk w;
@@ -896,7 +905,7 @@ __global__ __launch_bounds__(GB) void k_visibility_grid(
                                   src[3 * s + 1], src[3 * s + 2]);
             float db = rb.len;
             hit = dda_cast<ANY, SEM>(rb, db, G, cell_start, recs, box, nullptr, nullptr, nullptr,
-                                     nullptr, &r);
+                                     MacroTab{nullptr, nullptr}, &r);
         } else {
             hit = dda_cast<ANY, SEM>(r, dist, G, cell_start, recs, box);
         }
@@ -937,10 +946,15 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
     const int32_t *__restrict__ order, unsigned long long *__restrict__ bits,
     float *__restrict__ dist_out, unsigned long long *__restrict__ next_task, int skip_self)
 {
-    __shared__ uint2 smq[MACRO_LDS_MAX];
-    for (int i = threadIdx.x; i < nmacro; i += GB)
-        smq[i] = mq[i];
+    __shared__ uint32_t smqa[MACRO_LDS_MAX];
+    __shared__ uint16_t smqb[MACRO_LDS_MAX];
+    for (int i = threadIdx.x; i < nmacro; i += GB) {
+        const uint2 q = mq[i];
+        smqa[i] = q.x;
+        smqb[i] = (uint16_t)q.y;
+    }
     __syncthreads();
+    const MacroTab smq{smqa, smqb};
     const int64_t NI = SINK_MAJOR ? S : K, NO = SINK_MAJOR ? K : S;
     const int64_t W = (NI + 63) / 64;
     const int lane = threadIdx.x & 63;
@@ -1150,12 +1164,17 @@ __global__ __launch_bounds__(GB) void k_grid_stats(
     const float4 *__restrict__ box, const uint2 *__restrict__ mq, int nmacro,
     unsigned long long *__restrict__ stats, int64_t nbl)
 {
-    __shared__ uint2 smq[MACRO_LDS_MAX];
+    __shared__ uint32_t smqa[MACRO_LDS_MAX];
+    __shared__ uint16_t smqb[MACRO_LDS_MAX];
     const bool lds = nmacro <= MACRO_LDS_MAX;
     if (lds)
-        for (int i = threadIdx.x; i < nmacro; i += GB)
-            smq[i] = mq[i];
+        for (int i = threadIdx.x; i < nmacro; i += GB) {
+            const uint2 q = mq[i];
+            smqa[i] = q.x;
+            smqb[i] = (uint16_t)q.y;
+        }
     __syncthreads();
+    const MacroTab smq{smqa, smqb};
     const int64_t k = (int64_t)blockIdx.x / nbl;
     const int64_t s = ((int64_t)blockIdx.x % nbl) * GB + threadIdx.x;
     unsigned int nt = 0, nc = 0, nm = 0, nw = 0;
@@ -1554,7 +1573,7 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
             return E_HIP;
         }
         const int per_cu = std::max(1, std::min(LDS_WG_PER_CU,
-                                                lds / (int)(MACRO_LDS_MAX * sizeof(uint2))));
+                                                lds / (int)(MACRO_LDS_MAX * (sizeof(uint32_t) + sizeof(uint16_t)))));
         const int64_t words = NO * ceil_div(NI, 64);
         const int64_t nwg =
             std::max<int64_t>(1, std::min<int64_t>(ceil_div(words, 4), (int64_t)cus * per_cu));
