@@ -86,28 +86,29 @@ constexpr float NUMBA_EPS_BELOW = 9.999999406e-08f;
 // ---------------------------------------------------------------------------
 LEM_DEV bool hit_numba(const Ray &r, float dist, const Tri &T, double &t_out)
 {
+    // The float32 part is evaluated in full and its three early outs (A, U, V)
+    // are combined into one test: the values and decisions are those of the
+    // sequential form, without a divergent branch per test.
     float px = r.dy * T.e2z - r.dz * T.e2y;
     float py = r.dz * T.e2x - r.dx * T.e2z;
     float pz = r.dx * T.e2y - r.dy * T.e2x;
     float A = px * T.e1x + py * T.e1y;
     A = A + pz * T.e1z;
     float Aa = fabsf(A);
-    if (Aa <= NUMBA_EPS_BELOW)
-        return false;
     float tx = r.ox - T.v0x, ty = r.oy - T.v0y, tz = r.oz - T.v0z;
     float du = tx * px + ty * py;
     du = du + tz * pz;
     float sg = __builtin_copysignf(1.0f, A);
     float dus = du * sg;
-    if (dus < 0.0f || dus > Aa)
-        return false;
     float qx = ty * T.e1z - tz * T.e1y;
     float qy = tz * T.e1x - tx * T.e1z;
     float qz = tx * T.e1y - ty * T.e1x;
     float dv = r.dx * qx + r.dy * qy;
     dv = dv + r.dz * qz;
     float dvs = dv * sg;
-    if (dvs < 0.0f)
+    // negations of the sequential tests (a NaN passes them, as it does there)
+    const bool cand = !(Aa <= NUMBA_EPS_BELOW) & !(dus < 0.0f) & !(dus > Aa) & !(dvs < 0.0f);
+    if (!cand)
         return false;
     float s1 = dus + dvs;
     float s2 = s1 - Aa;
