@@ -930,7 +930,7 @@ constexpr int LDS_WG_PER_CU = 6;  // 6 x 25.5 KB of the 160 KB LDS of a CU
 // The 8 task counters sit 4 KB apart: device-scope atomics on one cache line
 // serialise where the line lives, which capped claiming near 82 M per second.
 constexpr int COUNTER_STRIDE = 512;  // uint64 elements
-constexpr int CLAIM_WORDS = 8;       // 64-ray words per claim
+constexpr int CLAIM_WORDS = 2;       // 64-ray words per claim
 
 // Persistent form of k_visibility_grid: each workgroup copies the quantised
 // macro table (at most MACRO_LDS_MAX blocks) into LDS once; then each
