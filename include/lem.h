@@ -74,8 +74,9 @@ extern "C" {
                                        word s/64 of row k), the layout
                                        lem_visibility_grid writes with
                                        LEM_VIS_SINK_MAJOR */
-#define LEM_FIELD_INPUT_ORDER 8     /* lem_field_los_grid: lanes take the sinks in input
-                                       order instead of a spatial (Morton) order */
+#define LEM_FIELD_INPUT_ORDER 8     /* lem_field_los_grid (sinks) and lem_field_los_adjoint
+                                       (sources): lanes take the points in input order
+                                       instead of a spatial (Morton) order */
 
 int lem_abi_version(void);
 const char *lem_last_error(void);
@@ -210,7 +211,9 @@ int lem_field_los_grid(const float *src, const float *src_w, int64_t S, const fl
  * grad_out: K x 3 complex64, dLoss/dE in PyTorch's convention
  * (dL/dRe E + j dL/dIm E).  Writes grad_w (S x 3 complex64, dLoss/dw_s) and
  * grad_p (S x 3 float32, dLoss/d source position); either may be NULL.
- * flags: the forward's LEM_FIELD_SINK_TANGENT; snk_w as in the forward.
+ * flags: the forward's LEM_FIELD_SINK_TANGENT; snk_w as in the forward.  Lanes
+ * run over the sources in a Morton order unless LEM_FIELD_INPUT_ORDER is set
+ * (the per-source sums, and so the result, are the same).
  * No counterpart in the reference (it has no gradients); it differentiates
  * lossy_propagation and the launch projector (empropagation.py:178-249,
  * 768-801) as freqdomainkernal combines them for 2-column paths. */

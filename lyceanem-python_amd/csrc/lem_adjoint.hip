@@ -66,16 +66,20 @@ __global__ __launch_bounds__(AB) void k_adjoint_partial(
     const float *__restrict__ src, const float *__restrict__ src_w, int64_t S,
     const float *__restrict__ snk, const float *__restrict__ snk_n, const float *__restrict__ g,
     int64_t K, const unsigned long long *__restrict__ hit_bits, int64_t W64, float alpha,
-    double beta, int64_t CH, int64_t nbs, float *__restrict__ ws_sum, float *__restrict__ ws_cmp)
+    double beta, int64_t CH, int64_t nbs, const int32_t *__restrict__ order,
+    float *__restrict__ ws_sum, float *__restrict__ ws_cmp)
 {
     // a * b + c may be fused in the gradient arithmetic (the Kahan updates
     // have no products, so they are not contracted)
 #pragma clang fp contract(fast)
     __shared__ SnkLds s_k[AK_TILE];
     const int64_t c = (int64_t)blockIdx.x / nbs;
-    const int64_t s = ((int64_t)blockIdx.x % nbs) * AB + threadIdx.x;
-    const bool active = s < S;
-    const int64_t ss = active ? s : 0;
+    const int64_t slot = ((int64_t)blockIdx.x % nbs) * AB + threadIdx.x;
+    const bool active = slot < S;
+    // lanes over a spatial order of the sources: neighbouring elements see the
+    // same sinks, so the mask skips whole waves more often
+    const int64_t s = active ? (order ? (int64_t)order[slot] : slot) : 0;
+    const int64_t ss = s;
     const double xs = src[3 * ss], ys = src[3 * ss + 1], zs = src[3 * ss + 2];
     const C3 w = load_w(src_w, ss);
     const int64_t k0 = c * CH;
@@ -254,12 +258,19 @@ inline int64_t adj_chunks(int64_t S, int64_t K)
 
 using namespace lem;
 
+inline int64_t adj_part_bytes(int64_t S, int64_t K)
+{
+    const int64_t NC = adj_chunks(S, K);
+    const int64_t b = 2 * NC * 9 * S * (int64_t)sizeof(float) + 6 * K * (int64_t)sizeof(float);
+    return ((b + 255) / 256) * 256;
+}
+
 extern "C" int64_t lem_field_los_adjoint_workspace(int64_t S, int64_t K)
 {
     if (S <= 0 || K <= 0)
         return 0;
-    const int64_t NC = adj_chunks(S, K);
-    return 2 * NC * 9 * S * (int64_t)sizeof(float) + 6 * K * (int64_t)sizeof(float) + 256;
+    const int64_t ord = point_order_workspace(S);
+    return ord < 0 ? -1 : adj_part_bytes(S, K) + ord;
 }
 
 extern "C" int lem_field_los_adjoint(const float *src, const float *src_w, int64_t S,
@@ -275,6 +286,10 @@ extern "C" int lem_field_los_adjoint(const float *src, const float *src_w, int64
                 "lem_field_los_adjoint: null pointer");
     LEM_REQUIRE(!(flags & LEM_FIELD_ELEMENTS), "lem_field_los_adjoint: source-sum forward only");
     const int64_t need = lem_field_los_adjoint_workspace(S, K);
+    if (need < 0) {
+        set_error("lem_field_los_adjoint: workspace size query failed");
+        return E_HIP;
+    }
     if (!workspace || workspace_bytes < need) {
         set_error("lem_field_los_adjoint: workspace smaller than lem_field_los_adjoint_workspace()");
         return E_WORKSPACE;
@@ -288,16 +303,23 @@ extern "C" int lem_field_los_adjoint(const float *src, const float *src_w, int64
     float *ws_cmp = ws_sum + NC * 9 * S;
     float *g = ws_cmp + NC * 9 * S;
     hipStream_t st = as_stream(stream);
+    const int32_t *order = nullptr;
+    if (!(flags & LEM_FIELD_INPUT_ORDER)) {
+        int rc = point_order(src, S, reinterpret_cast<char *>(workspace) + adj_part_bytes(S, K),
+                             &order, st);
+        if (rc)
+            return rc;
+    }
     hipLaunchKernelGGL(k_adjoint_prep, dim3((unsigned)ceil_div(K, AB)), dim3(AB), 0, st, grad_out,
                        snk_n, snk_w, (flags & LEM_FIELD_SINK_TANGENT) ? 1 : 0, K, g);
     if (alpha != 0.0)
         hipLaunchKernelGGL(k_adjoint_partial<true>, dim3((unsigned)(NCe * nbs)), dim3(AB), 0, st, src, src_w,
                        S, snk, snk_n, g, K, reinterpret_cast<const unsigned long long *>(hit_bits),
-                       ceil_div(K, 64), (float)alpha, beta, CH, nbs, ws_sum, ws_cmp);
+                       ceil_div(K, 64), (float)alpha, beta, CH, nbs, order, ws_sum, ws_cmp);
     else
         hipLaunchKernelGGL(k_adjoint_partial<false>, dim3((unsigned)(NCe * nbs)), dim3(AB), 0, st, src, src_w,
                        S, snk, snk_n, g, K, reinterpret_cast<const unsigned long long *>(hit_bits),
-                       ceil_div(K, 64), (float)alpha, beta, CH, nbs, ws_sum, ws_cmp);
+                       ceil_div(K, 64), (float)alpha, beta, CH, nbs, order, ws_sum, ws_cmp);
     int rc = check_launch("k_adjoint_partial");
     if (rc)
         return rc;

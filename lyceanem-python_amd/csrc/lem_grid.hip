@@ -1155,6 +1155,28 @@ int morton_order(const float *pts, int64_t n, const VisWs &L, hipStream_t st)
     return OK;
 }
 
+int64_t point_order_workspace(int64_t n)
+{
+    if (n <= 0 || n > 2147483647LL)
+        return -1;
+    const int64_t cub = vis_sort_bytes(n);
+    return cub < 0 ? -1 : vis_layout(n, (size_t)cub, nullptr, nullptr);
+}
+
+int point_order(const float *pts, int64_t n, void *ws, const int32_t **order, hipStream_t st)
+{
+    const int64_t cub = vis_sort_bytes(n);
+    if (cub < 0) {
+        set_error("radix sort size query failed");
+        return E_HIP;
+    }
+    VisWs L;
+    vis_layout(n, (size_t)cub, reinterpret_cast<char *>(ws), &L);
+    const int rc = morton_order(pts, n, L, st);
+    *order = L.order;
+    return rc;
+}
+
 // Instrumented traversal: counts the Moller-Trumbore tests and cell visits a
 // traversal performs (the algorithmic work of the roofline), no outputs.
 template <bool ANY>

@@ -31,6 +31,12 @@ int launch_los_finish(const float *ws_sum, const float *ws_cmp, int64_t NC, int6
                       const float *snk_n, const float *snk_w, int tangent, float *out,
                       hipStream_t st);
 
+// Spatial (3-D Morton) order of n points, computed on the device (lem_grid.hip):
+// point_order_workspace(n) bytes of workspace (256-byte aligned); *order is set
+// to the int32 order[n] inside it.  Returns -1 from the size query on failure.
+int64_t point_order_workspace(int64_t n);
+int point_order(const float *pts, int64_t n, void *ws, const int32_t **order, hipStream_t st);
+
 }  // namespace lem
 
 #define LEM_REQUIRE(cond, msg)            \

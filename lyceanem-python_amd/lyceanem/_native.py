@@ -250,7 +250,8 @@ def field_los(src, src_w, snk, snk_n, hit_bits=None, snk_w=None, alpha=0.0, beta
 
 
 def field_los_adjoint(src, src_w, snk, snk_n, grad_out, hit_bits=None, snk_w=None, alpha=0.0,
-                      beta=2 * np.pi, sink_tangent=False, want_w=True, want_p=True):
+                      beta=2 * np.pi, sink_tangent=False, want_w=True, want_p=True,
+                      input_order=False):
     """Adjoint of field_los (source sum): (grad_w (S,3) complex64, grad_p (S,3) float32)."""
     torch = _torch()
     lib = load_library()
@@ -267,8 +268,10 @@ def field_los_adjoint(src, src_w, snk, snk_n, grad_out, hit_bits=None, snk_w=Non
     gw = torch.empty((S, 3), dtype=torch.complex64, device=src.device) if want_w else None
     gp = torch.empty((S, 3), dtype=torch.float32, device=src.device) if want_p else None
     wsb = int(lib.lem_field_los_adjoint_workspace(S, K))
+    if wsb < 0:
+        raise LemError("lem_field_los_adjoint_workspace: size query failed")
     ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=src.device)
-    flags = FIELD_SINK_TANGENT if sink_tangent else 0
+    flags = (FIELD_SINK_TANGENT if sink_tangent else 0) | (FIELD_INPUT_ORDER if input_order else 0)
     check(lib.lem_field_los_adjoint(ptr(src), ptr(src_w), S, ptr(snk), ptr(snk_n), ptr(snk_w), K,
                                     ptr(hit_bits), float(alpha), float(beta), flags, ptr(g),
                                     ptr(gw), ptr(gp), ptr(ws), wsb, stream_ptr()),

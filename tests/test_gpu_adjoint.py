@@ -140,3 +140,8 @@ def test_adjoint_flags_match_autograd(native, oracle):
     rw = np.linalg.norm(gw.cpu().numpy() - w64.grad.numpy()) / np.linalg.norm(w64.grad.numpy())
     rp = np.linalg.norm(gp.cpu().numpy() - p64.grad.numpy()) / np.linalg.norm(p64.grad.numpy())
     assert rw < GRAD_RTOL and rp < GRAD_RTOL
+    # lanes over the sources in input order: the same per-source sums
+    gw2, gp2 = native.field_los_adjoint(src, w, snk, nrm, g, snk_w=wk, beta=beta,
+                                        sink_tangent=True, input_order=True)
+    assert np.array_equal(gw.cpu().numpy(), gw2.cpu().numpy())
+    assert np.array_equal(gp.cpu().numpy(), gp2.cpu().numpy())
