@@ -256,3 +256,27 @@ def test_sink_major_mask_field_matches_fused(native):
     assert np.array_equal(e1[m.astype(bool)], np.zeros_like(e1[m.astype(bool)]))
     with pytest.raises(native.LemError, match="hit_bits shape"):
         native.field_los(src, w, snk, nrm, hit_bits=bits, bits_sink_major=True)
+
+
+@pytest.mark.parametrize("S,K", [(1, 1), (1, 65), (65, 1), (3, 130), (130, 3)])
+def test_grid_tiny_and_ragged_shapes(native, S, K):
+    """Shapes where rows hold fewer than 8 words (some XCD owners get none),
+    single rays and ragged last words: every layout and mode equals brute force."""
+    rng = np.random.default_rng(S * 1000 + K)
+    ctr = rng.uniform(-2, 2, (2000, 1, 3))
+    tris = (ctr + rng.uniform(-0.3, 0.3, (2000, 3, 3))).reshape(-1, 9).astype(np.float32)
+    src = rng.uniform(-3, 3, (S, 3)).astype(np.float32)
+    snk = rng.uniform(-3, 3, (K, 3)).astype(np.float32)
+    g = native.Grid(tris)
+    ref_b, ref_d = native.visibility_dense(src, snk, tris, any_hit=False, want_dist=True)
+    m = native.unpack_bits(ref_b, K)
+    for sm in (False, True):
+        b, d = g.visibility(src, snk, any_hit=False, want_dist=True, sink_major=sm)
+        mb = native.unpack_bits(b, S).T if sm else native.unpack_bits(b, K)
+        db = d.cpu().numpy().T if sm else d.cpu().numpy()
+        assert np.array_equal(mb, m)
+        assert np.array_equal(db, ref_d.cpu().numpy())
+        for fs in (False, True):
+            a, _ = g.visibility(src, snk, any_hit=True, sink_major=sm, from_sink=fs)
+            ma = native.unpack_bits(a, S).T if sm else native.unpack_bits(a, K)
+            assert np.array_equal(ma, m)
