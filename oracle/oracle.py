@@ -17,9 +17,20 @@ Two parts:
   workchunkingv2's wave assembly (:2681-2955), targettingindex
   (empropagation.py:3394-3429), theta_phi_r (geometryfunctions.py:447-470) and
   the E-theta / E-phi conversion of calculate_farfield (frequency_domain.py:179-190).
+* The time-domain per-path amplitude / arrival and placement of timedomainkernal
+  (empropagation.py:1345-1612; C: orc_time_paths, orc_time_place), the
+  projected-area accumulation of visiblespace (rayfunctions.py:316-427) and the
+  directivity map of MaximumDirectivityMap / directivity_transform
+  (beamforming.py:484-673, 1037-1129).
 
 Every function is pinned by a check in tests/test_oracle_*.py that does not
-restate the same formula (closed forms, exact rational geometry, invariants).
+restate the same formula (closed forms, exact rational geometry, invariants,
+Python's own floor division, literal loops of the reference's cell sums).
+Parity unpinned, because no output of the reference fixes them and the
+reference cannot run here: numba-cuda's possible FMA contraction inside `hit`
+and `dot_vec` (everything here is evaluated without contraction), numba's
+float32 typing of calc_dv_norm (assumed float32), the order of the reference's
+float64 atomic sums, and pyvista's sink order, areas and normals.
 """
 from __future__ import annotations
 
