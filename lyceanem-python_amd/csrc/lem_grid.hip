@@ -684,11 +684,11 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
             const bool ax = Tx <= Ty && Tx <= Tz;
             const bool ay = !ax && Ty <= Tz;
             const float tex = ax ? Tx : (ay ? Ty : Tz);
-            if (tex >= t1 + margin || (hit && dist + margin <= tex))
-                break;
             const int sa = ax ? sx : (ay ? sy : sz);
             const int ja = (ax ? jx : (ay ? jy : jz)) + sa;
-            if (ja < 0 || ja >= (ax ? G.mx : (ay ? G.my : G.mz)))
+            // the exits are combined into one test (one branch per step)
+            if ((tex >= t1 + margin) | (hit & (dist + margin <= tex)) | (ja < 0) |
+                (ja >= (ax ? G.mx : (ay ? G.my : G.mz))))
                 break;
             const float Ta = tnext(last(ja, sa, ax ? G.nx : (ay ? G.ny : G.nz)), sa,
                                    ax ? G.lox : (ay ? G.loy : G.loz), ax ? G.cx : (ay ? G.cy : G.cz),
@@ -775,18 +775,15 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
                 }
             }
             const float texit = fminf(tx, fminf(ty, tz));
-            if ((hit && dist + margin <= texit) || texit >= t1 + margin) {
-                done = true;
-                break;
-            }
-            // next cell across the nearest face (axis selected, as above)
+            // next cell across the nearest face (axis selected, as above); the
+            // exits are combined into one test
             const bool ax = tx <= ty && tx <= tz;
             const bool ay = !ax && ty <= tz;
             const bool az = !ax && !ay;
             const int sa = ax ? sx : (ay ? sy : sz);
             const int ia = (ax ? ix : (ay ? iy : iz)) + sa;
             const int na = ax ? G.nx : (ay ? G.ny : G.nz);
-            if (ia < 0 || ia >= na) {
+            if ((hit & (dist + margin <= texit)) | (texit >= t1 + margin) | (ia < 0) | (ia >= na)) {
                 done = true;
                 break;
             }
