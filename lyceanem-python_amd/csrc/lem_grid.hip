@@ -737,9 +737,12 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
                 e = __builtin_bit_cast(int32_t, chi.w);
                 if (n_cells)
                     *n_cells += 1;
-                if (e > b && !seg_box<PAR>(r, idx, idy, idz, clo, chi, -margin, dist + margin,
-                                           margin))
-                    e = b;
+                // tested unconditionally (an empty cell's range stays empty
+                // whatever the test says), so the range and the box arrive in
+                // one round trip instead of the box being fetched after the
+                // range is known
+                e = seg_box<PAR>(r, idx, idy, idz, clo, chi, -margin, dist + margin, margin) ? e
+                                                                                              : b;
             }
             if (n_tests)
                 *n_tests += (unsigned int)(e - b);
