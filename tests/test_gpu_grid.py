@@ -161,6 +161,29 @@ def test_grid_terrain_small_vs_oracle(native, oracle):
     assert np.array_equal(d, odist), f"{(d != odist).sum()} dist mismatches"
 
 
+def test_grid_flat_terrain_vs_oracle(native, oracle):
+    """A flat terrain (few cells vertically, most rays grazing the surface):
+    every ray's mask and closest-hit distance equal the brute force, for both
+    lane layouts, and the any-hit mask walked from the sink end too."""
+    from lyceanem import scenes
+
+    sc = scenes.terrain_c2(nqx=200, nqy=100, size_x=200.0, size_y=100.0, n_src_side=16,
+                           n_snk=128, corr=8.0, sigma_h=1.0, src_alt=6.0)
+    src, snk, tris = sc["src"], sc["snk"], sc["tris"]
+    g = native.Grid(tris)
+    print(f"flat terrain: {tuple(g.info.n)} cells", flush=True)
+    hit, odist = oracle.visibility_dense(src, snk, tris)
+    assert 0.02 < hit.mean() < 0.98
+    for sm in (False, True):
+        bits, dist = g.visibility(src, snk, any_hit=False, want_dist=True, sink_major=sm)
+        m = native.unpack_bits(bits, len(src)).T if sm else native.unpack_bits(bits, len(snk))
+        d = dist.cpu().numpy().T if sm else dist.cpu().numpy()
+        assert np.array_equal(m, hit), f"sink_major={sm}: {(m != hit).sum()} mask mismatches"
+        assert np.array_equal(d, odist), f"sink_major={sm}: {(d != odist).sum()} dist mismatches"
+    bits_f, _ = g.visibility(src, snk, any_hit=True, sink_major=True, from_sink=True)
+    assert np.array_equal(native.unpack_bits(bits_f, len(src)).T, hit)
+
+
 def test_grid_terrain_full_size_sampled(native, oracle):
     """Full 1,000,000-triangle terrain: the grid (sink-major, as benchmarked) on
     a 64 x 64 source patch x all sinks, checked on sampled rays against the
