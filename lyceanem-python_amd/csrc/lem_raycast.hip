@@ -63,14 +63,14 @@ __device__ __forceinline__ void tile_walk(const TriLds *s_tri, int n, const Ray 
     for (int j = 0; j < n; ++j) {
         Tri t = load_tri(s_tri, j);
         if (SEM == 0) {
+            // hit_numba accepts t < dist - 2e-7 only, so an accepted hit is
+            // always closer than dist (kernel1Dv3's t < ray.dist test)
             double tt;
             if (hit_numba(r, dist, t, tt)) {
-                if (tt < (double)dist) {
-                    dist = (float)tt;
-                    hit = true;
-                    if (ANY)
-                        return;
-                }
+                dist = (float)tt;
+                hit = true;
+                if (ANY)
+                    return;
             }
         } else {
             if (hit_native(r, t)) {
