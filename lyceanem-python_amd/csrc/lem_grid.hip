@@ -927,6 +927,77 @@ __global__ __launch_bounds__(GB) void k_visibility_grid(
 }
 
 constexpr int LDS_WG_PER_CU = 6;  // 6 x 25.5 KB of the 160 KB LDS of a CU
+
+// Rays of the persistent traversal with a zero direction component (along a
+// grid plane), queued by k_visibility_grid_lds as (outer, lane slot): walked
+// here with the general slab test (the macro table in LDS, as there).  A
+// result adds the ray's bit (atomic OR) and overwrites its distance.  If the
+// queue overflowed (more than par_cap such rays), every ray is rechecked
+// instead (a scene whose points share a coordinate plane makes them all such).
+template <bool ANY, bool SINK_MAJOR, int SEM, bool REV>
+__global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_par(
+    const float *__restrict__ src, int64_t S, const float *__restrict__ snk, int64_t K, GridDev G,
+    const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
+    const float4 *__restrict__ box, const uint2 *__restrict__ mq, int nmacro,
+    const int32_t *__restrict__ order, unsigned long long *__restrict__ bits,
+    float *__restrict__ dist_out, int skip_self, const unsigned long long *__restrict__ par_count,
+    const uint2 *__restrict__ par_queue, int64_t par_cap)
+{
+    const unsigned long long n = *par_count;
+    if (n == 0)
+        return;  // nothing queued (the usual case)
+    __shared__ uint32_t smqa[MACRO_LDS_MAX];
+    __shared__ uint16_t smqb[MACRO_LDS_MAX];
+    for (int i = threadIdx.x; i < nmacro; i += GB) {
+        const uint2 q = mq[i];
+        smqa[i] = q.x;
+        smqb[i] = (uint16_t)q.y;
+    }
+    __syncthreads();
+    const MacroTab smq{smqa, smqb};
+    const int64_t NI = SINK_MAJOR ? S : K, NO = SINK_MAJOR ? K : S;
+    const int64_t W = (NI + 63) / 64;
+    const bool scan = n > (unsigned long long)par_cap;
+    const int64_t total = scan ? NO * NI : (int64_t)n;
+    const int64_t stride = (int64_t)gridDim.x * GB;
+    for (int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x; i < total; i += stride) {
+        int64_t outer, slot;
+        if (scan) {
+            outer = i / NI;
+            slot = i % NI;
+        } else {
+            const uint2 e = par_queue[i];
+            outer = e.x;
+            slot = e.y;
+        }
+        const int64_t inner = order ? (int64_t)order[slot] : slot;
+        const int64_t s = SINK_MAJOR ? inner : outer;
+        const int64_t k = SINK_MAJOR ? outer : inner;
+        if (skip_self && s == k)
+            continue;
+        const Ray r = charge(src[3 * s], src[3 * s + 1], src[3 * s + 2], snk[3 * k],
+                             snk[3 * k + 1], snk[3 * k + 2]);
+        const Ray rw = REV ? charge(snk[3 * k], snk[3 * k + 1], snk[3 * k + 2], src[3 * s],
+                                    src[3 * s + 1], src[3 * s + 2])
+                           : r;
+        if (!(rw.dx == 0.0f || rw.dy == 0.0f || rw.dz == 0.0f))
+            continue;  // (scan) walked by the persistent kernel
+        float dist = r.len, dw = rw.len;
+        float &dd = REV ? dw : dist;
+        const bool hit = dda_cast<ANY, SEM, true, true>(rw, dd, G, cell_start, recs, box, nullptr,
+                                                        nullptr, nullptr, smq, REV ? &r : nullptr);
+        if (hit)
+            atomicOr(&bits[outer * W + (inner >> 6)], 1ull << (inner & 63));
+        if (dist_out)
+            dist_out[outer * NI + inner] = dist;
+    }
+}
+
+// Queue of k_visibility_grid_par: a 64-bit count (256 bytes reserved) and
+// PAR_QUEUE_CAP (outer, slot) entries.
+constexpr int64_t PAR_QUEUE_CAP = 1 << 18;
+constexpr int64_t PAR_QUEUE_BYTES = 256 + 8 * PAR_QUEUE_CAP;
+
 // The 8 task counters sit 4 KB apart: device-scope atomics on one cache line
 // serialise where the line lives, which capped claiming near 82 M per second.
 constexpr int COUNTER_STRIDE = 512;  // uint64 elements
@@ -944,7 +1015,8 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
     const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
     const float4 *__restrict__ box, const uint2 *__restrict__ mq, int nmacro,
     const int32_t *__restrict__ order, unsigned long long *__restrict__ bits,
-    float *__restrict__ dist_out, unsigned long long *__restrict__ next_task, int skip_self)
+    float *__restrict__ dist_out, unsigned long long *__restrict__ next_task, int skip_self,
+    unsigned long long *__restrict__ par_count, uint2 *__restrict__ par_queue, int64_t par_cap)
 {
     __shared__ uint32_t smqa[MACRO_LDS_MAX];
     __shared__ uint16_t smqb[MACRO_LDS_MAX];
@@ -1001,13 +1073,21 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
                 float dw = rw.len;
                 float &dd = REV ? dw : dist;
                 const Ray *rt = REV ? &r : nullptr;
-                // rays along a grid plane take the general slab test
-                if (rw.dx == 0.0f || rw.dy == 0.0f || rw.dz == 0.0f)
-                    hit = dda_cast<ANY, SEM, true, true>(rw, dd, G, cell_start, recs, box,
-                                                         nullptr, nullptr, nullptr, smq, rt);
-                else
+                // rays along a grid plane need the general slab test: they are
+                // queued for k_visibility_grid_par (left unblocked, at full
+                // length, here), which keeps that walk's registers and code out
+                // of this kernel
+                if (rw.dx == 0.0f || rw.dy == 0.0f || rw.dz == 0.0f) {
+                    // past the capacity the count only has to stay above it
+                    if (__atomic_load_n(par_count, __ATOMIC_RELAXED) <= (unsigned long long)par_cap) {
+                        const unsigned long long q = atomicAdd(par_count, 1ull);
+                        if (q < (unsigned long long)par_cap)
+                            par_queue[q] = make_uint2((unsigned int)outer, (unsigned int)slot);
+                    }
+                } else {
                     hit = dda_cast<ANY, SEM, true, false>(rw, dd, G, cell_start, recs, box,
                                                           nullptr, nullptr, nullptr, smq, rt);
+                }
             }
             if (order) {
                 // the lanes' bits belong to several words (zeroed beforehand)
@@ -1516,14 +1596,17 @@ extern "C" int64_t lem_visibility_grid_workspace(int64_t S, int64_t K, int flags
     if (S < 0 || K < 0)
         return -1;
     const int64_t NI = (flags & LEM_VIS_SINK_MAJOR) ? S : K;
-    if ((flags & LEM_VIS_INPUT_ORDER) || NI == 0)
-        return VIS_HEAD;
-    if (NI > 2147483647LL)
-        return -1;
-    const int64_t cub = vis_sort_bytes(NI);
-    if (cub < 0)
-        return -1;
-    return vis_layout(NI, (size_t)cub, nullptr, nullptr);
+    int64_t base = VIS_HEAD;
+    if (!(flags & LEM_VIS_INPUT_ORDER) && NI > 0) {
+        if (NI > 2147483647LL)
+            return -1;
+        const int64_t cub = vis_sort_bytes(NI);
+        if (cub < 0)
+            return -1;
+        base = vis_layout(NI, (size_t)cub, nullptr, nullptr);
+    }
+    // the queue of rays along grid planes (k_visibility_grid_par) comes last
+    return ceil_div(base, 256) * 256 + PAR_QUEUE_BYTES;
 }
 
 extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk, int64_t K,
@@ -1549,6 +1632,7 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
     LEM_REQUIRE(!rev || any, "lem_visibility_grid: LEM_VIS_FROM_SINK needs the any-hit mode");
     const int64_t NI = sink_major ? S : K, NO = sink_major ? K : S;
     LEM_REQUIRE(NI <= 2147483647LL, "lem_visibility_grid: more than 2^31-1 lane-side points");
+    LEM_REQUIRE(NO <= 4294967295LL, "lem_visibility_grid: more than 2^32-1 outer points");
     const int64_t nbl = ceil_div(NI, GB);
     GridDev G = to_dev(*info);
     hipStream_t st = as_stream(stream);
@@ -1600,10 +1684,21 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
         const int64_t nwg =
             std::max<int64_t>(1, std::min<int64_t>(ceil_div(words, 4), (int64_t)cus * per_cu));
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, L.counter);
+        // queue of the rays along grid planes, walked afterwards by
+        // k_visibility_grid_par over as many workgroups as fill the chip
+        char *qb = reinterpret_cast<char *>(workspace) + (need - PAR_QUEUE_BYTES);
+        unsigned long long *pcount = reinterpret_cast<unsigned long long *>(qb);
+        uint2 *pq = reinterpret_cast<uint2 *>(qb + 256);
+        hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(GB), 0, st, pcount, 1);
 #define LEM_VGL(A, M, SM, R)                                                                  \
-    hipLaunchKernelGGL((k_visibility_grid_lds<A, M, SM, R>), dim3((unsigned)nwg), dim3(GB), 0, st, \
-                       src, S, snk, K, G, cell_start, rc4, bx4, mq, info->nmacro, order, hb, dist, \
-                       L.counter, skip_self)
+    do {                                                                                      \
+        hipLaunchKernelGGL((k_visibility_grid_lds<A, M, SM, R>), dim3((unsigned)nwg), dim3(GB), 0, \
+                           st, src, S, snk, K, G, cell_start, rc4, bx4, mq, info->nmacro, order, \
+                           hb, dist, L.counter, skip_self, pcount, pq, PAR_QUEUE_CAP);         \
+        hipLaunchKernelGGL((k_visibility_grid_par<A, M, SM, R>), dim3((unsigned)nwg), dim3(GB), 0, \
+                           st, src, S, snk, K, G, cell_start, rc4, bx4, mq, info->nmacro, order, \
+                           hb, dist, skip_self, pcount, pq, PAR_QUEUE_CAP);                    \
+    } while (0)
 #define LEM_VGL2(A, SM, R)                                                                    \
     do {                                                                                      \
         if (sink_major)                                                                       \
@@ -1626,7 +1721,7 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
         }
 #undef LEM_VGL2
 #undef LEM_VGL
-        return check_launch("k_visibility_grid_lds");
+        return check_launch("k_visibility_grid_lds / k_visibility_grid_par");
     }
     LEM_REQUIRE(NO * nbl <= 2147483647LL, "lem_visibility_grid: S*K too large for one launch");
     dim3 grid((unsigned)(NO * nbl));

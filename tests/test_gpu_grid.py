@@ -144,6 +144,54 @@ def test_grid_axis_aligned_and_outside_rays(native, oracle):
     assert np.array_equal(dist_g.cpu().numpy(), odist)
 
 
+@pytest.mark.parametrize("n_par", ["some", "overflow"])
+def test_grid_rays_along_grid_planes(native, n_par):
+    """Rays with a zero direction component are queued by the persistent
+    traversal and walked by a second kernel; past the queue's capacity
+    (262,144 rays) that kernel rechecks every ray.  Masks, distances and the
+    any-hit, sink-major and from-sink variants equal the brute force."""
+    rng = np.random.default_rng(11)
+    ctr = rng.uniform(-2, 2, (1500, 1, 3))
+    tris = (ctr + rng.uniform(-0.3, 0.3, (1500, 3, 3))).reshape(-1, 9).astype(np.float32)
+    if n_par == "some":
+        # sources on the plane z = 0.25; a third of the sinks on it too, and
+        # some sinks sharing a source's x exactly
+        S, K = 300, 400
+        src = rng.uniform(-3, 3, (S, 3)).astype(np.float32)
+        src[:, 2] = 0.25
+        snk = rng.uniform(-3, 3, (K, 3)).astype(np.float32)
+        snk[::3, 2] = 0.25
+        snk[1::7, 0] = src[: len(snk[1::7]), 0]
+    else:
+        # every ray in the plane z = 0.25: 640 x 480 = 307,200 parallel rays
+        S, K = 640, 480
+        src = rng.uniform(-3, 3, (S, 3)).astype(np.float32)
+        snk = rng.uniform(-3, 3, (K, 3)).astype(np.float32)
+        src[:, 2] = 0.25
+        snk[:, 2] = 0.25
+    print(f"rays along grid planes: {n_par}, S={S} K={K}", flush=True)
+    g = native.Grid(tris)
+    assert g.info.nmacro <= 4352  # the persistent traversal runs
+    bits_b, dist_b = native.visibility_dense(src, snk, tris, any_hit=False, want_dist=True)
+    mb, db = native.unpack_bits(bits_b, K), dist_b.cpu().numpy()
+    assert 0.05 < mb.mean() < 0.95
+    for sm in (False, True):
+        bits, dist = g.visibility(src, snk, any_hit=False, want_dist=True, sink_major=sm)
+        m = native.unpack_bits(bits, S).T if sm else native.unpack_bits(bits, K)
+        d = dist.cpu().numpy().T if sm else dist.cpu().numpy()
+        assert np.array_equal(m, mb), f"sink_major={sm}: {(m != mb).sum()} mask mismatches"
+        assert np.array_equal(d, db), f"sink_major={sm}: {(d != db).sum()} dist mismatches"
+        bits_a, _ = g.visibility(src, snk, any_hit=True, sink_major=sm)
+        ma = native.unpack_bits(bits_a, S).T if sm else native.unpack_bits(bits_a, K)
+        assert np.array_equal(ma, mb)
+        for io in (False, True):
+            bits_i, _ = g.visibility(src, snk, any_hit=True, sink_major=sm, input_order=io)
+            mi = native.unpack_bits(bits_i, S).T if sm else native.unpack_bits(bits_i, K)
+            assert np.array_equal(mi, mb), f"sink_major={sm} input_order={io}"
+    bits_f, _ = g.visibility(src, snk, any_hit=True, sink_major=True, from_sink=True)
+    assert np.array_equal(native.unpack_bits(bits_f, S).T, mb)
+
+
 def test_grid_terrain_small_vs_oracle(native, oracle):
     from lyceanem import scenes
 
