@@ -64,12 +64,31 @@ def build(arch: str = "gfx950", jobs: int = 8, verbose: bool = False) -> str:
     return LIB
 
 
+TEST_NATIVE = os.path.join(os.path.dirname(HERE), "tests", "native")
+
+
+def build_test_libs(arch: str = "gfx950") -> list:
+    """Check libraries of the GPU tests: tests/native/<name>.hip ->
+    tests/native/lib<name>.so (not part of the package)."""
+    out = []
+    for src in sorted(glob.glob(os.path.join(TEST_NATIVE, "*.hip"))):
+        lib = os.path.join(TEST_NATIVE, "lib" + os.path.basename(src)[:-4] + ".so")
+        cmd = [hipcc(), f"--offload-arch={arch}", *COMMON, "-shared", src, "-o", lib + ".tmp"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {src}:\n{r.stdout}\n{r.stderr}")
+        os.replace(lib + ".tmp", lib)
+        out.append(lib)
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--arch", default=os.environ.get("LEM_ARCH", "gfx950"))
     ap.add_argument("-j", type=int, default=8)
     a = ap.parse_args(argv)
     build(a.arch, a.j, verbose=True)
+    build_test_libs(a.arch)
     return 0
 
 

@@ -67,6 +67,23 @@ LEM_DEV Tri make_tri(const float *t9)
 // `-1e-7 < A < 1e-7` in float64 is exactly `|A| <= NUMBA_EPS_BELOW` in float32.
 constexpr float NUMBA_EPS_BELOW = 9.999999406e-08f;
 
+// float64 reciprocal of a float32 value, equal bit for bit to the IEEE
+// quotient 1.0 / (double)a for every finite non-zero float32 a (and NaN for
+// NaN): the hardware estimate refined by two Newton steps, y += y * (1 - a y)
+// with fused operations.  A float32 a has a 24-bit significand, so 1/a is
+// never a tie between two doubles; an exhaustive GPU test compares the two
+// over every float32 bit pattern.  About a third of the instructions of the
+// general float64 division (no scaling steps: 1/a stays far inside the double
+// range for any float32 a).  For a = +-inf the quotient is +-0 and this gives
+// NaN; hit_numba cannot tell them apart (see there).
+LEM_DEV double recip_f32_to_f64(float a)
+{
+    const double ad = (double)a;
+    double y = __builtin_amdgcn_rcp(ad);
+    y = __builtin_fma(y, __builtin_fma(-ad, y, 1.0), y);
+    return __builtin_fma(y, __builtin_fma(-ad, y, 1.0), y);
+}
+
 // ---------------------------------------------------------------------------
 // numba `hit` (rayfunctions.py:37-104): two-sided Moller-Trumbore with
 // float32 edges / cross / dot and float64 F = 1/A, U, V, U+V and t.
@@ -112,7 +129,10 @@ LEM_DEV bool hit_numba(const Ray &r, float dist, const Tri &T, double &t_out)
         return false;
     float s1 = dus + dvs;
     float s2 = s1 - Aa;
-    double F = 1.0 / (double)A;
+    // == 1.0 / (double)A for finite A.  An infinite A (float32 overflow of the
+    // products) makes F NaN instead of 0: U + V > 1 and the window test on t
+    // then compare NaN instead of 0 (or NaN), and reject either way.
+    double F = recip_f32_to_f64(A);
     if (fabsf(s2) > 1e-6f * (s1 + Aa)) {
         if (s2 > 0.0f)
             return false;
