@@ -130,8 +130,8 @@ LEM_DEV bool hit_numba(const Ray &r, float dist, const Tri &T, double &t_out)
     float s1 = dus + dvs;
     float s2 = s1 - Aa;
     // == 1.0 / (double)A for finite A.  An infinite A (float32 overflow of the
-    // products) makes F NaN instead of 0: U + V > 1 and the window test on t
-    // then compare NaN instead of 0 (or NaN), and reject either way.
+    // products) makes F NaN instead of 0: U + V > 1 is then false for both
+    // (NaN, or 0 / NaN), and the window test on t = F dt fails for both.
     double F = recip_f32_to_f64(A);
     if (fabsf(s2) > 1e-6f * (s1 + Aa)) {
         if (s2 > 0.0f)
