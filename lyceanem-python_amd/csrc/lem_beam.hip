@@ -36,7 +36,11 @@ __global__ __launch_bounds__(256) void k_beam_power(
 {
     // chunk staging (40 KB); reused as the epilogue exchange (32 KB)
     __shared__ float smem[2 * BK * BM + 4 * BK * BN];
-    float *sWr = smem, *sWi = smem + BK * BM;                 // [e][b]
+    // W tile [e][b ^ e]: the row index is XOR-swizzled by the element so that
+    // the transposing stores (32 lanes of one row, consecutive elements) and
+    // the MFMA operand reads (consecutive rows of one element) both fall in
+    // distinct LDS banks
+    float *sWr = smem, *sWi = smem + BK * BM;
     float *sE = smem + 2 * BK * BM;                          // [pol][re/im][e][p]
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int ct = wave & 1, part_ri = wave >> 1;
@@ -52,14 +56,14 @@ __global__ __launch_bounds__(256) void k_beam_power(
     }
     for (int64_t e0 = 0; e0 < ne; e0 += BK) {
         __syncthreads();
-        // W tile: 32 rows x 32 elements, stored [e][b]
+        // W tile: 32 rows x 32 elements, stored [e][b ^ e]
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int idx = tid + 256 * i;
             const int bl = idx >> 5, el = idx & 31;
             const float2 w = W[(b0 + bl) * ne + e0 + el];
-            sWr[el * BM + bl] = w.x;
-            sWi[el * BM + bl] = w.y;
+            sWr[el * BM + (bl ^ el)] = w.x;
+            sWi[el * BM + (bl ^ el)] = w.y;
         }
         // pattern tiles: 32 elements x 64 points, both polarisations
 #pragma unroll
@@ -79,7 +83,7 @@ __global__ __launch_bounds__(256) void k_beam_power(
             const int e = 2 * kk + (lane >> 5);
             const int row = lane & 31;
             const int col = ct * 32 + (lane & 31);
-            const float wr = sWr[e * BM + row], wi = sWi[e * BM + row];
+            const float wr = sWr[e * BM + (row ^ e)], wi = sWi[e * BM + (row ^ e)];
             const float tr = sE[(0 * BK + e) * BN + col], ti = sE[(1 * BK + e) * BN + col];
             const float fr = sE[(2 * BK + e) * BN + col], fi = sE[(3 * BK + e) * BN + col];
             if (part_ri == 0) {  // Re = wr er - wi ei
