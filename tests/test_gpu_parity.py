@@ -129,6 +129,38 @@ def test_field_los_sum_vs_fp64_oracle(native, oracle):
     assert nrm < FIELD_RTOL and per < FIELD_RTOL
 
 
+@pytest.mark.parametrize("S,K", [(10, 5), (5000, 3), (700, 130), (64, 4100)])
+def test_field_los_chunk_counts(native, oracle, S, K):
+    """The source sum is split into chunks by the sink count (about 2^21
+    (sink, chunk) threads, at least 64 sources a chunk): one chunk (S < 64),
+    the 64-source cap with few sinks, ragged chunks and a single chunk with
+    many sink blocks all match the float64 oracle, with and without a mask
+    and with lossy propagation."""
+    rng = np.random.default_rng(S + K)
+    src = rng.uniform(-1, 1, (S, 3)).astype(np.float32)
+    snk = (rng.normal(size=(K, 3)) * 50).astype(np.float32)
+    nrm = rng.normal(size=(K, 3)).astype(np.float32)
+    w = (rng.normal(size=(S, 3)) + 1j * rng.normal(size=(S, 3))).astype(np.complex64)
+    hit = rng.random((S, K)) < 0.3
+    W = (K + 63) // 64
+    packed = np.zeros((S, W), np.uint64)
+    for k in range(K):
+        packed[:, k >> 6] |= hit[:, k].astype(np.uint64) << np.uint64(k & 63)
+    import torch
+    bits = torch.from_numpy(packed.view(np.int64)).cuda()
+    for alpha, mask in ((0.0, False), (0.0, True), (0.3, True)):
+        E = native.field_los(src, w, snk, nrm, hit_bits=bits if mask else None, alpha=alpha,
+                             beta=2 * np.pi / 0.1).cpu().numpy()
+        ref = oracle.field_los(src, w, snk, nrm, alpha, 2 * np.pi / 0.1,
+                               hit=hit if mask else None)
+        El = oracle.field_los(src, w, snk, nrm, alpha, 2 * np.pi / 0.1,
+                              hit=hit if mask else None, elements=True)
+        scale = np.abs(El).sum(axis=0).max(axis=-1) + 1e-30
+        nrm_err, per = _field_errors(E, ref, scale)
+        print(f"chunks S={S} K={K} alpha={alpha} mask={mask}: {nrm_err:.2e} {per:.2e}", flush=True)
+        assert nrm_err < FIELD_RTOL and per < FIELD_RTOL
+
+
 def test_field_los_elements_and_phase(native, oracle):
     sc = _c1_small(n=8, n_tris=300, n_el=21, n_az=41)
     S, K = len(sc["src"]), len(sc["snk"])
