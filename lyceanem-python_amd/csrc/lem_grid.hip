@@ -1001,7 +1001,7 @@ constexpr int64_t PAR_QUEUE_BYTES = 256 + 8 * PAR_QUEUE_CAP;
 // The 8 task counters sit 4 KB apart: device-scope atomics on one cache line
 // serialise where the line lives, which capped claiming near 82 M per second.
 constexpr int COUNTER_STRIDE = 512;  // uint64 elements
-constexpr int CLAIM_WORDS = 2;       // 64-ray words per claim
+constexpr int CLAIM_WORDS = 3;       // 64-ray words per claim
 
 // Persistent form of k_visibility_grid: each workgroup copies the quantised
 // macro table (at most MACRO_LDS_MAX blocks) into LDS once; then each
