@@ -66,6 +66,30 @@ def _events(torch, n):
     return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
 
 
+def _bits_at(torch, bits, rows, cols):
+    """Bits (rows[i], cols[i]) of a (rows, words) int64 device tensor -> bool array."""
+    r = torch.from_numpy(np.asarray(rows, np.int64)).to(bits.device)
+    c = torch.from_numpy(np.asarray(cols, np.int64)).to(bits.device)
+    return ((bits[r, c >> 6] >> (c & 63)) & 1).bool().cpu().numpy()
+
+
+def _field_errors(got, ref):
+    """Normwise relative error against the float64 oracle, the worst relative
+    error of the entries with |ref| >= 1e-3 max |ref| (the rest sit in pattern
+    nulls) and the worst phase error of those entries (rad)."""
+    got = np.asarray(got, np.complex128).reshape(-1)
+    ref = np.asarray(ref, np.complex128).reshape(-1)
+    big = np.abs(ref) >= 1e-3 * np.abs(ref).max()
+    return {"normwise_rel": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
+            "max_rel_entry": float((np.abs(got[big] - ref[big]) / np.abs(ref[big])).max()),
+            "max_phase_rad": float(np.abs(np.angle(got[big] / ref[big])).max()),
+            "entries": int(got.size), "entries_compared_per_entry": int(big.sum())}
+
+
+def _mismatch(name, got, ref):
+    return {f"{name}_mismatches": int((np.asarray(got) != np.asarray(ref)).sum())}
+
+
 # --------------------------------------------------------------------------
 # config 1: brute-force visibility + line-of-sight field
 # --------------------------------------------------------------------------
@@ -142,16 +166,41 @@ class C1:
         dt = time.perf_counter() - t0
         rows = int(max(1, min(self.S, budget_s / max(dt, 1e-6))))
         t0 = time.perf_counter()
-        hit, _ = orc.visibility_dense(sc["src"][:rows], sc["snk"], sc["tris"], want_dist=True,
-                                      nthreads=nth)
+        hit, dist = orc.visibility_dense(sc["src"][:rows], sc["snk"], sc["tris"], want_dist=True,
+                                         nthreads=nth)
         orc.field_los(sc["src"][:rows], sc["w"][:rows], sc["snk"], sc["snk_n"], 0.0,
                       sc["beta"], hit=hit, nthreads=nth)
         dt = time.perf_counter() - t0
+        self.cpu_sample = (rows, hit, dist)
         return {"value": rows * self.K / dt / 1e6, "unit": "Mrays/s", "cores": nth,
                 "kind": "port",
                 "sample": f"first {rows} of {self.S} source rows x {self.K} sinks x {self.T} "
                           f"triangles (oracle closest-hit visibility + float64 field), "
                           f"{dt:.1f} s"}
+
+    cpu_sample = None
+
+    def parity(self):
+        """Masks and distances of the last step against the oracle's brute force
+        (the CPU baseline's source rows, else the first 8); the field against
+        the float64 oracle over every pair, given the step's mask."""
+        from oracle import oracle as orc
+
+        sc = self.sc
+        if self.cpu_sample is None:
+            rows = 8
+            hit, dist = orc.visibility_dense(sc["src"][:rows], sc["snk"], sc["tris"])
+        else:
+            rows, hit, dist = self.cpu_sample
+        mask = self.nat.unpack_bits(self.bits, self.K)
+        out = {"rays_vs_oracle": rows * self.K,
+               **_mismatch("mask", mask[:rows], hit),
+               **_mismatch("dist", self.dist[:rows].cpu().numpy(), dist),
+               "blocked_fraction": float(mask.mean())}
+        ref = orc.field_los(sc["src"], sc["w"], sc["snk"], sc["snk_n"], 0.0, sc["beta"],
+                            hit=mask)
+        out["field_vs_fp64_all_pairs"] = _field_errors(self.out.cpu().numpy(), ref)
+        return out
 
     def config(self):
         return {"workload": "c1: 32x32 array, 2000 random triangles, 91x181 sinks at 1 km, 3 GHz",
@@ -218,6 +267,7 @@ class C2:
         # the first hit (every reference of every crossed cell), 46 flops each
         work_all = conservative * scale
         achieved = work_all * MT_FLOPS / (vis * 1e-3)
+        executed = tests * scale * MT_FLOPS / (vis * 1e-3)
         bytes_out = self.bits.numel() * 8 + self.dist.numel() * 4
         per_ray = _traffic_per_ray()
         traffic = None if per_ray is None else per_ray * self.rays()
@@ -225,7 +275,14 @@ class C2:
             "kernel": "k_visibility_grid_lds<closest-hit, sink-major>",
             "kernel_ms": vis,
             "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic,
+                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                         "basis": "algorithmic: 46 FP32 flops x the MT tests a conservative "
+                                  "front-to-back DDA performs up to the first hit (every "
+                                  "triangle reference of every crossed cell)",
+                         "achieved_executed": executed / 1e12,
+                         "frac_executed": executed / PEAK_FP32,
+                         "basis_executed": "46 FP32 flops x the MT tests the kernel performs",
+                         "traffic": traffic,
                          "traffic_source": None if traffic is None else
                          "profiles/" + os.path.basename(TRAFFIC_PROFILE) + " (bytes per ray x "
                          "rays of this launch)"},
@@ -260,12 +317,48 @@ class C2:
         n = int(max(64, min(10 ** 6, n * budget_s / max(dt, 1e-6))))
         pairs = np.stack([rng.integers(0, self.S, n), rng.integers(0, self.K, n)], 1)
         t0 = time.perf_counter()
-        orc.visibility_sampled(sc["src"], sc["snk"], pairs, sc["tris"], nthreads=nth)
+        hit, dist = orc.visibility_sampled(sc["src"], sc["snk"], pairs, sc["tris"], nthreads=nth)
         dt = time.perf_counter() - t0
+        self.cpu_sample = (pairs, hit, dist)
         return {"value": n / dt / 1e6, "unit": "Mrays/s", "cores": nth, "kind": "port",
                 "sample": f"{n} random source->sink rays of the 1.718e10, brute force over all "
                           f"{self.T} triangles (kernel1Dv3 closest hit, the reference "
                           f"algorithm), {dt:.1f} s"}
+
+    cpu_sample = None
+
+    def parity(self, n=1_000_000):
+        """The last step's mask and distances on n random rays against the
+        brute-force kernel over every triangle (lem_raycast_pairs, bit-exact
+        against the oracle in the tests), and on the CPU baseline's rays (or
+        10,000 rays) against the oracle itself."""
+        from oracle import oracle as orc
+
+        torch, sc = self.torch, self.sc
+        rng = np.random.default_rng(12345)
+        s_i, k_i = rng.integers(0, self.S, n), rng.integers(0, self.K, n)
+        m_g = _bits_at(torch, self.bits, k_i, s_i)
+        dev = self.dist.device
+        d_g = self.dist[torch.from_numpy(k_i).to(dev), torch.from_numpy(s_i).to(dev)]
+        pts = torch.cat([self.src, self.snk])
+        pairs = torch.from_numpy(np.stack([s_i, self.S + k_i], 1).astype(np.int32)).to(dev)
+        inter, bd = self.nat.raycast_pairs(pts, pairs, self.tris_t, any_hit=False,
+                                           want_dist=True)
+        m_b = inter.cpu().numpy().astype(bool)
+        out = {"gpu_brute_force": {"rays": n, **_mismatch("mask", m_g, m_b),
+                                   **_mismatch("dist", d_g.cpu().numpy(), bd.cpu().numpy()),
+                                   "blocked_fraction": float(m_b.mean())}}
+        if self.cpu_sample is None:
+            pairs_c = np.stack([s_i[:10000], k_i[:10000]], 1)
+            hit, dist = orc.visibility_sampled(sc["src"], sc["snk"], pairs_c, sc["tris"])
+        else:
+            pairs_c, hit, dist = self.cpu_sample
+        m_c = _bits_at(torch, self.bits, pairs_c[:, 1], pairs_c[:, 0])
+        d_c = self.dist[torch.from_numpy(pairs_c[:, 1]).to(dev),
+                        torch.from_numpy(pairs_c[:, 0]).to(dev)].cpu().numpy()
+        out["oracle"] = {"rays": int(len(pairs_c)), **_mismatch("mask", m_c, hit),
+                         **_mismatch("dist", d_c, dist)}
+        return out
 
     def config(self):
         return {"workload": "c2: 1000x500-quad terrain (1,000,000 triangles, Gaussian heights "
@@ -333,12 +426,19 @@ class C3:
             self.grid, self.t["src"], sub, any_hit=True)
         n_sub = self.S * sub.shape[0]
         achieved = conservative * (self.P / sub.shape[0]) * MT_FLOPS / (vis * 1e-3)
+        executed = tests * (self.P / sub.shape[0]) * MT_FLOPS / (vis * 1e-3)
         return {
             "kernel": "k_visibility_grid_lds<any-hit, sink-major> (leg 1 mask: array -> "
                       "scatterers)",
             "kernel_ms": vis,
             "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None},
+                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                         "basis": "algorithmic: 46 FP32 flops x the MT tests a conservative "
+                                  "front-to-back DDA performs up to the first hit",
+                         "achieved_executed": executed / 1e12,
+                         "frac_executed": executed / PEAK_FP32,
+                         "basis_executed": "46 FP32 flops x the MT tests the kernel performs",
+                         "traffic": None},
             "extra": {"gpairs_per_s": self.rays() / ((vis + rest) * 1e-3) / 1e9,
                       "leg1_walk": "from scatterer" if self.from_sink else "from array",
                       "leg1_mask_grays_per_s": pairs1 / (vis * 1e-3) / 1e9,
@@ -371,6 +471,59 @@ class C3:
                 "sample": f"{n} random source->scatterer pairs, brute-force visibility over "
                           f"{self.T} triangles (the field term is negligible beside it), "
                           f"{dt:.1f} s"}
+
+    def parity(self, n_rays=100_000, n_scat=512, n_sinks=256):
+        """Outside the timed region: the three grid masks as the step casts them,
+        on random rays against the oracle's brute force over every triangle; the
+        scatterer fields H of leg 1 on n_scat random scatterers and the step's
+        sink fields on n_sinks random sinks against the float64 oracle given the
+        GPU masks (leg 2 takes the GPU's H as its input)."""
+        from oracle import oracle as orc
+
+        torch, nat, t, g, sc = self.torch, self.nat, self.t, self.grid, self.sc
+        b = sc["beta"]
+        S, P, K = self.S, self.P, self.K
+        rng = np.random.default_rng(54321)
+        out = {}
+        bits1, _ = g.visibility(t["src"], t["scat"], any_hit=True, sink_major=True,
+                                from_sink=self.from_sink)
+        s_i, p_i = rng.integers(0, S, n_rays), rng.integers(0, P, n_rays)
+        ref, _ = orc.visibility_sampled(sc["src"], sc["scat"], np.stack([s_i, p_i], 1),
+                                        sc["tris"], any_hit_only=True)
+        out["leg1_mask_vs_oracle"] = {"rays": n_rays, **_mismatch(
+            "mask", _bits_at(torch, bits1, p_i, s_i), ref), "blocked_fraction": float(ref.mean())}
+        H = nat.field_los(t["src"], t["w"], t["scat"], t["scat_n"], hit_bits=bits1,
+                          snk_w=t["scat_w"], beta=b, sink_tangent=True, bits_sink_major=True)
+        ps = np.sort(rng.choice(P, n_scat, replace=False))
+        hit1 = nat.unpack_bits(bits1[torch.from_numpy(ps).to(bits1.device)], S).T
+        del bits1
+        H_ref = orc.field_los(sc["src"], sc["w"], sc["scat"][ps], sc["scat_n"][ps], 0.0, b,
+                              hit=hit1, snk_w=sc["scat_w"][ps], sink_tangent=True)
+        H_np = H.cpu().numpy()
+        out["leg1_field_vs_fp64"] = {"scatterers": n_scat, **_field_errors(H_np[ps], H_ref)}
+        bits2, _ = g.visibility(t["scat"], t["snk"], any_hit=True)
+        q_i, k_i = rng.integers(0, P, n_rays), rng.integers(0, K, n_rays)
+        ref, _ = orc.visibility_sampled(sc["scat"], sc["snk"], np.stack([q_i, k_i], 1),
+                                        sc["tris"], any_hit_only=True)
+        out["leg2_mask_vs_oracle"] = {"rays": n_rays, **_mismatch(
+            "mask", _bits_at(torch, bits2, q_i, k_i), ref), "blocked_fraction": float(ref.mean())}
+        bits3, _ = g.visibility(t["src"], t["snk"], any_hit=True, sink_major=True)
+        m = n_rays // 5
+        s3, k3 = rng.integers(0, S, m), rng.integers(0, K, m)
+        ref, _ = orc.visibility_sampled(sc["src"], sc["snk"], np.stack([s3, k3], 1), sc["tris"],
+                                        any_hit_only=True)
+        out["los_mask_vs_oracle"] = {"rays": m, **_mismatch(
+            "mask", _bits_at(torch, bits3, k3, s3), ref), "blocked_fraction": float(ref.mean())}
+        ks = np.sort(rng.choice(K, n_sinks, replace=False))
+        pp, kk = np.meshgrid(np.arange(P), ks, indexing="ij")
+        hit2 = _bits_at(torch, bits2, pp.ravel(), kk.ravel()).reshape(P, n_sinks)
+        hit3 = nat.unpack_bits(bits3[torch.from_numpy(ks).to(bits3.device)], S).T
+        E_ref = orc.field_los(sc["scat"], H_np, sc["snk"][ks], sc["snk_n"][ks], 0.0, b, hit=hit2)
+        E_ref = E_ref + orc.field_los(sc["src"], sc["w"], sc["snk"][ks], sc["snk_n"][ks], 0.0, b,
+                                      hit=hit3, sink_tangent=True)
+        out["sink_field_vs_fp64"] = {"sinks": n_sinks,
+                                     **_field_errors(self.E.cpu().numpy()[ks], E_ref)}
+        return out
 
     def config(self):
         return {"workload": "c3: 256x256 array at 5.8 GHz, 50k random triangles, 200k "
@@ -454,6 +607,22 @@ class C4:
         return {"value": rows * self.K / dt / 1e9, "unit": "Gpairs/s", "cores": nth,
                 "kind": "port", "sample": f"float64 field of the first {rows} of {self.S} "
                                           f"elements x {self.K} sinks, {dt:.1f} s"}
+
+    def parity(self, n_sinks=4096):
+        """The forward field on n_sinks random sinks against the float64 oracle
+        over every element, given the frozen mask."""
+        from oracle import oracle as orc
+        from lyceanem.differentiable import far_field
+
+        sc = self.sc
+        with self.torch.no_grad():
+            E = far_field(self.w, self.pos, self.snk, self.snk_n, hit_bits=self.bits,
+                          beta=sc["beta"]).cpu().numpy()
+        ks = np.sort(np.random.default_rng(777).choice(self.K, n_sinks, replace=False))
+        hit = self.nat.unpack_bits(self.bits, self.K)[:, ks]
+        ref = orc.field_los(sc["src"], sc["w"], sc["snk"][ks], sc["snk_n"][ks], 0.0, sc["beta"],
+                            hit=hit)
+        return {"forward_field_vs_fp64": {"sinks": n_sinks, **_field_errors(E[ks], ref)}}
 
     def config(self):
         return {"workload": "c4: 64x64 conformal array on a 0.5 m cylinder, 10 GHz, 181x361 "
@@ -542,6 +711,22 @@ class C5:
         return {"value": rows * self.K * 64 / dt / 1e9, "unit": "G (pair x set)/s", "cores": nth,
                 "kind": "port", "sample": f"float64 G of {rows} sources x {self.K} sinks at one "
                                           f"frequency, contracted with 64 sets, {dt:.1f} s"}
+
+    def parity(self, n_sinks=64):
+        """The last step's E at the lowest and highest frequency on n_sinks random
+        sinks against float64 G . W (the oracle's per-element field times W)."""
+        from oracle import oracle as orc
+
+        sc = self.sc
+        ks = np.sort(np.random.default_rng(99).choice(self.K, n_sinks, replace=False))
+        out = {}
+        for f in (0, self.F - 1):
+            G = orc.field_los(sc["src"], sc["pol"].astype(np.complex64), sc["snk"][ks],
+                              sc["snk_n"][ks], 0.0, float(sc["wavenumbers"][f]), elements=True)
+            ref = np.einsum("skc,sb->ckb", G, sc["W"][f].astype(np.complex128))
+            got = self.out[f][:, self.torch.from_numpy(ks).to(self.out.device)].cpu().numpy()
+            out[f"freq{f}_vs_fp64"] = {"sinks": n_sinks, **_field_errors(got, ref)}
+        return out
 
     def config(self):
         return {"workload": "c5: 128x128 sources x 128x128 far-field sinks x 32 frequencies "
@@ -649,6 +834,36 @@ class TD:
                           f"{self.N}-sample trace (the reference's per-path scatter), "
                           f"{dt:.1f} s"}
 
+    def parity(self, n_paths=65536):
+        """Per-path amplitude and arrival of the last step on random paths, and
+        the trace of the first n_paths paths placed on their own, against the
+        oracle."""
+        from oracle import oracle as orc
+
+        sc, nat = self.sc, self.nat
+        rng = np.random.default_rng(4242)
+        pick = np.sort(rng.choice(self.V, 4096, replace=False))
+        amp_o, arr_o = orc.time_paths(sc["pos"], sc["nrm"], sc["w"], sc["index"][pick], 0.0,
+                                      sc["beta"])
+        amp = self.amp.cpu().numpy()[pick]
+        arr = self.arr.cpu().numpy()[pick]
+        out = {"paths_vs_oracle": {
+            "paths": 4096,
+            "amp_max_rel": float(np.abs(amp - amp_o).max() / np.abs(amp_o).max()),
+            "arrival_max_rel": float(np.abs(arr - arr_o).max() / np.abs(arr_o).max())}}
+        idx = sc["index"][:n_paths]
+        a, r = nat.time_paths(sc["pos"], sc["nrm"], sc["w"], idx, 0.0, sc["beta"])
+        wake = nat.time_wake(r)
+        bins = nat.time_bins(r, wake, sc["fs"])
+        tr = nat.time_trace(a, bins, sc["exc"], self.N).cpu().numpy()
+        amp_o, arr_o = orc.time_paths(sc["pos"], sc["nrm"], sc["w"], idx, 0.0, sc["beta"])
+        ref, _, bins_o = orc.time_place(amp_o, arr_o, np.zeros(len(idx), np.int64), 1, sc["exc"],
+                                        sc["fs"], self.N)
+        out["trace_vs_oracle"] = {
+            "paths": int(len(idx)), **_mismatch("start_sample", bins.cpu().numpy(), bins_o),
+            "max_abs_err_over_max": float(np.abs(tr - ref[0]).max() / np.abs(ref[0]).max())}
+        return out
+
     def config(self):
         return {"workload": "td: 64x64 array at 3 GHz -> 4096 sinks 50-500 m, 16,777,216 "
                             "line-of-sight paths, 256-sample pulse at 40 GS/s into a "
@@ -744,6 +959,20 @@ class BF:
                           f"{Et.shape[1]}x{Et.shape[2]} grid with 3 weight sets, {dt:.1f} s "
                           f"(the directivity integrals not included)"}
 
+    def parity(self, n_rows=16):
+        """Beamformed power of n_rows random weight rows against a float64 numpy
+        contraction of the same rows."""
+        rng = np.random.default_rng(31)
+        rows = np.sort(rng.choice(self.ncmd * 3, n_rows, replace=False))
+        W = self.W[:, : self.ne].cpu().numpy()[rows].astype(np.complex128)
+        Et = self.Et[: self.ne, : self.npts].cpu().numpy().astype(np.complex128)
+        Ep = self.Ep[: self.ne, : self.npts].cpu().numpy().astype(np.complex128)
+        q = self.q[: self.npts].cpu().numpy().astype(np.float64)
+        ref = (np.abs(W @ Et) ** 2 + np.abs(W @ Ep) ** 2) @ q
+        got = self.power.cpu().numpy()[rows]
+        return {"power_vs_fp64": {"rows": n_rows,
+                                  "max_rel": float(np.abs(got - ref).max() / np.abs(ref).max())}}
+
     def config(self):
         return {"workload": "bf: 32x32 array at 3 GHz, 91x181 pattern grid, every grid point "
                             "a command angle x 3 weight schemes (MaximumDirectivityMap)",
@@ -761,6 +990,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default=DEFAULT_CONFIG, choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-parity", action="store_true",
+                    help="skip the parity checks that run after the timed region")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--c2-sinks", type=int, default=16384, help="c2 sink count (probe runs)")
     ap.add_argument("--cells-per-tri", type=float, default=2.0, help="grid cell budget")
@@ -822,6 +1053,14 @@ def main(argv=None) -> int:
         cpu = cfg.cpu_baseline(a.cpu_seconds)
         print(f"[bench] cpu baseline: {cpu['value']:.3f} {cpu['unit']}", file=sys.stderr,
               flush=True)
+    parity = None
+    if not a.no_parity:
+        t0 = time.perf_counter()
+        parity = cfg.parity()
+        parity["checked_in_s"] = time.perf_counter() - t0
+        print(f"[bench] parity checked in {parity['checked_in_s']:.1f} s", file=sys.stderr,
+              flush=True)
+    rec["extra"]["parity"] = parity
     unit = {"c3": "Gpairs/s", "c4": "Gpairs/s", "c5": "G (pair x set)/s",
             "td": "Gpaths/s", "bf": "commands/s"}.get(a.config, "Mrays/s")
     scale = {"Mrays/s": 1e6, "commands/s": 1.0}.get(unit, 1e9)

@@ -469,7 +469,10 @@ class Grid:
         # float boxes of cells and macro blocks, then the 8-byte quantised macro table
         nbox = 8 * (self.info.ncells + self.info.nmacro) + 2 * self.info.nmacro
         self.cell_box = torch.empty(nbox, dtype=torch.float32, device=dev)
-        self.vis_ws = torch.empty(0, dtype=torch.uint8, device=dev)
+        # traversal workspaces (task counters, plane-ray queue, sort), one per
+        # stream: calls on one stream run in order, calls on two streams must
+        # not share the counters
+        self.vis_ws = {}
         check(lib.lem_grid_fill(ptr(self.tris), T, ctypes.byref(self.info), ptr(self.cell_start),
                                 ptr(self.tri_ids), ptr(self.recs), ptr(self.cell_box),
                                 ptr(self.ws), wsb, self.cells_per_tri, stream_ptr()),
@@ -477,13 +480,19 @@ class Grid:
         self.ws = None
 
     def vis_workspace(self, S, K, flags):
-        """lem_visibility_grid's workspace for (S, K, flags), kept and grown."""
+        """lem_visibility_grid's workspace for (S, K, flags) on the current
+        stream, kept and grown per stream (the traversal's task counters and
+        plane-ray queue live in it, so two streams never share one)."""
+        torch = _torch()
         wsb = int(load_library().lem_visibility_grid_workspace(S, K, flags))
         if wsb < 0:
             raise LemError("lem_visibility_grid_workspace: size query failed")
-        if self.vis_ws.numel() < wsb:
-            self.vis_ws = _torch().empty(wsb, dtype=_torch().uint8, device=self.tris.device)
-        return self.vis_ws
+        key = torch.cuda.current_stream().cuda_stream
+        ws = self.vis_ws.get(key)
+        if ws is None or ws.numel() < wsb:
+            ws = torch.empty(wsb, dtype=torch.uint8, device=self.tris.device)
+            self.vis_ws[key] = ws
+        return ws
 
     def visibility(self, src, snk, any_hit=True, want_dist=False, sink_major=False,
                    semantics=SEM_NUMBA, skip_self=False, input_order=False, from_sink=False):
@@ -542,10 +551,23 @@ def field_los_grid(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, beta=2 *
 
 MASK_CHUNK_BYTES = 1 << 31
 
+_POPCOUNT8 = None
+
+
+def count_blocked(bits):
+    """Number of set bits (blocked rays) in a hit-bit tensor; the padding bits
+    of each row's last word are never set."""
+    global _POPCOUNT8
+    torch = _torch()
+    if _POPCOUNT8 is None or _POPCOUNT8.device != bits.device:
+        _POPCOUNT8 = torch.tensor([bin(i).count("1") for i in range(256)], dtype=torch.int64,
+                                  device=bits.device)
+    return int(_POPCOUNT8[bits.contiguous().view(torch.uint8).long()].sum().item())
+
 
 def field_los_grid_masked(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, beta=2 * np.pi,
                           sink_tangent=False, sink_major=True, from_sink=False,
-                          chunk_bytes=MASK_CHUNK_BYTES):
+                          chunk_bytes=MASK_CHUNK_BYTES, stats=None):
     """field_los_grid's result, computed as an any-hit mask from the persistent
     grid traversal followed by the masked field kernel.
 
@@ -556,7 +578,8 @@ def field_los_grid_masked(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, b
     ray from its sink end, which reaches first the blockers that sit next to
     sinks on surfaces (a scatterer behind its own face); the mask is the same
     either way.  The mask is built in chunks (of sinks, or of sources) that stay
-    within chunk_bytes.  Returns (K,3) complex64."""
+    within chunk_bytes.  stats (a dict, optional) receives "blocked", the number
+    of blocked rays.  Returns (K,3) complex64."""
     torch = _torch()
     src = to_dev(src, torch.float32).reshape(-1, 3)
     snk = to_dev(snk, torch.float32).reshape(-1, 3)
@@ -568,34 +591,41 @@ def field_los_grid_masked(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, b
     src_w = to_dev(src_w, torch.complex64).reshape(S, 3)
     if snk_w is not None:
         snk_w = to_dev(snk_w, torch.complex64).reshape(K, 3)
+    blocked = 0
     if sink_major:
         kc = max(1, int(chunk_bytes) // (8 * ((S + 63) // 64)))
         for k0 in range(0, K, kc):
             k1 = min(K, k0 + kc)
             bits, _ = grid.visibility(src, snk[k0:k1], any_hit=True, sink_major=True,
                                       from_sink=from_sink)
+            if stats is not None:
+                blocked += count_blocked(bits)
             out[k0:k1] = field_los(src, src_w, snk[k0:k1], snk_n[k0:k1], hit_bits=bits,
                                    snk_w=None if snk_w is None else snk_w[k0:k1], alpha=alpha,
                                    beta=beta, sink_tangent=sink_tangent, bits_sink_major=True)
-        return out
-    sc = max(1, int(chunk_bytes) // (8 * ((K + 63) // 64)))
-    for s0 in range(0, S, sc):
-        s1 = min(S, s0 + sc)
-        bits, _ = grid.visibility(src[s0:s1], snk, any_hit=True, from_sink=from_sink)
-        part = field_los(src[s0:s1], src_w[s0:s1], snk, snk_n, hit_bits=bits, snk_w=snk_w,
-                         alpha=alpha, beta=beta, sink_tangent=sink_tangent)
-        out = part if s0 == 0 else out + part
+    else:
+        sc = max(1, int(chunk_bytes) // (8 * ((K + 63) // 64)))
+        for s0 in range(0, S, sc):
+            s1 = min(S, s0 + sc)
+            bits, _ = grid.visibility(src[s0:s1], snk, any_hit=True, from_sink=from_sink)
+            if stats is not None:
+                blocked += count_blocked(bits)
+            part = field_los(src[s0:s1], src_w[s0:s1], snk, snk_n, hit_bits=bits, snk_w=snk_w,
+                             alpha=alpha, beta=beta, sink_tangent=sink_tangent)
+            out = part if s0 == 0 else out + part
+    if stats is not None:
+        stats["blocked"] = blocked
     return out
 
 
 def field_los_grid_sink_major(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0,
                               beta=2 * np.pi, sink_tangent=False, chunk_bytes=MASK_CHUNK_BYTES,
-                              from_sink=True):
+                              from_sink=True, stats=None):
     """field_los_grid_masked with a sink-major mask walked from the sink end
     (the one-bounce leg from an aperture to scatterers)."""
     return field_los_grid_masked(grid, src, src_w, snk, snk_n, snk_w=snk_w, alpha=alpha,
                                  beta=beta, sink_tangent=sink_tangent, sink_major=True,
-                                 from_sink=from_sink, chunk_bytes=chunk_bytes)
+                                 from_sink=from_sink, chunk_bytes=chunk_bytes, stats=stats)
 
 
 def grid_stats(grid, src, snk, any_hit=False):

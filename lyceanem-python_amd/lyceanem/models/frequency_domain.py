@@ -164,16 +164,23 @@ def one_bounce_fields(src, src_w, snk, snk_n, scat, scat_n, tris9, scat_w=None, 
         scat_w = np.ones((P, 3), np.complex64)
     if P == 0:
         return np.zeros((K, 3), np.complex128)
+    S = len(src)
     if grid is not None:
         # sources are the (compact) aperture, scatterers are spread over the
         # scene: cast 64 neighbouring sources to one scatterer per wave
+        leg1 = {}
         H = _native.field_los_grid_sink_major(grid, src, src_w, scat, scat_n, snk_w=scat_w,
-                                              alpha=alpha, beta=beta, sink_tangent=True)
+                                              alpha=alpha, beta=beta, sink_tangent=True,
+                                              stats=leg1)
+        blocked = leg1["blocked"]
     else:
         bits = _visibility_bits(src, scat, tris9, None)
+        blocked = _native.count_blocked(bits)
         H = _native.field_los(src, src_w, scat, scat_n, hit_bits=bits, snk_w=scat_w,
                               alpha=alpha, beta=beta, sink_tangent=True)
-    if not bool(H.abs().sum().item() > 0):
+    if blocked == S * P:
+        # every source -> scatterer ray is blocked: workchunkingv2 returns no
+        # rows, line of sight included (rayfunctions.py:2835-2861)
         return np.zeros((K, 3), np.complex128)
     if grid is not None:
         # leg 2: one scatterer to 64 neighbouring sinks per wave; line of sight:

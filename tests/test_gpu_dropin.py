@@ -153,3 +153,48 @@ def test_one_bounce_grid_equals_brute_force_masks(oracle):
                           grid=_native.Grid(tris))
     assert np.abs(a).max() > 0
     assert np.allclose(a, b, rtol=1e-6, atol=1e-6 * np.abs(a).max())
+
+
+@pytest.mark.parametrize("use_grid", [False, True])
+def test_one_bounce_no_visible_scatterer_rule(oracle, use_grid):
+    """workchunkingv2 returns no rows at all, line of sight included, when every
+    source -> scatterer ray is blocked (rayfunctions.py:2835-2861).  The decision
+    comes from the leg-1 mask: scatterers that are visible but carry zero weight
+    keep the line of sight."""
+    from lyceanem import _native
+    from lyceanem.models.frequency_domain import one_bounce_fields
+
+    rng = np.random.default_rng(5)
+    # a lid at z = 1 over the sources; scatterers above it, sinks off to the side
+    lid = np.array([[-5, -5, 1, 5, -5, 1, 0, 5, 1]], np.float32)
+    clutter = (rng.uniform(-3, 3, (40, 1, 3)) * [1, 1, 0.1] + [0, 0, 4] +
+               rng.uniform(-0.2, 0.2, (40, 3, 3))).reshape(-1, 9).astype(np.float32)
+    tris = np.concatenate([lid, clutter])
+    src = (rng.uniform(-0.3, 0.3, (20, 3)) * [1, 1, 0]).astype(np.float32)
+    w = (rng.normal(size=(20, 3)) + 1j * rng.normal(size=(20, 3))).astype(np.complex64)
+    scat = (rng.uniform(-2, 2, (30, 3)) * [1, 1, 0] + [0, 0, 2]).astype(np.float32)
+    scat_n = np.tile(np.array([[0, 0, -1]], np.float32), (30, 1))
+    snk = (rng.uniform(-1, 1, (25, 3)) * [1, 1, 0] + [40, 0, 0.5]).astype(np.float32)
+    snk_n = np.tile(np.array([[-1, 0, 0]], np.float32), (25, 1))
+    beta = 2 * np.pi / 0.3
+    grid = _native.Grid(tris) if use_grid else None
+    full, _ = oracle.workchunking(src, snk, scat, tris, 2)
+    assert full.shape[0] == 0  # the lid blocks every source -> scatterer ray
+    E = one_bounce_fields(src, w, snk, snk_n, scat, scat_n, tris, beta=beta, grid=grid)
+    assert np.array_equal(E, np.zeros_like(E))
+    # no lid: scatterers visible, but with zero weight every H_p is zero; the
+    # line-of-sight rows stay (the oracle's path sum)
+    tris2 = clutter
+    grid2 = _native.Grid(tris2) if use_grid else None
+    full2, _ = oracle.workchunking(src, snk, scat, tris2, 2)
+    assert (full2[:, 2] != 0).any() and (full2[:, 2] == 0).any()
+    zero = np.zeros((30, 3), np.complex64)
+    E2 = one_bounce_fields(src, w, snk, snk_n, scat, scat_n, tris2, scat_w=zero, beta=beta,
+                           grid=grid2)
+    S, K = len(src), len(snk)
+    pos = np.concatenate([src, snk, scat])
+    nrm = np.concatenate([np.tile([[0, 0, 1.0]], (S, 1)), snk_n, scat_n]).astype(np.float32)
+    wt = np.concatenate([w, np.ones((K, 3), np.complex64), zero])
+    ref = oracle.field_paths(pos, nrm, wt, full2, S, K, 0.0, beta).sum(0)
+    assert np.abs(ref).max() > 0
+    assert np.linalg.norm(E2 - ref) / np.linalg.norm(ref) < 1e-5

@@ -232,30 +232,61 @@ def test_grid_flat_terrain_vs_oracle(native, oracle):
     assert np.array_equal(native.unpack_bits(bits_f, len(src)).T, hit)
 
 
-def test_grid_terrain_full_size_sampled(native, oracle):
-    """Full 1,000,000-triangle terrain: the grid (sink-major, as benchmarked) on
-    a 64 x 64 source patch x all sinks, checked on sampled rays against the
-    oracle's brute force over every triangle."""
+def _sample_bits(bits, rows, cols):
+    """Bits (rows[i], cols[i]) of a (rows, words) int64 device tensor -> bool numpy."""
+    import torch
+
+    r = torch.from_numpy(rows).to(bits.device)
+    c = torch.from_numpy(cols).to(bits.device)
+    w = bits[r, c >> 6]
+    return ((w >> (c & 63)) & 1).bool().cpu().numpy()
+
+
+def test_grid_terrain_full_launch_sampled(native, oracle):
+    """The benchmarked launch itself: all 1,048,576 sources x 16,384 sinks of
+    the 1,000,000-triangle terrain (sink-major, closest hit, mask + distance).
+    1,000,000 random rays of it are compared with the brute-force kernel over
+    every triangle (lem_raycast_pairs, itself bit-exact against the oracle),
+    and 10,000 of them with the oracle directly; mismatch counts are printed."""
+    import torch
     from lyceanem import scenes
 
     sc = scenes.terrain_c2()
     src, snk, tris = sc["src"], sc["snk"], sc["tris"]
+    S, K = len(src), len(snk)
     assert tris.shape == (1000000, 9)
-    rng = np.random.default_rng(1)
-    sel = rng.choice(len(src), 4096, replace=False)
     g = native.Grid(tris)
-    print(f"terrain grid: {tuple(g.info.n)} cells, {g.info.nrefs} refs", flush=True)
-    bits, dist = g.visibility(src[sel], snk, any_hit=False, want_dist=True, sink_major=True)
-    m = native.unpack_bits(bits, len(sel)).T
-    d = dist.cpu().numpy().T
-    pairs = np.stack([rng.integers(0, len(sel), 1500), rng.integers(0, len(snk), 1500)], 1)
-    hit, odist = oracle.visibility_sampled(src[sel], snk, pairs, tris)
-    assert np.array_equal(m[pairs[:, 0], pairs[:, 1]], hit)
-    assert np.array_equal(d[pairs[:, 0], pairs[:, 1]], odist)
-    assert 0.02 < hit.mean() < 0.98
-    # any-hit and closest-hit masks agree at full size
-    bits_a, _ = g.visibility(src[sel], snk, any_hit=True, sink_major=True)
-    assert np.array_equal(native.unpack_bits(bits_a, len(sel)).T, m)
+    print(f"terrain grid: {tuple(g.info.n)} cells, {g.info.nrefs} refs; full launch "
+          f"{S} x {K}", flush=True)
+    bits, dist = g.visibility(src, snk, any_hit=False, want_dist=True, sink_major=True)
+    rng = np.random.default_rng(1)
+    n = 1_000_000
+    s_i, k_i = rng.integers(0, S, n), rng.integers(0, K, n)
+    m_grid = _sample_bits(bits, k_i, s_i)
+    d_grid = dist[torch.from_numpy(k_i).to(dist.device),
+                  torch.from_numpy(s_i).to(dist.device)].cpu().numpy()
+    # any-hit mask of the same launch
+    del dist
+    bits_a, _ = g.visibility(src, snk, any_hit=True, sink_major=True)
+    m_any = _sample_bits(bits_a, k_i, s_i)
+    del bits, bits_a
+    torch.cuda.empty_cache()
+    print("terrain full launch: brute force on the sample", flush=True)
+    pts = np.concatenate([src, snk])
+    pairs = np.stack([s_i, S + k_i], 1).astype(np.int32)
+    inter, bd = native.raycast_pairs(pts, pairs, tris, any_hit=False, want_dist=True)
+    m_b, d_b = inter.cpu().numpy().astype(bool), bd.cpu().numpy()
+    mm, dm, am = int((m_grid != m_b).sum()), int((d_grid != d_b).sum()), \
+        int((m_any != m_b).sum())
+    print(f"terrain full launch: {n} rays vs brute force: {mm} mask, {dm} distance, "
+          f"{am} any-hit mask mismatches (blocked {m_b.mean():.4f})", flush=True)
+    no = 10_000
+    hit, odist = oracle.visibility_sampled(src, snk, np.stack([s_i[:no], k_i[:no]], 1), tris)
+    om, od = int((m_grid[:no] != hit).sum()), int((d_grid[:no] != odist).sum())
+    print(f"terrain full launch: {no} rays vs the oracle: {om} mask, {od} distance "
+          f"mismatches", flush=True)
+    assert 0.02 < m_b.mean() < 0.98
+    assert (mm, dm, am, om, od) == (0, 0, 0, 0, 0)
 
 
 def test_fused_grid_field_matches_masked_field(native):

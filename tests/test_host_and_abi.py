@@ -76,6 +76,10 @@ def test_ray_hits_split(oracle):
     f, g = RF.rayHits1Dv2(inter, idx, sinks)
     f2, g2 = oracle.ray_hits(inter, idx, sinks)
     assert np.array_equal(f, f2) and np.array_equal(g, g2)
+    # literal split (rayfunctions.py:1845-1878): the blocked row [2, 6] is
+    # dropped, rows ending on a sink (5..7) are final, the rest continue
+    assert np.array_equal(f2, [[1, 8], [2, 9]])
+    assert np.array_equal(g2, [[1, 5], [3, 7]])
 
 
 def test_targetting_index_and_slots(oracle):
@@ -85,6 +89,13 @@ def test_targetting_index_and_slots(oracle):
     d, sc = EM.targettingindex(fi)
     d2, sc2 = oracle.targetting_index(fi)
     assert np.array_equal(d, d2) and np.array_equal(sc, sc2)
+    # literal answer of empropagation.py:3394-3429: line-of-sight rows first
+    # (columns 0, 1), then the scattered rows (columns 0, 2); depth 2 marks the
+    # rows with a scatterer, in input order
+    assert np.array_equal(d2, [[1, 4], [2, 5], [3, 6], [1, 4], [2, 5]])
+    assert np.array_equal(sc2, [1, 1, 2, 2, 1])
+    d2, sc2 = oracle.targetting_index(fi[:, :2])
+    assert np.array_equal(d2, fi[:, :2]) and np.array_equal(sc2, np.ones(5))
     s, k = EM.path_slots(fi, 3, 3)
     assert np.array_equal(s, [0, 1, 0, 1, 2])
     assert np.array_equal(k, [0, 1, 0, 1, 2])
@@ -104,6 +115,15 @@ def test_spherical_field_order_and_angles(oracle):
     th, ph, r = oracle.theta_phi(m.points)
     assert np.allclose(m.point_data["theta_(Radians)"], th)
     assert np.allclose(m.point_data["phi_(Radians)"], ph)
+    # closed form of theta_phi_r (geometryfunctions.py:447-470) on this grid:
+    # theta = pi/2 - el, phi = az (az = +-180 both map to atan2's +-pi; the
+    # poles have no azimuth), r = the field radius
+    elg, azg = np.meshgrid(np.radians(el), np.radians(az), indexing="ij")
+    assert np.allclose(th.reshape(5, 7), np.pi / 2 - elg, atol=1e-6)
+    assert np.allclose(r, 2.0)
+    inner = (slice(1, 4), slice(1, 6))
+    assert np.allclose(ph.reshape(5, 7)[inner], azg[inner], atol=1e-6)
+    assert np.allclose(np.abs(ph.reshape(5, 7)[1:4, [0, 6]]), np.pi, atol=1e-6)
     inward = spherical_field(az, el, outward_normals=False)
     assert np.allclose(inward.point_data["Normals"], -inward.points)
 
