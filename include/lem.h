@@ -130,7 +130,11 @@ int lem_field_los(const float *src, const float *src_w, int64_t S, const float *
  * `order` lists the path rows slot by slot, seg_start (nseg + 1) bounds each
  * slot's run, seg_slot = s * K + k.  out: S x K x 3 complex128, only the listed
  * slots are written (zero it first).  err_flag (device int, zeroed by the caller)
- * is set to 1 if a row points outside the table. */
+ * is set to 1 if a row points outside the table.  ncol 2 to 4: a zero entry at
+ * column i >= 1 reads the last point of the table, as freqdomainkernal's
+ * negative index does under numba (a line-of-sight row [s, k, 0, 0] of the
+ * 4-column index is projected onto that point's tangent plane after the
+ * sink's). */
 int lem_field_paths(const float *pos, const float *nrm, const float *w, int64_t N,
                     const int32_t *index, int ncol, const int64_t *order,
                     const int64_t *seg_start, const int64_t *seg_slot, int64_t nseg,

@@ -4,9 +4,15 @@
 // float32 division / square root, so each float32 expression below is
 // evaluated exactly as written.  That is what makes the hit masks
 // reproducible bit for bit against the float32 reference arithmetic.
+//
+// The same code also compiles as host C++ for the CPU build of the grid walk
+// (LEM_HOST_BUILD): that build supplies LEM_DEV, the HIP vector types and the
+// three hardware estimates below.
 #pragma once
 
+#ifndef LEM_HOST_BUILD
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 #ifndef LEM_DEV
@@ -14,6 +20,14 @@
 #endif
 
 namespace lem {
+
+#ifndef LEM_HOST_BUILD
+// Hardware estimates (v_rcp_f64, v_rcp_f32, v_sqrt_f32); every use refines them
+// or tolerates their ~1 ulp error.
+LEM_DEV double rcp_estimate(double a) { return __builtin_amdgcn_rcp(a); }
+LEM_DEV float rcpf_estimate(float a) { return __builtin_amdgcn_rcpf(a); }
+LEM_DEV float sqrtf_estimate(float a) { return __builtin_amdgcn_sqrtf(a); }
+#endif
 
 // ---------------------------------------------------------------------------
 // Ray charging (calc_dv_norm, math_functions.py:148-156; points_to_rays,
@@ -79,7 +93,7 @@ constexpr float NUMBA_EPS_BELOW = 9.999999406e-08f;
 LEM_DEV double recip_f32_to_f64(float a)
 {
     const double ad = (double)a;
-    double y = __builtin_amdgcn_rcp(ad);
+    double y = rcp_estimate(ad);
     y = __builtin_fma(y, __builtin_fma(-ad, y, 1.0), y);
     return __builtin_fma(y, __builtin_fma(-ad, y, 1.0), y);
 }
@@ -230,13 +244,13 @@ LEM_DEV Phase phase_of(double dx, double dy, double dz, double beta)
 {
     Phase p;
     double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
-    float r0 = __builtin_amdgcn_sqrtf((float)r2);   // ~1 ulp seed
-    float h = 0.5f * __builtin_amdgcn_rcpf(r0);
+    float r0 = sqrtf_estimate((float)r2);   // ~1 ulp seed
+    float h = 0.5f * rcpf_estimate(r0);
     double r0d = (double)r0;
     double e = __builtin_fma(-r0d, r0d, r2);
     double r = __builtin_fma(e, (double)h, r0d);
     p.r = r;
-    p.inv_r = __builtin_amdgcn_rcpf((float)r);
+    p.inv_r = rcpf_estimate((float)r);
     double ph = -beta * r;
     const double TWO_OVER_PI = 0.63661977236758134308;
     const double PIO2_HI = 1.5707963267948966;     // float64(pi/2)

@@ -186,13 +186,15 @@ __global__ __launch_bounds__(FB) void k_field_paths(
         C3 E = load_w(w, row[0] - 1);
         float lr = 1.0f, li = 0.0f;
         for (int i = 0; i < ncol - 1; ++i) {
-            int64_t a = row[i] - 1;
+            // a zero entry (no point) reads the last point of the table, as
+            // freqdomainkernal's network_index - 1 = -1 does under numba's
+            // negative-index wrap (empropagation.py:1147-1156): line-of-sight
+            // rows of a 4-column index are projected onto that point's tangent
+            // plane after the sink's
+            int64_t a = row[i] > 0 ? row[i] - 1 : N - 1;
             int32_t b = row[i + 1];
-            if (i >= 1) {
-                if (a < 0)
-                    break;
+            if (i >= 1)
                 E = project_normal(E, nrm[3 * a], nrm[3 * a + 1], nrm[3 * a + 2]);
-            }
             if (b != 0) {
                 int64_t bb = b - 1;
                 double dx = (double)pos[3 * bb] - (double)pos[3 * a];

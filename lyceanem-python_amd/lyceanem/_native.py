@@ -565,9 +565,20 @@ def count_blocked(bits):
     return int(_POPCOUNT8[bits.contiguous().view(torch.uint8).long()].sum().item())
 
 
+def block_same_point(bits, first, n):
+    """Mark the rays i -> i blocked in rows [first, first + n) of a mask whose
+    sources and sinks are one point set (row r holds point first + r; its
+    column is the point's own index): the path list never pairs a scattering
+    point with itself (create_model_index drops such rows)."""
+    torch = _torch()
+    i = torch.arange(first, first + n, device=bits.device, dtype=torch.int64)
+    bits[i - first, i >> 6] |= torch.bitwise_left_shift(torch.ones_like(i), i & 63)
+    return bits
+
+
 def field_los_grid_masked(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, beta=2 * np.pi,
                           sink_tangent=False, sink_major=True, from_sink=False,
-                          chunk_bytes=MASK_CHUNK_BYTES, stats=None):
+                          chunk_bytes=MASK_CHUNK_BYTES, stats=None, same_points=False):
     """field_los_grid's result, computed as an any-hit mask from the persistent
     grid traversal followed by the masked field kernel.
 
@@ -579,7 +590,8 @@ def field_los_grid_masked(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, b
     sinks on surfaces (a scatterer behind its own face); the mask is the same
     either way.  The mask is built in chunks (of sinks, or of sources) that stay
     within chunk_bytes.  stats (a dict, optional) receives "blocked", the number
-    of blocked rays.  Returns (K,3) complex64."""
+    of blocked rays.  same_points: src and snk are the same point set and the
+    pairs i -> i are left out.  Returns (K,3) complex64."""
     torch = _torch()
     src = to_dev(src, torch.float32).reshape(-1, 3)
     snk = to_dev(snk, torch.float32).reshape(-1, 3)
@@ -598,6 +610,8 @@ def field_los_grid_masked(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, b
             k1 = min(K, k0 + kc)
             bits, _ = grid.visibility(src, snk[k0:k1], any_hit=True, sink_major=True,
                                       from_sink=from_sink)
+            if same_points:
+                block_same_point(bits, k0, k1 - k0)
             if stats is not None:
                 blocked += count_blocked(bits)
             out[k0:k1] = field_los(src, src_w, snk[k0:k1], snk_n[k0:k1], hit_bits=bits,
@@ -608,6 +622,8 @@ def field_los_grid_masked(grid, src, src_w, snk, snk_n, snk_w=None, alpha=0.0, b
         for s0 in range(0, S, sc):
             s1 = min(S, s0 + sc)
             bits, _ = grid.visibility(src[s0:s1], snk, any_hit=True, from_sink=from_sink)
+            if same_points:
+                block_same_point(bits, s0, s1 - s0)
             if stats is not None:
                 blocked += count_blocked(bits)
             part = field_los(src[s0:s1], src_w[s0:s1], snk, snk_n, hit_bits=bits, snk_w=snk_w,

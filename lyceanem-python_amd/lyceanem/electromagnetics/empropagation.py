@@ -105,8 +105,9 @@ def vector_mapping(local_E_vector, point_normal, rotation_matrix):
     return np.matmul(local_E_vector, uvn)
 
 
-def calculate_conformalVectors(desired_E_vector, source_normals, antenna_axes=np.eye(3)):
-    """Per-point conformal excitation vectors (empropagation.py:3807-3849)."""
+def calculate_conformalVectors(desired_E_vector, source_normals, antenna_axes):
+    """Per-point conformal excitation vectors (empropagation.py:3807-3849);
+    antenna_axes is required, as in the reference."""
     normals = np.asarray(source_normals)
     out = np.zeros((normals.shape[0], 3), dtype=np.complex64)
     dv = np.asarray(desired_E_vector)

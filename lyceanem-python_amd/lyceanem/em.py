@@ -41,42 +41,43 @@ def _check1d(a, name):
 
 
 def _tile_bins(tri_vertex, tri_index, ncells_x, ncells_y, ymin, cell_width, z_min):
-    """Bins of each triangle (terrain_acceleration_build.cuh:30-113): the (y, z)
-    cells of its three vertices, plus the fourth cell of an L-shaped triple."""
+    """(T, 4) int64 tile bins of each triangle, -1 where absent
+    (terrain_acceleration_build.cuh:30-113): the (y, z) cells of its three
+    vertices (float32 floor((v - min) / width), as bin_point computes them) in
+    vertex order without repeats, plus the cell completing the 2 x 2 block of
+    an L-shaped triple.  Vertices outside the bounds, or a triangle whose
+    vertex cells are not adjacent (check_triangle_side_length, :114-134, a
+    device assert there), raise RuntimeError."""
     v = np.asarray(tri_vertex, np.float32)
-    t = np.asarray(tri_index, np.int64)
+    t = np.asarray(tri_index, np.int64).reshape(-1, 3)
+    ny = int(ncells_y)
     by = np.floor((v[:, 1] - np.float32(ymin)) / np.float32(cell_width)).astype(np.int64)
     bz = np.floor((v[:, 2] - np.float32(z_min)) / np.float32(cell_width)).astype(np.int64)
-    if (by < 0).any() or (by >= ncells_x).any() or (bz < 0).any() or (bz >= ncells_y).any():
+    if (by < 0).any() or (by >= ncells_x).any() or (bz < 0).any() or (bz >= ny).any():
         raise RuntimeError("triangle vertices outside the provided y-z bounds")
-    b = by * ncells_y + bz
-    out = []
-    for i in range(len(t)):
-        bx, byy, bzz = b[t[i, 0]], b[t[i, 1]], b[t[i, 2]]
-        cells = [bx]
-        if byy != bx:
-            cells.append(byy)
-        if bzz != bx and bzz != byy:
-            cells.append(bzz)
-        if bzz != bx and bzz != byy and byy != bx:
-            # the cell completing the 2x2 block: for each pair of vertex cells
-            # sharing a row / column, take the remaining vertex's coordinate
-            x0, x1, x2 = [(c // ncells_y, c % ncells_y) for c in (bx, byy, bzz)]
-            r = [-1, -1]
-            if x0[0] == x1[0]:
-                r[0] = x2[0]
-            if x0[1] == x1[1]:
-                r[1] = x2[1]
-            if x0[0] == x2[0]:
-                r[0] = x1[0]
-            if x0[1] == x2[1]:
-                r[1] = x1[1]
-            if x1[0] == x2[0]:
-                r[0] = x0[0]
-            if x1[1] == x2[1]:
-                r[1] = x0[1]
-            cells.append(r[0] * ncells_y + r[1])
-        out.append(cells)
+    b = by * ny + bz
+    b0, b1, b2 = b[t[:, 0]], b[t[:, 1]], b[t[:, 2]]
+    (r0, c0), (r1, c1), (r2, c2) = divmod(b0, ny), divmod(b1, ny), divmod(b2, ny)
+    near = lambda ra, ca, rb, cb: (np.abs(ra - rb) <= 1) & (np.abs(ca - cb) <= 1)  # noqa: E731
+    if not (near(r0, c0, r1, c1) & near(r0, c0, r2, c2) & near(r1, c1, r2, c2)).all():
+        raise RuntimeError("triangle vertices not in adjacent tiles: a triangle edge is "
+                           "longer than one tile")
+    out = np.full((len(t), 4), -1, np.int64)
+    out[:, 0] = b0
+    out[:, 1] = np.where(b1 != b0, b1, -1)
+    third = (b2 != b0) & (b2 != b1)
+    out[:, 2] = np.where(third, b2, -1)
+    # the fourth cell: a coordinate shared by two vertex cells is taken from the
+    # third one, pair by pair in the reference's order
+    rx = np.full(len(t), -1, np.int64)
+    ry = np.full(len(t), -1, np.int64)
+    for (ra, ca), (rb, cb), (rc, cc) in (((r0, c0), (r1, c1), (r2, c2)),
+                                         ((r0, c0), (r2, c2), (r1, c1)),
+                                         ((r1, c1), (r2, c2), (r0, c0))):
+        rx = np.where(ra == rb, rc, rx)
+        ry = np.where(ca == cb, cc, ry)
+    four = third & (b1 != b0)
+    out[four, 3] = rx[four] * ny + ry[four]
     return out
 
 
@@ -84,35 +85,29 @@ def bin_counts_to_numpy(tri_vertex, tri_index, ncells_x, ncells_y, ymin, cell_wi
     """Triangles per (y, z) tile (terrain_acceleration_build.cuh:296-348)."""
     _check2d3(tri_vertex, "tri_vertex")
     _check2d3(tri_index, "tri_index")
-    counts = np.zeros(ncells_x * ncells_y, np.int32)
-    for cells in _tile_bins(tri_vertex, tri_index, ncells_x, ncells_y, ymin, cell_width, z_min):
-        for c in cells:
-            counts[c] += 1
-    return counts
+    bins = _tile_bins(tri_vertex, tri_index, ncells_x, ncells_y, ymin, cell_width, z_min)
+    return np.bincount(bins[bins >= 0], minlength=ncells_x * ncells_y).astype(np.int32)
 
 
 def bin_triangles_to_numpy(tri_vertex, tri_index, ncells_x, ncells_y, ymin, cell_width, z_min,
                            bin_counts, sum_bin_counts):
     """Vertex-index triples of the binned triangles, tile by tile
-    (terrain_acceleration_build.cuh:370-431); within a tile in triangle order."""
+    (terrain_acceleration_build.cuh:370-431); within a tile in triangle order
+    (the reference fills tiles with atomics, in no defined order).  Entries
+    beyond the binned count stay -1."""
     _check2d3(tri_vertex, "tri_vertex")
     t = _check2d3(tri_index, "tri_index")
     bc = np.asarray(bin_counts)
     if bc.ndim != 1 or bc.shape[0] != ncells_x * ncells_y:
         raise RuntimeError("bincounts is a 1d flattened 2d array thus must have 1d with "
                            "ncountsy *ncountz")
-    per = [[] for _ in range(ncells_x * ncells_y)]
-    for i, cells in enumerate(_tile_bins(tri_vertex, t, ncells_x, ncells_y, ymin, cell_width,
-                                         z_min)):
-        for c in cells:
-            per[c].append(i)
+    bins = _tile_bins(tri_vertex, t, ncells_x, ncells_y, ymin, cell_width, z_min)
+    tri, slot = np.nonzero(bins >= 0)  # triangle-major, so a stable sort keeps triangle order
+    order = np.argsort(bins[tri, slot], kind="stable")
+    rows = np.asarray(t, np.int32)[tri[order]]
     out = np.full((int(sum_bin_counts), 3), -1, np.int32)
-    pos = 0
-    for lst in per:
-        for i in lst:
-            if pos < len(out):
-                out[pos] = t[i]
-            pos += 1
+    n = min(len(out), len(rows))
+    out[:n] = rows[:n]
     return out.ravel()
 
 

@@ -50,9 +50,31 @@ def theta_phi_r(field_data):
     return field_data
 
 
+def _copy_mesh(mesh):
+    """A new mesh with the same points, cells and data (the reference's area and
+    normal helpers return the pyvista round trip's new mesh, leaving theirs
+    unchanged)."""
+    return Mesh(points=np.array(mesh.points, copy=True), cells=list(mesh.cells),
+                point_data=dict(mesh.point_data), cell_data=dict(mesh.cell_data))
+
+
+def locate_cell_index(field_data, cell_type="triangle"):
+    """Index of the last cell block of cell_type (geometryfunctions.py:228-252:
+    the loop keeps overwriting the index, so the last block wins)."""
+    desired = None
+    for inc, cell in enumerate(field_data.cells):
+        if cell.type == cell_type:
+            desired = inc
+    if desired is None:
+        raise ValueError(f"no {cell_type} cells in the mesh")
+    return desired
+
+
 def compute_areas(field_data):
-    """Triangle areas as cell data and one third of each adjacent triangle's area
-    per point as point data."""
+    """A copy of the mesh with triangle areas as cell data and one third of
+    each adjacent triangle's area per point as point data
+    (geometryfunctions.py:255-356)."""
+    field_data = _copy_mesh(field_data)
     tri = triangle_cells(field_data)
     p = np.asarray(field_data.points, np.float64)
     if len(tri):
@@ -63,13 +85,24 @@ def compute_areas(field_data):
     pa = np.zeros(len(p))
     for j in range(3):
         np.add.at(pa, tri[:, j], a / 3.0)
-    field_data.cell_data["Area"] = [a]
+    field_data.cell_data["Area"] = _per_block(field_data, a)
     field_data.point_data["Area"] = pa
     return field_data
 
 
+def _per_block(mesh, values):
+    """Cell data list: `values` for the triangle block, empty arrays for others."""
+    idx = locate_cell_index(mesh) if any(c.type == "triangle" for c in mesh.cells) else None
+    if idx is None:
+        return [values]
+    return [values if i == idx else np.zeros((len(c.data),) + values.shape[1:])
+            for i, c in enumerate(mesh.cells)]
+
+
 def compute_normals(mesh):
-    """Unit triangle normals (cell data) and area-weighted unit point normals."""
+    """A copy of the mesh with unit triangle normals (cell data) and
+    area-weighted unit point normals (geometryfunctions.py:359-444)."""
+    mesh = _copy_mesh(mesh)
     tri = triangle_cells(mesh)
     p = np.asarray(mesh.points, np.float64)
     n = np.cross(p[tri[:, 1]] - p[tri[:, 0]], p[tri[:, 2]] - p[tri[:, 0]])
@@ -79,15 +112,17 @@ def compute_normals(mesh):
     with np.errstate(invalid="ignore", divide="ignore"):
         cn = n / np.linalg.norm(n, axis=1, keepdims=True)
         pn = pn / np.linalg.norm(pn, axis=1, keepdims=True)
-    mesh.cell_data["Normals"] = [cn]
+    mesh.cell_data["Normals"] = _per_block(mesh, cn)
     mesh.point_data["Normals"] = pn
     return mesh
 
 
 def cell_centroids(field_data):
-    """Triangle centroids as a point cloud carrying the cell data as point data."""
+    """Triangle centroids as a point cloud carrying the triangle block's cell
+    data as point data (geometryfunctions.py:45-92)."""
     tri = triangle_cells(field_data)
     p = np.asarray(field_data.points, np.float64)
     c = (1.0 / 3.0) * (p[tri[:, 0]] + p[tri[:, 1]] + p[tri[:, 2]])
-    pd = {k: np.asarray(v[-1]) for k, v in field_data.cell_data.items()}
+    idx = locate_cell_index(field_data)
+    pd = {k: np.asarray(v[idx]) for k, v in field_data.cell_data.items()}
     return Mesh(points=c, cells=[("vertex", np.arange(len(c)).reshape(-1, 1))], point_data=pd)

@@ -50,8 +50,13 @@ def _scatter_branch_weights(aperture_coords, desired_E_axis, project_vectors, nu
     imaginary part is dropped), and does not multiply by Area."""
     desired = np.atleast_2d(np.asarray(desired_E_axis))
     if project_vectors:
-        normals = np.asarray(aperture_coords.point_data["Normals"]).astype(np.float32)
-        return EM.calculate_conformalVectors(desired[0, :].reshape(1, 3), normals)
+        # frequency_domain.py:456-459 passes two of calculate_conformalVectors'
+        # three arguments (empropagation.py:3807), which raises there
+        raise TypeError("calculate_scattering(scattering > 0, project_vectors=True): the "
+                        "reference calls calculate_conformalVectors without antenna_axes "
+                        "(frequency_domain.py:456-459) and raises; project the vectors with "
+                        "empropagation.calculate_conformalVectors and pass them as "
+                        "desired_E_axis instead")
     with np.errstate(all="ignore"):
         row = np.real(desired[0, :]).astype(np.float32).reshape(1, 3)
     return np.repeat(row, num_sources, axis=0).astype(np.complex64)
@@ -106,15 +111,20 @@ def calculate_scattering(aperture_coords, sink_coords, antenna_solid, desired_E_
         out = out.cpu().numpy().astype(np.complex128)
         return out[..., 0], out[..., 1], out[..., 2]
 
-    if scattering != 1:
-        raise NotImplementedError("calculate_scattering: scattering 0 and 1 are supported")
+    if scattering not in (1, 2):
+        # workchunkingv2 defines max_scatter 1 to 3 only (rayfunctions.py:2745-2953)
+        raise NotImplementedError("calculate_scattering: scattering 0, 1 and 2 are supported")
     scat = np.asarray(scatter_points.points).astype(np.float32)
     scat_n = np.asarray(scatter_points.point_data["Normals"]).astype(np.float32)
     P = scat.shape[0]
     w = _scatter_branch_weights(aperture_coords, desired_E_axis, project_vectors, S)
     if not elements:
-        E = one_bounce_fields(src, w, snk, snk_n, scat, scat_n, tris9, alpha=alpha, beta=beta,
-                              los=los)
+        if scattering == 1:
+            E = one_bounce_fields(src, w, snk, snk_n, scat, scat_n, tris9, alpha=alpha,
+                                  beta=beta, los=los)
+        else:
+            E = two_bounce_fields(src, w, snk, snk_n, scat, scat_n, tris9, alpha=alpha,
+                                  beta=beta, los=los)
         return E[:, 0], E[:, 1], E[:, 2]
     # per-source network: the reference's path list (S x P x K rows at most)
     src_n = np.asarray(aperture_coords.point_data["Normals"]).astype(np.float32)
@@ -200,6 +210,77 @@ def one_bounce_fields(src, src_w, snk, snk_n, scat, scat_n, tris9, scat_w=None, 
     return E.cpu().numpy().astype(np.complex128)
 
 
+def _project_plane(E, n):
+    """E - n (n . E) for a unit-normalised n: the tangent-plane projector
+    freqdomainkernal applies at a point (sourcelaunchtransformGPU with the
+    point's normal, empropagation.py:178-249, 1147-1156)."""
+    n = np.asarray(n, np.float64).reshape(3)
+    n = n / np.linalg.norm(n)
+    nt = _native.to_dev(n).to(E.dtype)
+    return E - nt[None, :] * (E * nt[None, :]).sum(-1, keepdim=True)
+
+
+def two_bounce_fields(src, src_w, snk, snk_n, scat, scat_n, tris9, scat_w=None, alpha=0.0,
+                      beta=2 * np.pi, los=True, grid=None):
+    """Source-summed fields of every path calculate_scattering(scattering=2)
+    builds -- workchunkingv2 with max_scatter 3 (rayfunctions.py:2863-2953):
+    rows [s, k, 0, 0], [s, p, k, 0] and [s, p1, p2, k] -- through
+    freqdomainkernal (empropagation.py:1083-1265), evaluated leg by leg:
+
+        H_p = P(n_p) (sum_s vis(s,p) L_sp P(d_sp) w_s * w_p)            at scatterers
+        G_q = P(n_q) (sum_{p != q} vis(p,q) L_pq P(d_pq) H_p * w_q)     second bounce
+        E_k = P(n_N) P(n_k) sum_s vis(s,k) L_sk P(d_sk) w_s             line of sight
+            + P(n_k) sum_p vis(p,k) L_pk P(d_pk) H_p                    one bounce
+            + sum_q vis(q,k) L_qk P(d_qk) G_q                           two bounces
+
+    P(n) = I - n n^T.  In a 4-column row the kernel visits columns 1 and 2 as
+    intermediate points: the sink of a one-bounce row sits in column 2 and is
+    projected onto its tangent plane, and the zero column 2 of a line-of-sight
+    row reads point -1 (numba wraps the negative index), so those rows are also
+    projected with n_N, the normal of the last point of the table (the last
+    scatterer).  Unlike max_scatter 2, the line of sight is kept when no
+    scatterer is visible.  Returns (K, 3) complex128."""
+    tris9 = np.asarray(tris9, np.float32).reshape(-1, 9)
+    T = tris9.shape[0]
+    if grid is None and T >= GRID_MIN_TRIANGLES:
+        grid = _native.Grid(tris9)
+    P, K = len(scat), len(snk)
+    if K == 0:
+        return np.zeros((0, 3), np.complex128)
+    if scat_w is None:
+        scat_w = np.ones((P, 3), np.complex64)
+    # the last row of the point table (sources, sinks, scatterers)
+    n_last = scat_n[-1] if P else snk_n[-1]
+
+    def leg(a, a_w, b, b_n, b_w=None, tangent=False, sink_major=True, from_sink=False,
+            same=False):
+        if grid is not None:
+            return _native.field_los_grid_masked(grid, a, a_w, b, b_n, snk_w=b_w, alpha=alpha,
+                                                 beta=beta, sink_tangent=tangent,
+                                                 sink_major=sink_major, from_sink=from_sink,
+                                                 same_points=same)
+        bits = _visibility_bits(a, b, tris9, None)
+        if same:
+            _native.block_same_point(bits, 0, len(a))
+        return _native.field_los(a, a_w, b, b_n, hit_bits=bits, snk_w=b_w, alpha=alpha,
+                                 beta=beta, sink_tangent=tangent)
+
+    E = None
+    if los:
+        E = _project_plane(leg(src, src_w, snk, snk_n, tangent=True), n_last)
+    if P:
+        # array -> scatterers: 64 neighbouring elements to one scatterer per wave
+        H = leg(src, src_w, scat, scat_n, b_w=scat_w, tangent=True, from_sink=True)
+        # scatterers -> scatterers: one scatterer to 64 neighbouring ones per wave
+        G = leg(scat, H, scat, scat_n, b_w=scat_w, tangent=True, sink_major=False, same=True)
+        E1 = leg(scat, H, snk, snk_n, tangent=True, sink_major=False)
+        E2 = leg(scat, G, snk, snk_n, sink_major=False)
+        E = E1 + E2 if E is None else E + E1 + E2
+    if E is None:
+        return np.zeros((K, 3), np.complex128)
+    return E.cpu().numpy().astype(np.complex128)
+
+
 def calculate_farfield(aperture_coords, antenna_solid, desired_E_axis, az_range=None,
                        el_range=None, scatter_points=None, wavelength=1.0, farfield_distance=2.0,
                        scattering=0, elements=False, los=True, project_vectors=False, cuda=False,
@@ -237,9 +318,16 @@ def aperture_projection(aperture, environment=None, wavelength=1.0, az_range=Non
     if elev_range is None:
         elev_range = np.linspace(-90.0, 90.0, 19)
     blocking = GF.mesh_conversion(aperture if environment is None else environment)
-    aperture = GF.compute_normals(GF.compute_areas(aperture))
+    # the reference's order (frequency_domain.py:63-80): the centroid cloud takes
+    # the aperture's own cell data (its "Area" weights the projected area), the
+    # normals come from the area / normal helpers' copy of the mesh
     centroids = GF.cell_centroids(aperture)
-    normals = np.asarray(aperture.cell_data["Normals"][-1])
+    if "Area" not in centroids.point_data:
+        raise KeyError("aperture_projection weights each triangle by the aperture's own cell "
+                       "data 'Area' (frequency_domain.py:63-80); compute it first, e.g. with "
+                       "geometryfunctions.compute_areas")
+    meshed = GF.compute_normals(GF.compute_areas(aperture))
+    normals = np.asarray(meshed.cell_data["Normals"][GF.locate_cell_index(meshed)])
     centroids.points = centroids.points + EPSILON * normals
     pattern, pcd = RF.visiblespace(centroids, normals, blocking,
                                    vertex_area=centroids.point_data["Area"], az_range=az_range,

@@ -53,7 +53,8 @@ def create_model_index(source_index, sink_index, scattering_point_index):
     then scattering points); rows whose last two columns match are dropped
     (rayfunctions.py:2624-2678)."""
     src = np.asarray(source_index)
-    src = src.reshape(src.shape[0], -1)
+    if src.ndim == 1:
+        src = src.reshape(-1, 1)
     targets = np.asarray(sink_index).ravel()
     scat = np.asarray(scattering_point_index)
     if scat.shape[0] != 0:
@@ -174,13 +175,17 @@ def chunkingRaycaster1Dv3(sources, sinks, scattering_points, filtered_index, env
 
 def workchunkingv2(sources, sinks, scattering_points, environment, max_scatter,
                    line_of_sight=True):
-    """Ray waves for max_scatter 1 or 2 (rayfunctions.py:2681-2955); returns
+    """Ray waves for max_scatter 1, 2 or 3 (rayfunctions.py:2681-2955); returns
     (full_index int32 (V, max_scatter+1), RAYS_CAST).
 
     max_scatter 2 keeps the reference's row layout: line-of-sight rows padded
     with a 0 column, then the (source, scatterer, sink) rows.  As in the
     reference (rayfunctions.py:2835-2861), when no scattering point is visible
-    from any source the line-of-sight rows are not returned either."""
+    from any source the line-of-sight rows are not returned either.
+    max_scatter 3 runs the reference's source chunks (np.linspace bounds,
+    :2864-2867); per chunk the rows are [s, k, 0, 0], [s, p, k, 0] and
+    [s, p1, p2, k], and RAYS_CAST counts the last chunk's rays as the
+    reference's assignment does (:2936)."""
     S = np.asarray(sources).reshape(-1, 3).shape[0]
     K = np.asarray(sinks).reshape(-1, 3).shape[0]
     P = np.asarray(scattering_points).reshape(-1, 3).shape[0]
@@ -226,7 +231,30 @@ def workchunkingv2(sources, sinks, scattering_points, environment, max_scatter,
             full.append(np.concatenate(
                 [np.concatenate([fin, np.zeros((fin.shape[0], 1), np.int32)], axis=1), fin2]))
         return np.concatenate(full).astype(np.int32), rays
-    raise NotImplementedError("workchunkingv2: max_scatter 1 and 2 are supported")
+    if max_scatter == 3:
+        estimate = S * (K + P) + S * P * K * 3
+        chunknum = max(2, min(S, math.ceil(estimate / 2e8)))
+        bounds = np.linspace(0, S, chunknum, dtype=np.int32)
+        full = [np.empty((0, 4), np.int32)]
+        rays = 0
+        for c in range(len(bounds) - 1):
+            srows = source_index[bounds[c]:bounds[c + 1]]
+            io = create_model_index(srows, sink_index if line_of_sight else np.empty((0, 1)),
+                                    scat_index)
+            fil, fin, n1 = launchRaycaster1Dv3(sources, sinks, scattering_points, io, environment)
+            fil2, fin2, n2 = chunkingRaycaster1Dv3(sources, sinks, scattering_points, fil,
+                                                   environment, False)
+            _, fin3, n3 = chunkingRaycaster1Dv3(sources, sinks, scattering_points, fil2,
+                                                environment, True)
+            rays = n1 + n2 + n3
+            fin = fin.reshape(-1, 2)
+            t = np.concatenate([np.concatenate([fin, np.zeros((len(fin), 1), np.int32)], 1),
+                                fin2.reshape(-1, 3)])
+            full.append(np.concatenate([np.concatenate([t, np.zeros((len(t), 1), np.int32)], 1),
+                                        fin3.reshape(-1, 4)]))
+        return np.concatenate(full).astype(np.int32), rays
+    raise NotImplementedError("workchunkingv2: max_scatter 1, 2 and 3 are supported (the "
+                              "reference defines no other)")
 
 
 def visiblespace(source_coords, source_normals, environment, vertex_area=0, az_range=None,

@@ -470,14 +470,16 @@ ORC_API void orc_field_paths(const float *pos, const float *nrm, const float *w,
                              const int64_t *index, int64_t V, int ncol, int64_t S, int64_t K,
                              double alpha, double beta, int r_f32, double *out)
 {
-    (void)N_points;
     memset(out, 0, sizeof(double) * 6 * (size_t)(S * K));
     for (int64_t p = 0; p < V; ++p) {
         const int64_t *row = index + (int64_t)ncol * p;
         cplx E[3];
         int64_t last = row[1];
         for (int i = 0; i < ncol - 1; ++i) {
-            int64_t a = row[i] - 1;
+            /* a zero entry (no point) reads point -1, the last row of the table:
+             * numba wraps the negative index network_index - 1
+             * (empropagation.py:1147-1156) */
+            int64_t a = row[i] > 0 ? row[i] - 1 : N_points - 1;
             int64_t b = row[i + 1];
             if (i == 0) {
                 for (int c = 0; c < 3; ++c)
@@ -506,7 +508,8 @@ ORC_API void orc_field_paths(const float *pos, const float *nrm, const float *w,
                           alpha, beta, r_f32, dir);
         for (int i = 1; i < ncol - 1; ++i) {
             if (row[i + 1] != 0) {
-                cplx l2 = lossy(pos + 3 * (row[i] - 1), pos + 3 * (row[i + 1] - 1),
+                int64_t a = row[i] > 0 ? row[i] - 1 : N_points - 1;
+                cplx l2 = lossy(pos + 3 * a, pos + 3 * (row[i + 1] - 1),
                                 nrm + 3 * (row[i + 1] - 1), alpha, beta, r_f32, dir);
                 loss = cmul(loss, l2);
             }

@@ -222,7 +222,9 @@ def create_model_index(source_index, sink_index, scattering_point_index):
     (sinks then scattering points), the source row's columns followed by the
     target; rows whose last two columns are equal are dropped."""
     rows = []
-    src = np.asarray(source_index).reshape(len(source_index), -1)
+    src = np.asarray(source_index)
+    if src.ndim == 1:
+        src = src.reshape(-1, 1)
     targets = [int(x) for x in np.asarray(sink_index).ravel()]
     targets += [int(x) for x in np.asarray(scattering_point_index).ravel()]
     for n in range(src.shape[0]):
@@ -247,8 +249,8 @@ def ray_hits(intersect, point_indexing, sink_index):
 
 
 def workchunking(src, snk, scat, tris9, max_scatter, line_of_sight=True):
-    """workchunkingv2 (rayfunctions.py:2681-2955) for max_scatter 1 and 2 without
-    memory chunking: returns (full_index int32, RAYS_CAST)."""
+    """workchunkingv2 (rayfunctions.py:2681-2955) for max_scatter 1, 2 and 3
+    without memory chunking: returns (full_index int32, RAYS_CAST)."""
     src, snk = _f32(src).reshape(-1, 3), _f32(snk).reshape(-1, 3)
     scat = _f32(scat).reshape(-1, 3)
     S, K, P = len(src), len(snk), len(scat)
@@ -270,8 +272,34 @@ def workchunking(src, snk, scat, tris9, max_scatter, line_of_sight=True):
         io = create_model_index(source_index, sink_index, scat_index)
         _, fin, n = launch(io)
         return fin.reshape(-1, 2).astype(np.int32), n
+    if max_scatter == 3:
+        # rayfunctions.py:2863-2953: sources in chunks (np.linspace bounds);
+        # per chunk wave 1 (sinks and scatterers), wave 2 from the surviving
+        # scatterer rows to sinks and scatterers (chunkingRaycaster1Dv3 with
+        # terminate_flag False, :2120-2166), wave 3 from the rows that reached a
+        # second scatterer to the sinks; rows padded with zero columns to 4
+        estimate = S * (K + P) + S * P * K * 3
+        chunknum = max(2, min(S, int(np.ceil(estimate / 2e8))))
+        bounds = np.linspace(0, S, chunknum, dtype=np.int32)
+        parts, rays = [], 0
+        for c in range(len(bounds) - 1):
+            srows = source_index[bounds[c]:bounds[c + 1]]
+            io = create_model_index(srows, sink_index if line_of_sight else np.empty((0, 1)),
+                                    scat_index)
+            fil, fin, n1 = launch(io)
+            io2 = create_model_index(fil, sink_index, scat_index)
+            fil2, fin2, n2 = launch(io2)
+            io3 = create_model_index(fil2, sink_index, np.empty((0, 0), np.int32))
+            _, fin3, n3 = launch(io3)
+            rays = n1 + n2 + n3  # the reference keeps only the last chunk's count
+            t = np.concatenate([np.concatenate([fin.reshape(-1, 2),
+                                                np.zeros((len(fin), 1), np.int32)], 1),
+                                fin2.reshape(-1, 3)])
+            parts.append(np.concatenate([np.concatenate([t, np.zeros((len(t), 1), np.int32)], 1),
+                                         fin3.reshape(-1, 4)]))
+        return np.concatenate(parts).astype(np.int32).reshape(-1, 4), rays
     if max_scatter != 2:
-        raise NotImplementedError("oracle covers max_scatter 1 and 2")
+        raise NotImplementedError("oracle covers max_scatter 1, 2 and 3")
     if line_of_sight:
         io = create_model_index(source_index, sink_index, scat_index)
     else:

@@ -198,3 +198,119 @@ def test_one_bounce_no_visible_scatterer_rule(oracle, use_grid):
     ref = oracle.field_paths(pos, nrm, wt, full2, S, K, 0.0, beta).sum(0)
     assert np.abs(ref).max() > 0
     assert np.linalg.norm(E2 - ref) / np.linalg.norm(ref) < 1e-5
+
+
+def _x_vertical_terrain(n=10, seed=2):
+    """Height field with x vertical (the orientation the tile grid bins in y, z)."""
+    rng = np.random.default_rng(seed)
+    y, z = np.meshgrid(np.linspace(-1, 1, n + 1), np.linspace(-1, 1, n + 1))
+    x = 0.15 * rng.normal(size=y.shape)
+    pts = np.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+    idx = np.arange((n + 1) ** 2).reshape(n + 1, n + 1)
+    a, b, c, d = idx[:-1, :-1].ravel(), idx[:-1, 1:].ravel(), idx[1:, :-1].ravel(), \
+        idx[1:, 1:].ravel()
+    tri = np.concatenate([np.stack([a, b, d], 1), np.stack([a, d, c], 1)])
+    tri = tri[rng.random(len(tri)) < 0.35]
+    return Mesh(points=pts, cells=[("triangle", tri)])
+
+
+@pytest.mark.parametrize("scattering", [0, 1])
+def test_calculate_scattering_cuda_branch_vs_oracle(oracle, scattering):
+    """calculate_scattering(cuda=True) (frequency_domain.py:719-926) with
+    calculate_scattering_cuda (:195-264): native culled predicate, source row ==
+    sink row pairs untested, weights x Area; one bounce as the source ->
+    scatterer network, the summed scatterer weights and the matrix product of
+    the two networks per component, against the oracle."""
+    from lyceanem.models import frequency_domain as FD
+
+    solid = _x_vertical_terrain()
+    tris9 = np.asarray(solid.points, np.float32)[solid.cells[0].data].reshape(-1, 9)
+    rng = np.random.default_rng(8)
+    S, K, P = 12, 30, 9
+    src = np.stack([rng.uniform(0.5, 1.5, S), rng.uniform(-0.8, 0.8, S),
+                    rng.uniform(-0.8, 0.8, S)], 1)
+    snk = np.stack([rng.uniform(-1.5, -0.5, K), rng.uniform(-0.8, 0.8, K),
+                    rng.uniform(-0.8, 0.8, K)], 1)
+    scp = np.stack([rng.uniform(-0.4, 0.4, P), rng.uniform(-0.8, 0.8, P),
+                    rng.uniform(-0.8, 0.8, P)], 1)
+    area = rng.uniform(0.5, 1.5, S)
+    aperture = Mesh(points=src, point_data={"Normals": np.tile([-1.0, 0, 0], (S, 1)),
+                                            "Area": area})
+    sinks = Mesh(points=snk, point_data={"Normals": np.tile([1.0, 0, 0], (K, 1))})
+    scat = Mesh(points=scp, point_data={"Normals": rng.normal(size=(P, 3))})
+    desired = np.array([[0.2, 1.0, 0.5j]])
+    lam = 0.2
+    beta = 2 * np.pi / lam
+    out = {}
+    for elements in (False, True):
+        Ex, Ey, Ez = FD.calculate_scattering(aperture, sinks, solid, desired,
+                                             scatter_points=scat, wavelength=lam,
+                                             scattering=scattering, elements=elements,
+                                             beta=beta, cuda=True)
+        out[elements] = np.stack([Ex, Ey, Ez], -1)
+    src32, snk32, scp32 = (x.astype(np.float32) for x in (src, snk, scp))
+    w = (np.repeat(desired.astype(np.complex64), S, 0) * area[:, None]).astype(np.complex64)
+
+    def native_net(a, wa, b, b_n):
+        hit, _ = oracle.visibility_dense(a, b, tris9, semantics=1, any_hit_only=True,
+                                         skip_self=True, want_dist=False)
+        return oracle.field_los(a, wa, b, b_n, 0.0, beta, hit=hit, elements=True), hit
+
+    net, hit = native_net(src32, w, snk32, sinks.point_data["Normals"])
+    assert 0.05 < hit.mean() < 0.95
+    if scattering == 1:
+        src_scat, _ = native_net(src32, w, scp32, scat.point_data["Normals"])
+        ws = src_scat.sum(0).astype(np.complex64)
+        scat_snk, _ = native_net(scp32, ws, snk32, sinks.point_data["Normals"])
+        net = net + np.einsum("spc,pkc->skc", src_scat, scat_snk)
+    for elements, ref in ((True, net), (False, net.sum(0))):
+        err = np.linalg.norm(out[elements] - ref) / np.linalg.norm(ref)
+        print(f"cuda branch scattering={scattering} elements={elements}: {err:.2e}", flush=True)
+        assert out[elements].shape == ref.shape and err < 1e-4
+
+
+def test_aperture_projection_vs_oracle(oracle):
+    """aperture_projection (frequency_domain.py:15-91): triangle centroids lifted
+    by 1e-7 along their normals, weighted by the aperture's own cell areas, the
+    visible projected area on the shell as 4 pi A / lambda^2; the caller's mesh
+    is left unchanged, and a mesh without cell areas raises as the
+    reference's does."""
+    from lyceanem.geometry import geometryfunctions as GF
+    from lyceanem.geometry.targets import spherical_field
+    from lyceanem.models import frequency_domain as FD
+
+    g = np.linspace(-0.5, 0.5, 5)
+    X, Y = np.meshgrid(g, g)
+    pts = np.stack([X.ravel(), Y.ravel(), np.zeros(25)], 1)
+    idx = np.arange(25).reshape(5, 5)
+    a, b, c, d = idx[:-1, :-1].ravel(), idx[:-1, 1:].ravel(), idx[1:, :-1].ravel(), \
+        idx[1:, 1:].ravel()
+    tri = np.concatenate([np.stack([a, b, d], 1), np.stack([a, d, c], 1)])
+    plate = GF.compute_areas(Mesh(points=pts, cells=[("triangle", tri)]))
+    lid = Mesh(points=np.array([[-0.2, -2, 1.0], [0.9, -2, 1.0], [0.3, 2, 1.0]]),
+               cells=[("triangle", np.array([[0, 1, 2]]))])
+    az = np.linspace(-180, 180, 13)
+    el = np.linspace(-90, 90, 9)
+    lam = 0.3
+    before = {k: np.array(v[0], copy=True) for k, v in plate.cell_data.items()}
+    env, pcd = FD.aperture_projection(plate, environment=[plate, lid], wavelength=lam,
+                                      az_range=az, elev_range=el, farfield_distance=5.0)
+    assert set(plate.cell_data) == set(before)
+    for k, v in before.items():
+        assert np.array_equal(plate.cell_data[k][0], v)
+    cen = (pts[tri].mean(1) + 1e-7 * np.array([0, 0, 1.0])).astype(np.float32)
+    shell = spherical_field(az, el, outward_normals=True, field_radius=5.0)
+    tris9 = np.concatenate([pts[tri].reshape(-1, 9), lid.points.reshape(1, 9)]).astype(
+        np.float32)
+    hit, _ = oracle.visibility_dense(cen, shell.points, tris9, any_hit_only=True,
+                                     want_dist=False)
+    assert 0.01 < hit.mean() < 0.99
+    nrm = np.tile(np.array([[0, 0, 1.0]], np.float32), (len(tri), 1))
+    pat = oracle.projected_area(cen, nrm, plate.cell_data["Area"][0], shell.points, hit)
+    ref = 4 * np.pi * pat.reshape(len(el), len(az)) / lam ** 2
+    assert env.shape == (len(el), len(az)) and env.dtype == np.float32
+    assert np.allclose(env, ref, rtol=1e-5, atol=1e-6 * ref.max())
+    assert np.allclose(pcd.point_data["Directivity_Envelope"].reshape(env.shape), env, rtol=1e-5)
+    bare = Mesh(points=pts, cells=[("triangle", tri)])
+    with pytest.raises(KeyError, match="Area"):
+        FD.aperture_projection(bare, wavelength=lam, az_range=az, elev_range=el)

@@ -260,3 +260,59 @@ def test_tile_source_chunks_follow_the_reference_loop(S, chunks):
             end = S
     assert source_chunks(S, chunks) == exp
     assert exp[-1][1] == S
+
+
+def test_workchunking_three_waves_by_enumeration(oracle):
+    """oracle.workchunking(..., 3) (rayfunctions.py:2863-2953) holds exactly the
+    paths s -> k, s -> p -> k and s -> p -> q -> k (q != p) whose every leg is
+    unblocked, with the reference's index values and zero padding; without the
+    line of sight the first group is absent."""
+    rng = np.random.default_rng(3)
+    S, K, P = 3, 4, 6
+    src = (rng.uniform(-1, 1, (S, 3)) * [1, 1, 0]).astype(np.float32)
+    snk = (rng.uniform(-1, 1, (K, 3)) * [1, 1, 0] + [0, 0, 6]).astype(np.float32)
+    sct = (rng.uniform(-2, 2, (P, 3)) * [1, 1, 0.5] + [0, 0, 3]).astype(np.float32)
+    ctr = rng.uniform(-2, 2, (30, 1, 3)) * [1, 1, 0.6] + [0, 0, 3]
+    tris = (ctr + rng.uniform(-0.5, 0.5, (30, 3, 3))).reshape(-1, 9).astype(np.float32)
+    pts = np.concatenate([src, snk, sct])
+    blk, _ = oracle.visibility_dense(pts, pts, tris, any_hit_only=True, want_dist=False)
+    si, ki, pi = range(S), range(S, S + K), range(S + K, S + K + P)
+    ok = lambda a, b: not blk[a, b]  # noqa: E731
+    los = {(s + 1, k + 1, 0, 0) for s in si for k in ki if ok(s, k)}
+    one = {(s + 1, p + 1, k + 1, 0) for s in si for p in pi for k in ki if ok(s, p) and ok(p, k)}
+    two = {(s + 1, p + 1, q + 1, k + 1) for s in si for p in pi for q in pi for k in ki
+           if q != p and ok(s, p) and ok(p, q) and ok(q, k)}
+    assert los and one and two and 0.1 < blk.mean() < 0.9
+    full, rays = oracle.workchunking(src, snk, sct, tris, 3)
+    assert full.shape[1] == 4 and full.dtype == np.int32
+    got = [tuple(int(x) for x in r) for r in full]
+    assert len(got) == len(set(got)) and set(got) == los | one | two
+    # rows come as line of sight, one bounce, two bounces (one source chunk)
+    depth = (full[:, 1:] != 0).sum(1)
+    assert np.all(np.diff(depth) >= 0)
+    n_vis1 = sum(ok(s, p) for s in si for p in pi)
+    n_vis2 = sum(ok(s, p) and ok(p, q) for s in si for p in pi for q in pi if q != p)
+    assert rays == S * (K + P) + n_vis1 * (K + P - 1) + n_vis2 * K
+    full_n, _ = oracle.workchunking(src, snk, sct, tris, 3, line_of_sight=False)
+    assert set(tuple(int(x) for x in r) for r in full_n) == one | two
+
+
+def test_scatter_branch_project_vectors_raises_like_the_reference():
+    """calculate_scattering(scattering > 0, project_vectors=True) calls
+    calculate_conformalVectors with two of its three arguments in the
+    reference (frequency_domain.py:456-459, empropagation.py:3807) and raises
+    TypeError; so does the drop-in, before any device work."""
+    from lyceanem.electromagnetics import empropagation as EM
+    from lyceanem.mesh import Mesh
+    from lyceanem.models import frequency_domain as FD
+
+    ap = Mesh(points=np.zeros((2, 3)), point_data={"Normals": np.tile([0, 0, 1.0], (2, 1)),
+                                                   "Area": np.ones(2)})
+    sk = Mesh(points=np.ones((3, 3)), point_data={"Normals": np.tile([0, 0, 1.0], (3, 1))})
+    sc = Mesh(points=np.ones((2, 3)) * 2, point_data={"Normals": np.tile([0, 0, 1.0], (2, 1))})
+    solid = Mesh(points=np.eye(3) + 5, cells=[("triangle", np.array([[0, 1, 2]]))])
+    with pytest.raises(TypeError, match="antenna_axes"):
+        FD.calculate_scattering(ap, sk, solid, np.array([[1.0, 0, 0]]), scatter_points=sc,
+                                scattering=1, project_vectors=True)
+    with pytest.raises(TypeError):
+        EM.calculate_conformalVectors(np.array([1.0, 0, 0]), np.tile([0, 0, 1.0], (2, 1)))

@@ -222,3 +222,96 @@ def test_projected_area_closed_form(oracle):
     both = oracle.projected_area(src2, nrm2, np.array([0.25, 0.5]), snk, np.zeros((2, 400), bool))
     one = oracle.projected_area(src2[1:], nrm2[1:], np.array([0.5]), snk, np.zeros((1, 400), bool))
     np.testing.assert_allclose(both, pat + one, rtol=1e-5, atol=1e-6)
+
+
+def _proj(E, n):
+    n = np.asarray(n, np.float64)
+    n = n / np.linalg.norm(n)
+    return E - n * (E @ n)
+
+
+def _leg(a, b, n_b, beta):
+    """lossy_propagation's factor and direction of one leg a -> b (float64)."""
+    D = np.asarray(b, np.float64) - np.asarray(a, np.float64)
+    r = np.linalg.norm(D)
+    d = D / r
+    g = 1j * beta
+    L = -(1 / (2 * np.pi)) * (-g - 1 / r) * np.exp(-g * r) / r * (d @ np.asarray(n_b, np.float64))
+    return L, d
+
+
+def test_four_column_rows_closed_form(oracle):
+    """Rows of a 4-column index (workchunkingv2 with max_scatter 3) against the
+    per-row closed form of freqdomainkernal (empropagation.py:1083-1265): a
+    zero column at position i >= 1 reads point -1 (the last row of the table,
+    numba's negative-index wrap), so a line-of-sight row [s, k, 0, 0] is
+    projected onto the sink's tangent plane and then onto the last point's; a
+    one-bounce row [s, p, k, 0] ends with the sink's tangent projection; a
+    two-bounce row [s, p, q, k] has none at the sink."""
+    rng = np.random.default_rng(12)
+    pos = rng.uniform(-3, 3, (5, 3)).astype(np.float32)  # s, k, p, q, last
+    nrm = rng.normal(size=(5, 3)).astype(np.float32)
+    w = (rng.normal(size=(5, 3)) + 1j * rng.normal(size=(5, 3))).astype(np.complex64)
+    w[1] = 1.0  # the sink weight (1 in calculate_scattering)
+    P64, N64, W = pos.astype(np.float64), nrm.astype(np.float64), w.astype(np.complex128)
+    S, K = 1, 1
+    # line of sight
+    net = oracle.field_paths(pos, nrm, w, np.array([[1, 2, 0, 0]]), S, K, 0.0, BETA)
+    L, d = _leg(P64[0], P64[1], N64[1], BETA)
+    expect = L * _proj(_proj(_proj(W[0], d) * W[1], N64[1]), N64[4])
+    assert np.allclose(net[0, 0], expect, rtol=1e-12, atol=1e-15)
+    # the last-point projection really acts (not a sink-only projection)
+    only_sink = L * _proj(_proj(W[0], d) * W[1], N64[1])
+    assert np.abs(expect - only_sink).max() > 1e-3 * np.abs(expect).max()
+    # one bounce: s -> p -> k
+    net = oracle.field_paths(pos, nrm, w, np.array([[1, 3, 2, 0]]), S, K, 0.0, BETA)
+    L1, d1 = _leg(P64[0], P64[2], N64[2], BETA)
+    L2, d2 = _leg(P64[2], P64[1], N64[1], BETA)
+    e = _proj(_proj(W[0], d1) * W[2], N64[2])
+    expect = L1 * L2 * _proj(_proj(e, d2) * W[1], N64[1])
+    assert np.allclose(net[0, 0], expect, rtol=1e-12, atol=1e-15)
+    # two bounces: s -> p -> q -> k
+    net = oracle.field_paths(pos, nrm, w, np.array([[1, 3, 4, 2]]), S, K, 0.0, BETA)
+    L3, d3 = _leg(P64[2], P64[3], N64[3], BETA)
+    L4, d4 = _leg(P64[3], P64[1], N64[1], BETA)
+    e = _proj(_proj(_proj(W[0], d1) * W[2], N64[2]), d3) * W[3]
+    expect = L1 * L3 * L4 * (_proj(_proj(e, N64[3]), d4) * W[1])
+    assert np.allclose(net[0, 0], expect, rtol=1e-12, atol=1e-15)
+
+
+def test_four_column_paths_three_leg_factorisation(oracle):
+    """Every path of a 4-column index summed over sources equals the three-leg
+    evaluation the GPU two-bounce path uses (scatterer fields H, second-bounce
+    fields G with the self pairs left out, the extra projections of the
+    line-of-sight and one-bounce rows)."""
+    rng = np.random.default_rng(8)
+    S, K, P = 3, 4, 5
+    src = rng.uniform(-1, 1, (S, 3)).astype(np.float32)
+    snk = rng.uniform(5, 9, (K, 3)).astype(np.float32)
+    sct = rng.uniform(1, 4, (P, 3)).astype(np.float32)
+    src_n = np.tile(np.array([[0, 0, 1]], np.float32), (S, 1))
+    snk_n = rng.normal(size=(K, 3)).astype(np.float32)
+    sct_n = rng.normal(size=(P, 3)).astype(np.float32)
+    ws = (rng.normal(size=(S, 3)) + 1j * rng.normal(size=(S, 3))).astype(np.complex64)
+    wp = (0.5 * np.ones((P, 3))).astype(np.complex64)
+    pos = np.concatenate([src, snk, sct])
+    nrm = np.concatenate([src_n, snk_n, sct_n])
+    wts = np.concatenate([ws, np.ones((K, 3), np.complex64), wp])
+    s1, k1, p1 = np.arange(S) + 1, S + 1 + np.arange(K), S + K + 1 + np.arange(P)
+    rows = [[s, k, 0, 0] for s in s1 for k in k1]
+    rows += [[s, p, k, 0] for s in s1 for p in p1 for k in k1]
+    rows += [[s, p, q, k] for s in s1 for p in p1 for q in p1 if q != p for k in k1]
+    net = oracle.field_paths(pos, nrm, wts, np.array(rows, np.int64), S, K, 0.0, BETA)
+    n_last = sct_n[-1].astype(np.float64)
+    n_last = n_last / np.linalg.norm(n_last)
+    Elt = oracle.field_los(src, ws, snk, snk_n, 0.0, BETA, sink_tangent=True)
+    E0 = Elt - n_last[None, :] * (Elt @ n_last)[:, None]
+    H = oracle.field_los(src, ws, sct, sct_n, 0.0, BETA, snk_w=wp, sink_tangent=True)
+    H32 = H.astype(np.complex64)
+    E1 = oracle.field_los(sct, H32, snk, snk_n, 0.0, BETA, sink_tangent=True)
+    G = oracle.field_los(sct, H32, sct, sct_n, 0.0, BETA, snk_w=wp, sink_tangent=True,
+                         hit=np.eye(P, dtype=bool))
+    E2 = oracle.field_los(sct, G.astype(np.complex64), snk, snk_n, 0.0, BETA)
+    total = net.sum(0)
+    # H and G are rounded to complex64 between the legs -> ~1e-6 relative
+    assert np.allclose(total, E0 + E1 + E2, rtol=5e-6, atol=5e-6 * np.abs(total).max())
