@@ -122,6 +122,25 @@ int lem_field_los(const float *src, const float *src_w, int64_t S, const float *
                   double alpha, double beta, int flags, float *out, void *workspace,
                   int64_t workspace_bytes, void *stream);
 
+/* lem_field_los without a workspace and without any allocation, for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  The source-sum partials live in a static device array of
+ * LEM_FIELD_LOS_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends);
+ * K above lem_field_los_noalloc_max_sinks() is rejected with LEM_E_INVALID.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (two
+ * streams, two threads) race and give wrong sums.  The result equals
+ * lem_field_los's; when 12 * K * (its chunk count) floats exceed the array,
+ * fewer source chunks are used and the float32 partial sums are grouped
+ * differently (same float64 merge).  Arguments as lem_field_los, minus the
+ * workspace; LEM_FIELD_ELEMENTS needs no scratch and is passed through. */
+#define LEM_FIELD_LOS_NOALLOC_SCRATCH_BYTES (16ll << 20)
+int64_t lem_field_los_noalloc_max_sinks(void);
+int lem_field_los_noalloc(const float *src, const float *src_w, int64_t S, const float *snk,
+                          const float *snk_n, const float *snk_w, int64_t K,
+                          const uint64_t *hit_bits, double alpha, double beta, int flags,
+                          float *out, void *stream);
+
 /* Generic path list: the network EMGPUFreqDomain returns
  * (empropagation.py:2197-2410).  pos, nrm: N x 3; w: N x 3 complex64 (the
  * scattering_t table, base_types.py:89-119); index: V x ncol int32, 1-based,

@@ -19,7 +19,19 @@
 #include "lem_internal.h"
 #include "../../include/lem.h"
 
+#include <mutex>
+
 namespace lem {
+
+// Scratch of lem_field_los_noalloc: a file-scope device array, one copy per GPU
+// the library runs on, held until the process ends.  Host code never passes
+// its name to a kernel (on the host it is only a stand-in for the symbol): the
+// device address comes from hipGetSymbolAddress, once per device.
+constexpr int64_t NOALLOC_SCRATCH_FLOATS = LEM_FIELD_LOS_NOALLOC_SCRATCH_BYTES / 4;
+static __device__ __align__(256) float scratch[NOALLOC_SCRATCH_FLOATS];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static float *g_scratch_ptr[MAX_DEVICES];
 
 // Source-summed partials: block b covers sinks [(b % nbk)*256, +256) and
 // source chunk c = b / nbk ([c*CH, min(S, (c+1)*CH))).
@@ -292,6 +304,36 @@ __global__ __launch_bounds__(256) void k_projected_area(
     out[k] = (float)acc;
 }
 
+// Source-summed field with NC source chunks whose partials go to ws
+// (2 * NC * 6 * K floats: Kahan sums, then compensations).  Every partial is
+// written before the merge reads it, so ws needs no clearing.
+static int field_los_sum(const float *src, const float *src_w, int64_t S, const float *snk,
+                         const float *snk_n, const float *snk_w, int64_t K,
+                         const unsigned long long *hb, int64_t w64, int sm, double alpha,
+                         double beta, int tangent, float *out, int64_t NC, float *ws,
+                         hipStream_t st)
+{
+    const int64_t nbk = ceil_div(K, FB);
+    const int64_t CH = ceil_div(S, NC);
+    float *ws_sum = ws;
+    float *ws_cmp = ws_sum + NC * 6 * K;
+    LEM_REQUIRE(NC * nbk <= 2147483647LL, "lem_field_los: grid too large");
+    if (alpha != 0.0)
+        hipLaunchKernelGGL(k_field_los_partial<true>, dim3((unsigned)(NC * nbk)), dim3(FB), 0, st,
+                           src, src_w, S, snk, snk_n, K, hb, w64, sm, (float)alpha, beta, CH, nbk,
+                           ws_sum, ws_cmp);
+    else
+        hipLaunchKernelGGL(k_field_los_partial<false>, dim3((unsigned)(NC * nbk)), dim3(FB), 0, st,
+                           src, src_w, S, snk, snk_n, K, hb, w64, sm, (float)alpha, beta, CH, nbk,
+                           ws_sum, ws_cmp);
+    int rc = check_launch("k_field_los_partial");
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_field_los_finish, dim3((unsigned)nbk), dim3(FB), 0, st, ws_sum, ws_cmp,
+                       NC, K, snk_n, snk_w, tangent, out);
+    return check_launch("k_field_los_finish");
+}
+
 }  // namespace lem
 
 using namespace lem;
@@ -347,25 +389,8 @@ extern "C" int lem_field_los(const float *src, const float *src_w, int64_t S, co
         set_error("lem_field_los: workspace smaller than lem_field_los_workspace()");
         return E_WORKSPACE;
     }
-    const int64_t NC = los_chunks(S, K);
-    const int64_t CH = ceil_div(S, NC);
-    float *ws_sum = reinterpret_cast<float *>(workspace);
-    float *ws_cmp = ws_sum + NC * 6 * K;
-    LEM_REQUIRE(NC * nbk <= 2147483647LL, "lem_field_los: grid too large");
-    if (alpha != 0.0)
-        hipLaunchKernelGGL(k_field_los_partial<true>, dim3((unsigned)(NC * nbk)), dim3(FB), 0, st,
-                           src, src_w, S, snk, snk_n, K, hb, w64, sm, (float)alpha, beta, CH, nbk,
-                           ws_sum, ws_cmp);
-    else
-        hipLaunchKernelGGL(k_field_los_partial<false>, dim3((unsigned)(NC * nbk)), dim3(FB), 0, st,
-                           src, src_w, S, snk, snk_n, K, hb, w64, sm, (float)alpha, beta, CH, nbk,
-                           ws_sum, ws_cmp);
-    int rc = check_launch("k_field_los_partial");
-    if (rc)
-        return rc;
-    hipLaunchKernelGGL(k_field_los_finish, dim3((unsigned)nbk), dim3(FB), 0, st, ws_sum, ws_cmp,
-                       NC, K, snk_n, snk_w, tangent, out);
-    return check_launch("k_field_los_finish");
+    return field_los_sum(src, src_w, S, snk, snk_n, snk_w, K, hb, w64, sm, alpha, beta, tangent,
+                         out, los_chunks(S, K), reinterpret_cast<float *>(workspace), st);
 }
 
 extern "C" int lem_field_paths(const float *pos, const float *nrm, const float *w, int64_t N,
@@ -407,4 +432,78 @@ extern "C" int lem_projected_area(const float *src, const float *nrm, const floa
                        as_stream(stream), src, nrm, area, S, snk, K,
                        reinterpret_cast<const unsigned long long *>(hit_bits), out);
     return check_launch("k_projected_area");
+}
+
+// The device address of `scratch` on the current device, looked up on the
+// first call on that device and kept.
+static int scratch_on_current_device(float **ptr)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error(std::string("lem_field_los_noalloc: hipGetDevice: ") + hipGetErrorString(e));
+        return E_HIP;
+    }
+    if (dev < 0 || dev >= MAX_DEVICES) {
+        set_error("lem_field_los_noalloc: device index out of range");
+        return E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("lem_field_los_noalloc: hipGetSymbolAddress: ") +
+                      hipGetErrorString(e));
+            return E_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<float *>(p);
+    }
+    *ptr = g_scratch_ptr[dev];
+    return OK;
+}
+
+extern "C" int64_t lem_field_los_noalloc_max_sinks(void)
+{
+    return NOALLOC_SCRATCH_FLOATS / 12;
+}
+
+extern "C" int lem_field_los_noalloc(const float *src, const float *src_w, int64_t S,
+                                     const float *snk, const float *snk_n, const float *snk_w,
+                                     int64_t K, const uint64_t *hit_bits, double alpha,
+                                     double beta, int flags, float *out, void *stream)
+{
+    clear_error();
+    LEM_REQUIRE(S >= 0 && K >= 0, "lem_field_los_noalloc: negative size");
+    if (K == 0)
+        return OK;
+    // the per-element network and the empty source set use no scratch
+    if ((flags & LEM_FIELD_ELEMENTS) || S == 0)
+        return lem_field_los(src, src_w, S, snk, snk_n, snk_w, K, hit_bits, alpha, beta, flags,
+                             out, nullptr, 0, stream);
+    LEM_REQUIRE(out && snk && snk_n, "lem_field_los_noalloc: null pointer");
+    LEM_REQUIRE(src && src_w, "lem_field_los_noalloc: null source arrays");
+    if (K > NOALLOC_SCRATCH_FLOATS / 12) {
+        set_error("lem_field_los_noalloc: K = " + std::to_string(K) + " sinks exceed the " +
+                  std::to_string(NOALLOC_SCRATCH_FLOATS / 12) + " that its " +
+                  std::to_string(LEM_FIELD_LOS_NOALLOC_SCRATCH_BYTES) +
+                  "-byte static scratch holds; use lem_field_los with a workspace");
+        return E_INVALID;
+    }
+    float *ws = nullptr;
+    int rc = scratch_on_current_device(&ws);
+    if (rc)
+        return rc;
+    // as many source chunks as lem_field_los uses, fewer if their partials
+    // would not fit the scratch (then the sums are grouped differently)
+    int64_t NC = los_chunks(S, K);
+    const int64_t fit = NOALLOC_SCRATCH_FLOATS / (12 * K);
+    if (NC > fit)
+        NC = fit;
+    const int sm = (flags & LEM_FIELD_BITS_SINK_MAJOR) ? 1 : 0;
+    const int64_t w64 = sm ? ceil_div(S, 64) : ceil_div(K, 64);
+    return field_los_sum(src, src_w, S, snk, snk_n, snk_w, K,
+                         reinterpret_cast<const unsigned long long *>(hit_bits), w64, sm, alpha,
+                         beta, (flags & LEM_FIELD_SINK_TANGENT) ? 1 : 0, out, NC, ws,
+                         as_stream(stream));
 }

@@ -65,6 +65,8 @@ SIGNATURES = {
                             _P, _I64, _P], _INT),
     "lem_gw_mfma": ([_P, _I64, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P, _I64, _P,
                      _I64, _D, _P, _I64, _P, _P], _INT),
+    "lem_field_los_noalloc_max_sinks": ([], _I64),
+    "lem_field_los_noalloc": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _D, _D, _INT, _P, _P], _INT),
     "lem_field_los_adjoint_workspace": ([_I64, _I64], _I64),
     "lem_field_los_adjoint": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _D, _D, _INT, _P, _P, _P, _P,
                                _I64, _P], _INT),

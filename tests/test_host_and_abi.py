@@ -187,6 +187,15 @@ def test_library_rejects_bad_arguments_without_gpu():
     assert rc == -1 and b"columns" in lib.lem_last_error()
     assert lib.lem_field_los_workspace(100, 1000, 0) > 0
     assert lib.lem_field_los_workspace(100, 1000, _native.FIELD_ELEMENTS) == 0
+    # the allocation-free variant: its sink limit follows from the 16 MiB scratch,
+    # and sizes beyond it are rejected before any HIP call
+    kmax = lib.lem_field_los_noalloc_max_sinks()
+    assert kmax == (16 << 20) // 4 // 12
+    rc = lib.lem_field_los_noalloc(None, None, 10, None, None, None, kmax + 1, None, 0.0, 1.0, 0,
+                                   None, None)
+    assert rc == -1
+    rc = lib.lem_field_los_noalloc(1, 1, 10, 1, 1, None, kmax + 1, None, 0.0, 1.0, 0, 1, None)
+    assert rc == -1 and b"scratch" in lib.lem_last_error()
 
 
 def test_product_fails_loudly_without_gpu():
