@@ -310,7 +310,7 @@ __global__ __launch_bounds__(GB) void k_visibility_grid(
                                   src[3 * s + 1], src[3 * s + 2]);
             float db = rb.len;
             hit = dda_cast<ANY, SEM>(rb, db, G, cell_start, recs, box, nullptr, nullptr, nullptr,
-                                     MacroTab{nullptr, nullptr}, &r);
+                                     MacroTab{nullptr, nullptr, nullptr}, &r);
         } else {
             hit = dda_cast<ANY, SEM>(r, dist, G, cell_start, recs, box);
         }
@@ -353,13 +353,16 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_par(
         return;  // nothing queued (the usual case)
     __shared__ uint32_t smqa[MACRO_LDS_MAX];
     __shared__ uint16_t smqb[MACRO_LDS_MAX];
+    __shared__ float4 saxis[3];
     for (int i = threadIdx.x; i < nmacro; i += GB) {
         const uint2 q = mq[i];
         smqa[i] = q.x;
         smqb[i] = (uint16_t)q.y;
     }
+    if (threadIdx.x < 3)
+        saxis[threadIdx.x] = axis_entry(G, threadIdx.x);
     __syncthreads();
-    const MacroTab smq{smqa, smqb};
+    const MacroTab smq{smqa, smqb, saxis};
     const int64_t NI = SINK_MAJOR ? S : K, NO = SINK_MAJOR ? K : S;
     const int64_t W = (NI + 63) / 64;
     const bool scan = n > (unsigned long long)par_cap;
@@ -425,13 +428,16 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
 {
     __shared__ uint32_t smqa[MACRO_LDS_MAX];
     __shared__ uint16_t smqb[MACRO_LDS_MAX];
+    __shared__ float4 saxis[3];
     for (int i = threadIdx.x; i < nmacro; i += GB) {
         const uint2 q = mq[i];
         smqa[i] = q.x;
         smqb[i] = (uint16_t)q.y;
     }
+    if (threadIdx.x < 3)
+        saxis[threadIdx.x] = axis_entry(G, threadIdx.x);
     __syncthreads();
-    const MacroTab smq{smqa, smqb};
+    const MacroTab smq{smqa, smqb, saxis};
     const int64_t NI = SINK_MAJOR ? S : K, NO = SINK_MAJOR ? K : S;
     const int64_t W = (NI + 63) / 64;
     const int lane = threadIdx.x & 63;
@@ -448,7 +454,11 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
         unsigned long long t = 0;
         if (lane == 0)
             t = atomicAdd(&next_task[owner * COUNTER_STRIDE], 1ull);
-        t = __shfl(t, 0);
+        // the claim is the same for the whole wave: read it into scalar
+        // registers, so the claim's indices (and the outer point's loads) stay
+        // scalar and the walk keeps its vector registers
+        t = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned int)(t >> 32)) << 32) |
+            (unsigned long long)__builtin_amdgcn_readfirstlane((unsigned int)t);
         const int64_t outer = (int64_t)(t / (unsigned long long)UPR);
         if (outer >= NO) {
             if (++tried == 8)
@@ -673,6 +683,7 @@ __global__ __launch_bounds__(GB) void k_grid_stats(
 {
     __shared__ uint32_t smqa[MACRO_LDS_MAX];
     __shared__ uint16_t smqb[MACRO_LDS_MAX];
+    __shared__ float4 saxis[3];
     const bool lds = nmacro <= MACRO_LDS_MAX;
     if (lds)
         for (int i = threadIdx.x; i < nmacro; i += GB) {
@@ -680,8 +691,10 @@ __global__ __launch_bounds__(GB) void k_grid_stats(
             smqa[i] = q.x;
             smqb[i] = (uint16_t)q.y;
         }
+    if (threadIdx.x < 3)
+        saxis[threadIdx.x] = axis_entry(G, threadIdx.x);
     __syncthreads();
-    const MacroTab smq{smqa, smqb};
+    const MacroTab smq{smqa, smqb, saxis};
     const int64_t k = (int64_t)blockIdx.x / nbl;
     const int64_t s = ((int64_t)blockIdx.x % nbl) * GB + threadIdx.x;
     unsigned int nt = 0, nc = 0, nm = 0, nw = 0;

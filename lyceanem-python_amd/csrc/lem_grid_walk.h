@@ -27,9 +27,13 @@ constexpr int MACRO_LDS_MAX = 4352;  // blocks (6 B each, 25.5 KB of LDS: 6 work
 // The quantised macro table as the walk reads it from LDS: per block the
 // 32-bit word (lo.x, lo.y, lo.z, hi.x) and the 16-bit word (hi.y, hi.z) of
 // the 8-byte table entry, kept in two arrays so a block takes 6 bytes.
+// Beside it, the per-axis grid constants the walk's steps pick by axis (one
+// 16-byte LDS read instead of selecting among three scalar registers, which a
+// vector instruction can read only one at a time).
 struct MacroTab {
     const uint32_t *a;
     const uint16_t *b;
+    const float4 *axis;  // per axis: lo, cell, cells (int bits), macro blocks (int bits)
     LEM_DEV uint2 operator[](int m) const { return make_uint2(a[m], (uint32_t)b[m]); }
 };
 
@@ -75,6 +79,35 @@ inline GridDev to_dev(const lem_grid_info &g)
 }
 
 LEM_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Entry a of MacroTab::axis.
+LEM_DEV float4 axis_entry(const GridDev &G, int a)
+{
+    return a == 0 ? make_float4(G.lox, G.cx, __builtin_bit_cast(float, G.nx),
+                                __builtin_bit_cast(float, G.mx))
+         : a == 1 ? make_float4(G.loy, G.cy, __builtin_bit_cast(float, G.ny),
+                                __builtin_bit_cast(float, G.my))
+                  : make_float4(G.loz, G.cz, __builtin_bit_cast(float, G.nz),
+                                __builtin_bit_cast(float, G.mz));
+}
+
+// Grid constants of the axis a step crosses (ax: x, else ay: y, else z).
+struct AxisK {
+    float lo, c;
+    int n, nm;
+};
+
+template <bool LDSM>
+LEM_DEV AxisK axis_k(const GridDev &G, const MacroTab &smq, bool ax, bool ay)
+{
+    if constexpr (LDSM) {
+        const float4 k = smq.axis[ax ? 0 : (ay ? 1 : 2)];
+        return AxisK{k.x, k.y, __builtin_bit_cast(int, k.z), __builtin_bit_cast(int, k.w)};
+    } else {
+        return AxisK{ax ? G.lox : (ay ? G.loy : G.loz), ax ? G.cx : (ay ? G.cy : G.cz),
+                     ax ? G.nx : (ay ? G.ny : G.nz), ax ? G.mx : (ay ? G.my : G.mz)};
+    }
+}
 
 LEM_DEV void tri_cell_range(const GridDev &G, const float *t9, int lo[3], int hi[3])
 {
@@ -301,7 +334,8 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
                       const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
                       const float4 *__restrict__ box, unsigned int *n_tests = nullptr,
                       unsigned int *n_cells = nullptr, unsigned int *n_macro = nullptr,
-                      MacroTab smq = MacroTab{nullptr, nullptr}, const Ray *rt = nullptr)
+                      MacroTab smq = MacroTab{nullptr, nullptr, nullptr},
+                      const Ray *rt = nullptr)
 {
     const Ray &rq = rt ? *rt : r;  // the predicate's ray
     Walk w;
@@ -345,13 +379,17 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
         bool live;
         if (LDSM && !PAR) {
             const uint2 qv = smq[m];
-            const float bx = Axx + Cxx * (float)jx, by = Ayy + Cyy * (float)jy,
-                        bz = Azz + Czz * (float)jz;
-            const float x0 = bx + Sxx * (float)(qv.x & 255u), x1 = bx + Sxx * (float)(qv.x >> 24);
-            const float y0 = by + Syy * (float)((qv.x >> 8) & 255u),
-                        y1 = by + Syy * (float)(qv.y & 255u);
-            const float z0 = bz + Szz * (float)((qv.x >> 16) & 255u),
-                        z1 = bz + Szz * (float)((qv.y >> 8) & 255u);
+            // fused multiply-adds: the eps-widened boxes and the 2 eps margin in t
+            // absorb the difference in rounding
+            const float bx = __builtin_fmaf(Cxx, (float)jx, Axx),
+                        by = __builtin_fmaf(Cyy, (float)jy, Ayy),
+                        bz = __builtin_fmaf(Czz, (float)jz, Azz);
+            const float x0 = __builtin_fmaf(Sxx, (float)(qv.x & 255u), bx),
+                        x1 = __builtin_fmaf(Sxx, (float)(qv.x >> 24), bx);
+            const float y0 = __builtin_fmaf(Syy, (float)((qv.x >> 8) & 255u), by),
+                        y1 = __builtin_fmaf(Syy, (float)(qv.y & 255u), by);
+            const float z0 = __builtin_fmaf(Szz, (float)((qv.x >> 16) & 255u), bz),
+                        z1 = __builtin_fmaf(Szz, (float)((qv.y >> 8) & 255u), bz);
             const float tn = fmaxf(fmaxf(-margin, fminf(x0, x1)), fmaxf(fminf(y0, y1), fminf(z0, z1)));
             const float tf = fminf(fminf(dist + margin, fmaxf(x0, x1)),
                                    fminf(fmaxf(y0, y1), fmaxf(z0, z1)));
@@ -386,13 +424,12 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
             const float tex = ax ? Tx : (ay ? Ty : Tz);
             const int sa = ax ? sx : (ay ? sy : sz);
             const int ja = (ax ? jx : (ay ? jy : jz)) + sa;
+            const AxisK K = axis_k<LDSM>(G, smq, ax, ay);
             // the exits are combined into one test (one branch per step)
-            if ((tex >= t1 + margin) | (hit & (dist + margin <= tex)) | (ja < 0) |
-                (ja >= (ax ? G.mx : (ay ? G.my : G.mz))))
+            if ((tex >= t1 + margin) | (hit & (dist + margin <= tex)) | (ja < 0) | (ja >= K.nm))
                 break;
-            const float Ta = tnext(last(ja, sa, ax ? G.nx : (ay ? G.ny : G.nz)), sa,
-                                   ax ? G.lox : (ay ? G.loy : G.loz), ax ? G.cx : (ay ? G.cy : G.cz),
-                                   ax ? r.ox : (ay ? r.oy : r.oz), ax ? idx : (ay ? idy : idz));
+            const float Ta = tnext(last(ja, sa, K.n), sa, K.lo, K.c, ax ? r.ox : (ay ? r.oy : r.oz),
+                                   ax ? idx : (ay ? idy : idz));
             jx = ax ? ja : jx;
             jy = ay ? ja : jy;
             jz = (!ax && !ay) ? ja : jz;
@@ -485,13 +522,14 @@ LEM_DEV bool dda_cast(const Ray &r, float &dist, const GridDev &G,
             const bool az = !ax && !ay;
             const int sa = ax ? sx : (ay ? sy : sz);
             const int ia = (ax ? ix : (ay ? iy : iz)) + sa;
-            const int na = ax ? G.nx : (ay ? G.ny : G.nz);
+            const AxisK K = axis_k<LDSM>(G, smq, ax, ay);
+            const int na = K.n;
             if ((hit & (dist + margin <= texit)) | (texit >= t1 + margin) | (ia < 0) | (ia >= na)) {
                 done = true;
                 break;
             }
-            const float loa = ax ? G.lox : (ay ? G.loy : G.loz);
-            const float ca = ax ? G.cx : (ay ? G.cy : G.cz);
+            const float loa = K.lo;
+            const float ca = K.c;
             const float oa = ax ? r.ox : (ay ? r.oy : r.oz);
             const float ida = ax ? idx : (ay ? idy : idz);
             const float ta = tnext(ia, sa, loa, ca, oa, ida);

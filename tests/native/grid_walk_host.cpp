@@ -63,6 +63,7 @@ struct HostGrid {
     std::vector<float4> recs, box;   // box: 2 float4 per cell, then per macro block
     std::vector<uint32_t> qa;        // quantised macro table as the LDS holds it
     std::vector<uint16_t> qb;
+    float4 axis[3];                  // the per-axis constants the LDS holds beside the table
 };
 
 // force_mshift < 0: the plan's own choice.
@@ -127,6 +128,8 @@ HostGrid build_grid(const std::vector<float> &tris, float cells_per_tri, int for
                     H.box[2 * c + 1]);
     for (int64_t m = 0; m < H.info.nmacro; ++m)
         macro_box_of(G, H.box.data(), m, H.box[2 * (nc + m)], H.box[2 * (nc + m) + 1]);
+    for (int a = 0; a < 3; ++a)
+        H.axis[a] = axis_entry(G, a);
     H.qa.resize(H.info.nmacro);
     H.qb.resize(H.info.nmacro);
     for (int64_t m = 0; m < H.info.nmacro; ++m) {
@@ -148,7 +151,7 @@ bool cast(const HostGrid &H, Variant v, bool any, int sem, const float *s, const
     const GridDev &G = H.G;
     const int32_t *cs = H.cell_start.data();
     const float4 *rc = H.recs.data(), *bx = H.box.data();
-    const MacroTab smq{H.qa.data(), H.qb.data()};
+    const MacroTab smq{H.qa.data(), H.qb.data(), H.axis};
     if (v == GLOBAL) {
         if (sem == 1)
             return dda_cast<true, 1>(r, dist, G, cs, rc, bx);
@@ -177,9 +180,9 @@ bool cast(const HostGrid &H, Variant v, bool any, int sem, const float *s, const
     float dw = rw.len;
     if (v == REV_GLOBAL)
         return sem == 1 ? dda_cast<true, 1>(rw, dw, G, cs, rc, bx, nullptr, nullptr, nullptr,
-                                            MacroTab{nullptr, nullptr}, &r)
+                                            MacroTab{nullptr, nullptr, nullptr}, &r)
                         : dda_cast<true, 0>(rw, dw, G, cs, rc, bx, nullptr, nullptr, nullptr,
-                                            MacroTab{nullptr, nullptr}, &r);
+                                            MacroTab{nullptr, nullptr, nullptr}, &r);
     const bool par = rw.dx == 0.0f || rw.dy == 0.0f || rw.dz == 0.0f;
     if (sem == 1)
         return par ? dda_cast<true, 1, true, true>(rw, dw, G, cs, rc, bx, nullptr, nullptr,
