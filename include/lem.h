@@ -145,7 +145,11 @@ int lem_field_los_noalloc(const float *src, const float *src_w, int64_t S, const
  * (empropagation.py:2197-2410).  pos, nrm: N x 3; w: N x 3 complex64 (the
  * scattering_t table, base_types.py:89-119); index: V x ncol int32, 1-based,
  * 0 = absent point (workchunkingv2's full_index).  Paths are grouped by their
- * (source, sink) slot from targettingindex (empropagation.py:3394-3429):
+ * (source, sink) slot from targettingindex (empropagation.py:3394-3429).
+ * eta (N float32, or NULL when every point is electric): eta[i] > 0 marks a
+ * magnetic point (Electric = False) of impedance sqrt(Re permeability /
+ * Re permittivity); a path starting at one launches (w x d) / eta, d the unit
+ * direction of its first leg, instead of w (empropagation.py:1111-1146):
  * `order` lists the path rows slot by slot, seg_start (nseg + 1) bounds each
  * slot's run, seg_slot = s * K + k.  out: S x K x 3 complex128, only the listed
  * slots are written (zero it first).  err_flag (device int, zeroed by the caller)
@@ -154,8 +158,8 @@ int lem_field_los_noalloc(const float *src, const float *src_w, int64_t S, const
  * negative index does under numba (a line-of-sight row [s, k, 0, 0] of the
  * 4-column index is projected onto that point's tangent plane after the
  * sink's). */
-int lem_field_paths(const float *pos, const float *nrm, const float *w, int64_t N,
-                    const int32_t *index, int ncol, const int64_t *order,
+int lem_field_paths(const float *pos, const float *nrm, const float *w, const float *eta,
+                    int64_t N, const int32_t *index, int ncol, const int64_t *order,
                     const int64_t *seg_start, const int64_t *seg_slot, int64_t nseg,
                     double alpha, double beta, double *out, int *err_flag, void *stream);
 
@@ -307,11 +311,13 @@ int lem_beam_power(const float *W, int64_t nb, const float *Et, const float *Ep,
  * (amp, V x 3 float64) and the arrival time path length / c + the imaginary
  * parts of the point weights (arrival, V float64).  Points: N x 3 float32 with
  * normals, weights N x 3 complex64.  Replaces the per-path part of
- * timedomainkernal (empropagation.py:1345-1497); electric sources only.
+ * timedomainkernal (empropagation.py:1345-1497).  eta as for
+ * lem_field_paths: a magnetic first point launches Re(w) x d / eta
+ * (:1395-1419).
  * A row whose first entry is 0 or with an entry outside [0, N] sets *err
  * (device int32, zero it first) and gets NaN outputs. */
-int lem_time_paths(const float *pos, const float *nrm, const float *w, int64_t N,
-                   const int32_t *index, int64_t V, int ncol, double alpha, double beta,
+int lem_time_paths(const float *pos, const float *nrm, const float *w, const float *eta,
+                   int64_t N, const int32_t *index, int64_t V, int ncol, double alpha, double beta,
                    double *amp, double *arrival, int32_t *err, void *stream);
 
 /* *wake (device float64) = min(1 s, min arrival): the wake time of

@@ -173,11 +173,13 @@ __global__ __launch_bounds__(FB) void k_field_los_elements(
 
 // Generic path list (EMGPUFreqDomain's input): one thread per (source, sink)
 // slot, summing that slot's paths in index order.  index rows are 1-based,
-// 0 = absent point; `order` lists path rows grouped by slot.
+// 0 = absent point; `order` lists path rows grouped by slot.  eta (nullable):
+// eta[i] > 0 marks a magnetic point, whose paths launch (w x d) / eta.
 template <bool LOSSY>
 __global__ __launch_bounds__(FB) void k_field_paths(
     const float *__restrict__ pos, const float *__restrict__ nrm, const float *__restrict__ w,
-    int64_t N, const int32_t *__restrict__ index, int ncol, const int64_t *__restrict__ order,
+    const float *__restrict__ eta, int64_t N, const int32_t *__restrict__ index, int ncol,
+    const int64_t *__restrict__ order,
     const int64_t *__restrict__ seg_start, const int64_t *__restrict__ seg_slot, int64_t nseg,
     float alpha, double beta, double *__restrict__ out, int *__restrict__ err)
 {
@@ -218,6 +220,18 @@ __global__ __launch_bounds__(FB) void k_field_paths(
                 float cosang = ux * nrm[3 * bb] + uy * nrm[3 * bb + 1] + uz * nrm[3 * bb + 2];
                 float l2r, l2i;
                 leg_factor<LOSSY>(p, alpha, (float)beta, cosang, l2r, l2i);
+                if (i == 0 && eta && eta[a] > 0.0f) {
+                    // magnetic source: (w x d) / eta
+                    const float ie = 1.0f / eta[a];
+                    C3 m;
+                    m.xr = (E.yr * uz - E.zr * uy) * ie;
+                    m.xi = (E.yi * uz - E.zi * uy) * ie;
+                    m.yr = (E.zr * ux - E.xr * uz) * ie;
+                    m.yi = (E.zi * ux - E.xi * uz) * ie;
+                    m.zr = (E.xr * uy - E.yr * ux) * ie;
+                    m.zi = (E.xi * uy - E.yi * ux) * ie;
+                    E = m;
+                }
                 float nr = lr * l2r - li * l2i;
                 float ni = lr * l2i + li * l2r;
                 lr = nr;
@@ -393,7 +407,8 @@ extern "C" int lem_field_los(const float *src, const float *src_w, int64_t S, co
                          out, los_chunks(S, K), reinterpret_cast<float *>(workspace), st);
 }
 
-extern "C" int lem_field_paths(const float *pos, const float *nrm, const float *w, int64_t N,
+extern "C" int lem_field_paths(const float *pos, const float *nrm, const float *w,
+                               const float *eta, int64_t N,
                                const int32_t *index, int ncol, const int64_t *order,
                                const int64_t *seg_start, const int64_t *seg_slot, int64_t nseg,
                                double alpha, double beta, double *out, int *err_flag,
@@ -409,11 +424,11 @@ extern "C" int lem_field_paths(const float *pos, const float *nrm, const float *
     LEM_REQUIRE(ceil_div(nseg, FB) <= 2147483647LL, "lem_field_paths: too many segments");
     if (alpha != 0.0)
         hipLaunchKernelGGL(k_field_paths<true>, dim3((unsigned)ceil_div(nseg, FB)), dim3(FB), 0,
-                       as_stream(stream), pos, nrm, w, N, index, ncol, order, seg_start, seg_slot,
+                       as_stream(stream), pos, nrm, w, eta, N, index, ncol, order, seg_start, seg_slot,
                        nseg, (float)alpha, beta, out, err_flag);
     else
         hipLaunchKernelGGL(k_field_paths<false>, dim3((unsigned)ceil_div(nseg, FB)), dim3(FB), 0,
-                       as_stream(stream), pos, nrm, w, N, index, ncol, order, seg_start, seg_slot,
+                       as_stream(stream), pos, nrm, w, eta, N, index, ncol, order, seg_start, seg_slot,
                        nseg, (float)alpha, beta, out, err_flag);
     return check_launch("k_field_paths");
 }

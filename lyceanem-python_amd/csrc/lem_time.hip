@@ -93,6 +93,24 @@ LEM_DEV void project_dir(const float *__restrict__ pos, int64_t a, int64_t b, do
     v[2] -= p * uz;
 }
 
+// A magnetic first point (timedomainkernal, empropagation.py:1395-1419):
+// v = Re(w) x d / eta, d the float64 unit direction a -> b
+LEM_DEV void magnetic_launch(const float *__restrict__ pos, int64_t a, int64_t b, double eta,
+                             double v[3])
+{
+    const double dx = (double)(pos[3 * b] - pos[3 * a]);
+    const double dy = (double)(pos[3 * b + 1] - pos[3 * a + 1]);
+    const double dz = (double)(pos[3 * b + 2] - pos[3 * a + 2]);
+    const double n = sqrt(dx * dx + dy * dy + dz * dz);
+    const double ux = dx / n, uy = dy / n, uz = dz / n;
+    const double mx = (v[1] * uz - v[2] * uy) / eta;
+    const double my = (v[2] * ux - v[0] * uz) / eta;
+    const double mz = (v[0] * uy - v[1] * ux) / eta;
+    v[0] = mx;
+    v[1] = my;
+    v[2] = mz;
+}
+
 // (I - n n^T) v with n the normalised point normal (the u/v basis of
 // sourcelaunchtransformGPU is built from normalised cross products)
 LEM_DEV void project_normal64(const float *__restrict__ nrm, int64_t a, double v[3])
@@ -108,7 +126,8 @@ LEM_DEV void project_normal64(const float *__restrict__ nrm, int64_t a, double v
 
 __global__ __launch_bounds__(TB) void k_time_paths(
     const float *__restrict__ pos, const float *__restrict__ nrm, const float2 *__restrict__ w,
-    int64_t N, const int32_t *__restrict__ index, int64_t V, int ncol, double alpha, double beta,
+    const float *__restrict__ eta, int64_t N, const int32_t *__restrict__ index, int64_t V,
+    int ncol, double alpha, double beta,
     double *__restrict__ amp, double *__restrict__ arrival, int32_t *__restrict__ err)
 {
     const int64_t v = (int64_t)blockIdx.x * TB + threadIdx.x;
@@ -128,6 +147,8 @@ __global__ __launch_bounds__(TB) void k_time_paths(
     double len = sep64(pos, p0, p1);
     double delay = (double)w[3 * p0].y;
     double rc[3] = {(double)w[3 * p0].x, (double)w[3 * p0 + 1].x, (double)w[3 * p0 + 2].x};
+    if (eta && eta[p0] > 0.0f)
+        magnetic_launch(pos, p0, p1, (double)eta[p0], rc);
     if (row[1] != 0) {
         project_dir(pos, p0, p1, rc);
         rc[0] *= (double)w[3 * p1].x;
@@ -410,7 +431,8 @@ inline int64_t cub_sort_bytes(int64_t V, int64_t N)
 
 using namespace lem;
 
-extern "C" int lem_time_paths(const float *pos, const float *nrm, const float *w, int64_t N,
+extern "C" int lem_time_paths(const float *pos, const float *nrm, const float *w,
+                              const float *eta, int64_t N,
                               const int32_t *index, int64_t V, int ncol, double alpha, double beta,
                               double *amp, double *arrival, int32_t *err, void *stream)
 {
@@ -422,8 +444,8 @@ extern "C" int lem_time_paths(const float *pos, const float *nrm, const float *w
     LEM_REQUIRE(N > 0 && pos && nrm && w && index && amp && arrival && err,
                 "lem_time_paths: null pointer");
     hipLaunchKernelGGL(k_time_paths, dim3((unsigned)ceil_div(V, TB)), dim3(TB), 0,
-                       as_stream(stream), pos, nrm, reinterpret_cast<const float2 *>(w), N, index,
-                       V, ncol, alpha, beta, amp, arrival, err);
+                       as_stream(stream), pos, nrm, reinterpret_cast<const float2 *>(w), eta, N,
+                       index, V, ncol, alpha, beta, amp, arrival, err);
     return check_launch("k_time_paths");
 }
 

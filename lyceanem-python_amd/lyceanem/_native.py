@@ -42,7 +42,7 @@ SIGNATURES = {
     "lem_field_los_workspace": ([_I64, _I64, _INT], _I64),
     "lem_field_los": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _D, _D, _INT, _P, _P, _I64, _P],
                       _INT),
-    "lem_field_paths": ([_P, _P, _P, _I64, _P, _INT, _P, _P, _P, _I64, _D, _D, _P, _P, _P],
+    "lem_field_paths": ([_P, _P, _P, _P, _I64, _P, _INT, _P, _P, _P, _I64, _D, _D, _P, _P, _P],
                         _INT),
     "lem_grid_workspace": ([_I64, ctypes.c_float], _I64),
     "lem_grid_plan": ([_P, _I64, ctypes.c_float, _P, _P, _I64, _P], _INT),
@@ -54,7 +54,7 @@ SIGNATURES = {
     "lem_beam_workspace": ([_I64, _I64], _I64),
     "lem_beam_power": ([_P, _I64, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _I64, _P], _INT),
     "lem_projected_area": ([_P, _P, _P, _I64, _P, _I64, _P, _P, _P], _INT),
-    "lem_time_paths": ([_P, _P, _P, _I64, _P, _I64, _INT, _D, _D, _P, _P, _P, _P], _INT),
+    "lem_time_paths": ([_P, _P, _P, _P, _I64, _P, _I64, _INT, _D, _D, _P, _P, _P, _P], _INT),
     "lem_time_wake": ([_P, _I64, _P, _P], _INT),
     "lem_time_bins": ([_P, _I64, _P, _D, _P, _P], _INT),
     "lem_time_map": ([_P, _P, _P, _P, _P, _I64, _P, _I64, _I64, _P, _P], _INT),
@@ -281,14 +281,28 @@ def field_los_adjoint(src, src_w, snk, snk_n, grad_out, hit_bits=None, snk_w=Non
     return gw, gp
 
 
-def field_paths(pos, nrm, w, index, order, seg_start, seg_slot, S, K, alpha, beta):
-    """Network of an explicit path list -> (S,K,3) complex128 tensor."""
+def _eta(eta, N):
+    """Per-point impedance (> 0: magnetic point) as an N float32 device tensor,
+    or None when every point is electric."""
+    if eta is None:
+        return None
+    torch = _torch()
+    e = to_dev(eta, torch.float32).reshape(-1)
+    if e.shape[0] != N:
+        raise LemError("eta: one impedance per point")
+    return e
+
+
+def field_paths(pos, nrm, w, index, order, seg_start, seg_slot, S, K, alpha, beta, eta=None):
+    """Network of an explicit path list -> (S,K,3) complex128 tensor.  eta:
+    per-point impedance, > 0 for magnetic points (None: all electric)."""
     torch = _torch()
     lib = load_library()
     pos = to_dev(pos, torch.float32).reshape(-1, 3)
     nrm = to_dev(nrm, torch.float32).reshape(-1, 3)
     N = pos.shape[0]
     w = to_dev(w, torch.complex64).reshape(N, 3)
+    eta = _eta(eta, N)
     idx = to_dev(index, torch.int32)
     order = to_dev(order, torch.int64)
     seg_start = to_dev(seg_start, torch.int64)
@@ -296,7 +310,8 @@ def field_paths(pos, nrm, w, index, order, seg_start, seg_slot, S, K, alpha, bet
     nseg = seg_slot.shape[0]
     out = torch.zeros((S, K, 3), dtype=torch.complex128, device=pos.device)
     err = torch.zeros(1, dtype=torch.int32, device=pos.device)
-    check(lib.lem_field_paths(ptr(pos), ptr(nrm), ptr(w), N, ptr(idx), idx.shape[1], ptr(order),
+    check(lib.lem_field_paths(ptr(pos), ptr(nrm), ptr(w), ptr(eta), N, ptr(idx), idx.shape[1],
+                              ptr(order),
                               ptr(seg_start), ptr(seg_slot), nseg, float(alpha), float(beta),
                               ptr(out), ptr(err), stream_ptr()), "lem_field_paths")
     if int(err.item()) != 0:
@@ -356,14 +371,16 @@ def beam_power(W, Et, Ep, q, pstar):
     return power[:nb].cpu().numpy(), ustar[:nb].cpu().numpy()
 
 
-def time_paths(pos, nrm, w, index, alpha, beta):
-    """Per path (amp (V, 3), arrival (V,)) float64 tensors of a 1-based index."""
+def time_paths(pos, nrm, w, index, alpha, beta, eta=None):
+    """Per path (amp (V, 3), arrival (V,)) float64 tensors of a 1-based index;
+    eta as for field_paths."""
     torch = _torch()
     lib = load_library()
     pos = to_dev(pos, torch.float32).reshape(-1, 3)
     nrm = to_dev(nrm, torch.float32).reshape(-1, 3)
     N = pos.shape[0]
     w = to_dev(w, torch.complex64).reshape(N, 3)
+    eta = _eta(eta, N)
     idx = to_dev(index, torch.int32)
     if idx.dim() != 2:
         raise LemError("time_paths: index must be a 2-D array")
@@ -371,7 +388,8 @@ def time_paths(pos, nrm, w, index, alpha, beta):
     amp = torch.empty((V, 3), dtype=torch.float64, device=pos.device)
     arrival = torch.empty(V, dtype=torch.float64, device=pos.device)
     err = torch.zeros(1, dtype=torch.int32, device=pos.device)
-    check(lib.lem_time_paths(ptr(pos), ptr(nrm), ptr(w), N, ptr(idx), V, ncol, float(alpha),
+    check(lib.lem_time_paths(ptr(pos), ptr(nrm), ptr(w), ptr(eta), N, ptr(idx), V, ncol,
+                             float(alpha),
                              float(beta), ptr(amp), ptr(arrival), ptr(err), stream_ptr()),
           "lem_time_paths")
     if V and int(err.item()) != 0:

@@ -61,17 +61,16 @@ def EMGPUFreqDomain(source_num, sink_num, full_index, point_information, wavelen
     normals and complex weights of sources, sinks and scattering points.  The
     paths of each (source, sink) slot are summed on the GPU in index order
     (Kahan-compensated float32), so the result does not depend on launch order;
-    the reference adds them with float64 atomics in arbitrary order."""
+    the reference adds them with float64 atomics in arbitrary order.  A path
+    that starts at a magnetic point (Electric False) launches (w x d) / eta,
+    eta = sqrt(Re permeability / Re permittivity) of that point
+    (empropagation.py:1111-1146)."""
     pi = np.asarray(point_information)
-    if pi.shape[0] and not np.all(pi["Electric"]):
-        raise NotImplementedError("EMGPUFreqDomain: magnetic (Electric=False) sources")
     fi = np.asarray(full_index)
     S, K = int(source_num), int(sink_num)
     if fi.shape[0] == 0:
         return np.zeros((S, K, 3), dtype=np.complex128)
-    pos = np.stack([pi["px"], pi["py"], pi["pz"]], axis=1).astype(np.float32)
-    nrm = np.stack([pi["nx"], pi["ny"], pi["nz"]], axis=1).astype(np.float32)
-    w = np.stack([pi["ex"], pi["ey"], pi["ez"]], axis=1).astype(np.complex64)
+    pos, nrm, w, eta = _path_table(pi)
     s, k = path_slots(fi, S, K)
     if s.min() < 0 or s.max() >= S or k.min() < 0 or k.max() >= K:
         raise IndexError("full_index rows do not start on a source and end on a sink")
@@ -82,7 +81,7 @@ def EMGPUFreqDomain(source_num, sink_num, full_index, point_information, wavelen
     seg_start = np.append(starts, len(ss)).astype(np.int64)
     seg_slot = ss[starts].astype(np.int64)
     net = _native.field_paths(pos, nrm, w, np.ascontiguousarray(fi, np.int32), order, seg_start,
-                              seg_slot, S, K, float(alpha), float(beta))
+                              seg_slot, S, K, float(alpha), float(beta), eta=eta)
     return net.cpu().numpy()
 
 
@@ -120,21 +119,34 @@ def calculate_conformalVectors(desired_E_vector, source_normals, antenna_axes):
 
 
 def _path_table(point_information):
+    """Points, normals, weights and per-point impedances (None when every point
+    is electric; 0 marks an electric point) of a scattering_t table."""
     pi = np.asarray(point_information)
-    if pi.shape[0] and not np.all(pi["Electric"]):
-        raise NotImplementedError("time domain: magnetic (Electric=False) sources")
     pos = np.stack([pi["px"], pi["py"], pi["pz"]], axis=1).astype(np.float32)
     nrm = np.stack([pi["nx"], pi["ny"], pi["nz"]], axis=1).astype(np.float32)
     w = np.stack([pi["ex"], pi["ey"], pi["ez"]], axis=1).astype(np.complex64)
-    return pos, nrm, w
+    eta = None
+    magnetic = ~pi["Electric"].astype(bool) if pi.shape[0] else np.zeros(0, bool)
+    if magnetic.any():
+        mu = pi["permeability"].real.astype(np.float64)
+        eps = pi["permittivity"].real.astype(np.float64)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            z = np.sqrt(mu / eps)
+        bad = magnetic & ~(np.isfinite(z) & (z > 0))
+        if bad.any():
+            raise ValueError("magnetic points need a positive real impedance "
+                             "sqrt(Re permeability / Re permittivity)")
+        eta = np.where(magnetic, z, 0.0).astype(np.float32)
+    return pos, nrm, w, eta
 
 
 def time_domain_paths(point_information, full_index, alpha, beta, sampling_freq):
     """(amp (V, 3), arrival (V,), wake (1,), bins (V,)) device tensors of every
-    row of full_index: timedomainkernal's per-path part and start samples."""
-    pos, nrm, w = _path_table(point_information)
+    row of full_index: timedomainkernal's per-path part and start samples; a
+    magnetic first point launches Re(w) x d / eta (empropagation.py:1395-1419)."""
+    pos, nrm, w, eta = _path_table(point_information)
     fi = np.ascontiguousarray(full_index, np.int32)
-    amp, arrival = _native.time_paths(pos, nrm, w, fi, alpha, beta)
+    amp, arrival = _native.time_paths(pos, nrm, w, fi, alpha, beta, eta=eta)
     wake = _native.time_wake(arrival)
     bins = _native.time_bins(arrival, wake, sampling_freq)
     return amp, arrival, wake, bins

@@ -62,7 +62,11 @@ def _scatter_branch_weights(aperture_coords, desired_E_axis, project_vectors, nu
     return np.repeat(row, num_sources, axis=0).astype(np.complex64)
 
 
-def _point_table(points_list, normals_list, weights_list):
+def _point_table(points_list, normals_list, weights_list, permittivity=8.8541878176e-12,
+                 permeability=1.25663706212e-6):
+    """scattering_t table of electric points with the given free-space
+    constants, as the reference fills point_informationv2
+    (frequency_domain.py:401-405)."""
     pts = np.concatenate([np.asarray(p, np.float32).reshape(-1, 3) for p in points_list])
     nrm = np.concatenate([np.asarray(n, np.float32).reshape(-1, 3) for n in normals_list])
     w = np.concatenate([np.asarray(x, np.complex64).reshape(-1, 3) for x in weights_list])
@@ -71,6 +75,8 @@ def _point_table(points_list, normals_list, weights_list):
     table["nx"], table["ny"], table["nz"] = nrm[:, 0], nrm[:, 1], nrm[:, 2]
     table["ex"], table["ey"], table["ez"] = w[:, 0], w[:, 1], w[:, 2]
     table["Electric"] = True
+    table["permittivity"] = permittivity
+    table["permeability"] = permeability
     return table
 
 
@@ -129,7 +135,7 @@ def calculate_scattering(aperture_coords, sink_coords, antenna_solid, desired_E_
     # per-source network: the reference's path list (S x P x K rows at most)
     src_n = np.asarray(aperture_coords.point_data["Normals"]).astype(np.float32)
     table = _point_table([src, snk, scat], [src_n, snk_n, scat_n],
-                         [w, np.ones((K, 3)), np.ones((P, 3))])
+                         [w, np.ones((K, 3)), np.ones((P, 3))], permittivity, permeability)
     full_index, _ = RF.workchunkingv2(src, snk, scat, tris9, scattering + 1, line_of_sight=los)
     net = EM.EMGPUFreqDomain(S, K, full_index, table, wavelength, alpha, beta)
     return net[:, :, 0], net[:, :, 1], net[:, :, 2]

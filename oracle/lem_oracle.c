@@ -461,14 +461,19 @@ ORC_API void orc_field_los(const float *src_pos, const float *src_w, int64_t S,
  * network EMGPUFreqDomain returns (:2197-2410).  Float64, paths summed in index
  * order (the reference adds with float64 atomics in an unspecified order).
  * pos / nrm: N x 3 float; w: N x 3 complex64 (interleaved) -- the scattering_t
- * table (base_types.py:89-119) with Electric = True everywhere.
+ * table (base_types.py:89-119).  eta (N doubles, or NULL when every point is
+ * electric): eta[i] > 0 marks a magnetic point (Electric = False) of impedance
+ * sqrt(Re permeability / Re permittivity); a path starting there launches
+ * (w x d) / eta, d the unit direction to its next point
+ * (empropagation.py:1111-1146), instead of w.
  * index: V x ncol int64, 1-based, 0 = no point (a padded LOS row).
  * The (source, sink) slot of each row comes from targettingindex
  * (empropagation.py:3394-3429): LOS rows -> (col0, col1), others -> (col0, last).
  * out: S x K x 3 complex128, zeroed here. */
-ORC_API void orc_field_paths(const float *pos, const float *nrm, const float *w, int64_t N_points,
-                             const int64_t *index, int64_t V, int ncol, int64_t S, int64_t K,
-                             double alpha, double beta, int r_f32, double *out)
+ORC_API void orc_field_paths(const float *pos, const float *nrm, const float *w,
+                             const double *eta, int64_t N_points, const int64_t *index, int64_t V,
+                             int ncol, int64_t S, int64_t K, double alpha, double beta, int r_f32,
+                             double *out)
 {
     memset(out, 0, sizeof(double) * 6 * (size_t)(S * K));
     for (int64_t p = 0; p < V; ++p) {
@@ -484,6 +489,23 @@ ORC_API void orc_field_paths(const float *pos, const float *nrm, const float *w,
             if (i == 0) {
                 for (int c = 0; c < 3; ++c)
                     E[c] = ldc(w, a, c);
+                if (eta && eta[a] > 0 && b != 0) {
+                    const float *pa = pos + 3 * a, *pb = pos + 3 * (b - 1);
+                    double d[3] = {(double)pb[0] - pa[0], (double)pb[1] - pa[1],
+                                   (double)pb[2] - pa[2]};
+                    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                    for (int c = 0; c < 3; ++c)
+                        d[c] /= r;
+                    cplx m[3];
+                    for (int c = 0; c < 3; ++c) {
+                        /* (w x d)_c = w_{c+1} d_{c+2} - w_{c+2} d_{c+1} */
+                        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+                        m[c].re = (E[c1].re * d[c2] - E[c2].re * d[c1]) / eta[a];
+                        m[c].im = (E[c1].im * d[c2] - E[c2].im * d[c1]) / eta[a];
+                    }
+                    for (int c = 0; c < 3; ++c)
+                        E[c] = m[c];
+                }
             } else {
                 double n[3] = {nrm[3 * a], nrm[3 * a + 1], nrm[3 * a + 2]};
                 launch_transform(E, n);
@@ -622,15 +644,17 @@ static void orc_launch(const double d_in[3], double v[3])
 }
 
 /* Per path row (timedomainkernal, empropagation.py:1361-1491): ray_component
- * = Re(weight of the first point), then per leg i -> i+1: (i > 0) the launch
+ * = Re(weight of the first point) -- for a magnetic first point (eta[p] > 0,
+ * eta as for orc_field_paths) Re(w) x d / eta, d the unit direction of the
+ * first leg (:1395-1419) -- then per leg i -> i+1: (i > 0) the launch
  * transform with point i's normal and the leg length added; if the next entry
  * is non-zero, the launch transform along the leg, times Re(next weight), the
  * delay increased by Im(next ex), and (i > 0) the loss product extended.
  * amp = ray_component * |loss product| (cmath.polar), arrival = length / c +
  * delay.  w: N x 3 complex64 as interleaved float pairs. */
-ORC_API void orc_time_paths(const float *pos, const float *nrm, const float *w, int64_t N,
-                            const int32_t *index, int64_t V, int ncol, double alpha, double beta,
-                            double *amp, double *arrival)
+ORC_API void orc_time_paths(const float *pos, const float *nrm, const float *w,
+                            const double *eta, int64_t N, const int32_t *index, int64_t V,
+                            int ncol, double alpha, double beta, double *amp, double *arrival)
 {
     const double C = 299792458.0;
     for (int64_t v = 0; v < V; ++v) {
@@ -644,6 +668,21 @@ ORC_API void orc_time_paths(const float *pos, const float *nrm, const float *w, 
                 rc[0] = w[6 * a];
                 rc[1] = w[6 * a + 2];
                 rc[2] = w[6 * a + 4];
+                if (eta && eta[a] > 0) {
+                    double d[3] = {(double)(pos[3 * b] - pos[3 * a]),
+                                   (double)(pos[3 * b + 1] - pos[3 * a + 1]),
+                                   (double)(pos[3 * b + 2] - pos[3 * a + 2])};
+                    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                    for (int c = 0; c < 3; ++c)
+                        d[c] /= r;
+                    double m[3];
+                    for (int c = 0; c < 3; ++c) {
+                        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+                        m[c] = (rc[c1] * d[c2] - rc[c2] * d[c1]) / eta[a];
+                    }
+                    for (int c = 0; c < 3; ++c)
+                        rc[c] = m[c];
+                }
             } else {
                 double nn[3] = {nrm[3 * a], nrm[3 * a + 1], nrm[3 * a + 2]};
                 orc_launch(nn, rc);

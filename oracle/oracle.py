@@ -73,11 +73,11 @@ def lib():
                                              ctypes.c_int]
         L.orc_field_los.argtypes = [P, P, i64, P, P, P, i64, P, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_int, P, ctypes.c_int]
-        L.orc_field_paths.argtypes = [P, P, P, i64, P, i64, ctypes.c_int, i64, i64,
+        L.orc_field_paths.argtypes = [P, P, P, P, i64, P, i64, ctypes.c_int, i64, i64,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, P]
         L.orc_py_floordiv.argtypes = [ctypes.c_double, ctypes.c_double]
         L.orc_py_floordiv.restype = ctypes.c_double
-        L.orc_time_paths.argtypes = [P, P, P, i64, P, i64, ctypes.c_int, ctypes.c_double,
+        L.orc_time_paths.argtypes = [P, P, P, P, i64, P, i64, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_double, P, P]
         L.orc_time_place.argtypes = [P, P, i64, P, P, i64, ctypes.c_double, i64, P, P, P]
         _lib = L
@@ -199,16 +199,22 @@ def field_los(src_pos, src_w, snk_pos, snk_nrm, alpha, beta, hit=None, snk_w=Non
     return out.view(np.complex128).reshape(((S, K, 3) if elements else (K, 3)))
 
 
-def field_paths(pos, nrm, w, index, S, K, alpha, beta, r_f32=False):
-    """freqdomainkernal over an explicit 1-based path index -> (S,K,3) complex128."""
+def _eta(eta, N):
+    return None if eta is None else np.ascontiguousarray(np.broadcast_to(eta, (N,)), np.float64)
+
+
+def field_paths(pos, nrm, w, index, S, K, alpha, beta, r_f32=False, eta=None):
+    """freqdomainkernal over an explicit 1-based path index -> (S,K,3) complex128.
+    eta: per-point impedance, > 0 for magnetic points (None: all electric)."""
     p = _f32(pos).reshape(-1, 3)
     n = _f32(nrm).reshape(-1, 3)
     N = p.shape[0]
     ww = _c64x3(w, N)
+    et = _eta(eta, N)
     idx = np.ascontiguousarray(index, dtype=np.int64)
     V, ncol = idx.shape
     out = np.zeros((S, K, 6), np.float64)
-    lib().orc_field_paths(_p(p), _p(n), _p(ww), N, _p(idx), V, ncol, S, K, float(alpha),
+    lib().orc_field_paths(_p(p), _p(n), _p(ww), _p(et), N, _p(idx), V, ncol, S, K, float(alpha),
                           float(beta), int(r_f32), _p(out))
     return out.view(np.complex128).reshape(S, K, 3)
 
@@ -414,18 +420,20 @@ def py_floordiv(a, b):
     return lib().orc_py_floordiv(float(a), float(b))
 
 
-def time_paths(pos, nrm, w, index, alpha, beta):
-    """(amp (V, 3), arrival (V,)) float64 of every row of a 1-based index."""
+def time_paths(pos, nrm, w, index, alpha, beta, eta=None):
+    """(amp (V, 3), arrival (V,)) float64 of every row of a 1-based index;
+    eta as for field_paths."""
     p = _f32(pos).reshape(-1, 3)
     n = _f32(nrm).reshape(-1, 3)
     N = p.shape[0]
     ww = _c64x3(w, N)
+    et = _eta(eta, N)
     idx = np.ascontiguousarray(index, np.int32)
     V, ncol = idx.shape
     amp = np.zeros((V, 3), np.float64)
     arr = np.zeros(V, np.float64)
-    lib().orc_time_paths(_p(p), _p(n), _p(ww), N, _p(idx), V, ncol, float(alpha), float(beta),
-                         _p(amp), _p(arr))
+    lib().orc_time_paths(_p(p), _p(n), _p(ww), _p(et), N, _p(idx), V, ncol, float(alpha),
+                         float(beta), _p(amp), _p(arr))
     return amp, arr
 
 

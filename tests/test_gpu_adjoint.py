@@ -10,10 +10,11 @@ pytestmark = pytest.mark.gpu
 GRAD_RTOL = 1e-3
 
 
-def _small_c4():
+def _small_c4(radius=0.5):
     from lyceanem import scenes
 
-    return scenes.conformal_c4(n_az_el=16, n_z_el=8, n_el=19, n_az=37, n_side=32, n_ring=8)
+    return scenes.conformal_c4(n_az_el=16, n_z_el=8, n_el=19, n_az=37, n_side=32, n_ring=8,
+                               radius=radius)
 
 
 def _mask(native, sc):
@@ -30,10 +31,12 @@ def _loss64(oracle, w, pos, sc, visible, in_beam, ref_power=None):
     return db_mask_loss(E, torch.from_numpy(in_beam), ref_power=ref_power)
 
 
-def test_adjoint_vs_dense_reference_gradient(native, oracle):
+@pytest.mark.parametrize("radius", [0.5, 3.0])
+def test_adjoint_vs_dense_reference_gradient(native, oracle, radius):
+    """A 0.5 m and a 3 m cylinder of 128 elements at 10 GHz."""
     from lyceanem.differentiable import far_field, db_mask_loss
 
-    sc = _small_c4()
+    sc = _small_c4(radius)
     bits, visible = _mask(native, sc)
     assert 0.1 < visible.mean() < 0.9
     dev = native.device()

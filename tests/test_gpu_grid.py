@@ -310,7 +310,7 @@ def test_fused_grid_field_matches_masked_field(native):
         assert np.array_equal(a, c)
 
 
-def test_sink_major_mask_field_matches_fused(native):
+def test_sink_major_mask_field_matches_fused(native, term_bound):
     """The field read through a sink-major mask (K x W64(S)), whole or in sink
     chunks, equals the fused grid field and the source-major masked field."""
     src, snk, tris = _small_tri_scene(8, S=150, K=230)
@@ -338,7 +338,8 @@ def test_sink_major_mask_field_matches_fused(native):
                                                  chunk_bytes=cb).cpu().numpy()
             assert np.array_equal(b, ref)
     # source-major mask from the persistent traversal: whole (identical sums)
-    # and in source chunks (chunk sums added: float32 rounding apart)
+    # and in source chunks (chunk sums added: float32 summation rounding, below
+    # 1e-5 of the sum of the sink's |terms|)
     for tangent in (False, True):
         ref = native.field_los_grid(g, src, w, snk, nrm, snk_w=wk, beta=beta,
                                     sink_tangent=tangent).cpu().numpy()
@@ -348,7 +349,10 @@ def test_sink_major_mask_field_matches_fused(native):
         b = native.field_los_grid_masked(g, src, w, snk, nrm, snk_w=wk, beta=beta,
                                          sink_tangent=tangent, sink_major=False,
                                          chunk_bytes=32 * 40).cpu().numpy()
-        np.testing.assert_allclose(b, ref, rtol=1e-5, atol=1e-6 * np.abs(ref).max())
+        bound = term_bound(src, w, snk, nrm, wk, 0.0, beta)
+        per = (np.abs(b - ref).max(axis=-1) / bound).max()
+        print(f"source-chunked masked field: per-sink {per:.2e}", flush=True)
+        assert per < 1e-5
     # per-element network from either layout
     e1 = native.field_los(src, w, snk, nrm, hit_bits=bits, snk_w=wk, beta=beta,
                           elements=True).cpu().numpy()

@@ -182,13 +182,15 @@ def test_field_los_elements_and_phase(native, oracle):
         assert (E[hit] == 0).all()
 
 
-def test_field_long_paths_phase(native, oracle):
-    """1-10 km paths at 5.8 GHz: the float64 path length keeps the phase error
-    far below 1e-3 rad (the float32 path length would not)."""
+@pytest.mark.parametrize("extent", [0.25, 5.0])
+def test_field_long_paths_phase(native, oracle, extent):
+    """1-10 km paths at 5.8 GHz, sources within +-0.25 m and +-5 m: the float64
+    path length keeps the phase error far below 1e-3 rad (a float32 path length
+    would not)."""
     rng = np.random.default_rng(4)
     lam = 299792458.0 / 5.8e9
     beta = 2 * np.pi / lam
-    src = rng.uniform(-5, 5, (64, 3)).astype(np.float32)
+    src = rng.uniform(-extent, extent, (64, 3)).astype(np.float32)
     u = rng.normal(size=(300, 3))
     u /= np.linalg.norm(u, axis=1, keepdims=True)
     snk = (u * rng.uniform(1e3, 1e4, (300, 1))).astype(np.float32)
@@ -208,8 +210,50 @@ def test_field_long_paths_phase(native, oracle):
     assert np.abs(E[big] - ref[big]).max() / np.abs(ref[big]).max() < FIELD_RTOL
 
 
-@pytest.mark.parametrize("ncol", [2, 3])
-def test_field_paths_vs_oracle(native, oracle, ncol):
+def test_field_los_mixed_source_extents(native, oracle):
+    """One launch over compact and spread source chunks: 2048 sources in a
+    0.2 m cube followed by 2048 spread over 40 m, sinks from 30 m to 10 km at
+    10 GHz, with a mask; per element (phase and magnitude) and summed."""
+    rng = np.random.default_rng(21)
+    beta = 2 * np.pi / 0.03
+    src = np.concatenate([rng.uniform(-0.1, 0.1, (2048, 3)),
+                          rng.uniform(-20, 20, (2048, 3))]).astype(np.float32)
+    u = rng.normal(size=(3, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    snk = (u * np.array([[30.0], [700.0], [1e4]])).astype(np.float32)
+    nrm = u.astype(np.float32)
+    S, K = len(src), len(snk)
+    w = (rng.normal(size=(S, 3)) + 1j * rng.normal(size=(S, 3))).astype(np.complex64)
+    hit = rng.random((S, K)) < 0.2
+    packed = np.zeros((S, 1), np.uint64)
+    for k in range(K):
+        packed[:, 0] |= hit[:, k].astype(np.uint64) << np.uint64(k)
+    import torch
+    bits = torch.from_numpy(packed.view(np.int64)).cuda()
+    El = oracle.field_los(src, w, snk, nrm, 0.0, beta, hit=hit, elements=True)
+    big = np.abs(El) > 1e-3 * np.abs(El).max(axis=(0, 2), keepdims=True)
+    for elements in (True, False):
+        E = native.field_los(src, w, snk, nrm, hit_bits=bits, beta=beta,
+                             elements=elements).cpu().numpy()
+        if elements:
+            dph = np.abs(np.angle(E[big] / El[big]))
+            rel = np.abs(E[big] - El[big]) / np.abs(El[big])
+            print(f"mixed extents, elements: max phase {dph.max():.2e} rad, max rel "
+                  f"{rel.max():.2e}", flush=True)
+            assert dph.max() < 1e-4 and rel.max() < FIELD_RTOL
+        else:
+            ref = El.sum(axis=0)
+            scale = np.abs(El).sum(axis=0).max(axis=-1) + 1e-30
+            nrm_err, per = _field_errors(E, ref, scale)
+            print(f"mixed extents, sum: {nrm_err:.2e} {per:.2e}", flush=True)
+            assert nrm_err < FIELD_RTOL and per < FIELD_RTOL
+
+
+@pytest.mark.parametrize("ncol,magnetic", [(2, False), (3, False), (2, True), (3, True)])
+def test_field_paths_vs_oracle(native, oracle, ncol, magnetic):
+    """EMGPUFreqDomain on 2- and 3-column paths; with magnetic=True every other
+    source is a magnetic point (Electric False, free-space impedance), whose
+    paths launch (w x d) / eta."""
     rng = np.random.default_rng(10 + ncol)
     S, K, P = 6, 9, 7
     pos = np.concatenate([rng.uniform(-1, 1, (S, 3)), rng.uniform(20, 40, (K, 3)),
@@ -235,8 +279,16 @@ def test_field_paths_vs_oracle(native, oracle, ncol):
         table["n" + c] = nrm[:, i]
         table["e" + c] = w[:, i]
     table["Electric"] = True
+    table["permittivity"] = 8.8541878187e-12
+    table["permeability"] = 1.25663706212e-6
+    eta = None
+    if magnetic:
+        table["Electric"][0:S:2] = False
+        eta0 = np.float32(np.sqrt(np.float64(np.float32(1.25663706212e-6))
+                                  / np.float64(np.float32(8.8541878187e-12))))
+        eta = np.where(~table["Electric"], eta0, 0.0)
     net = EM.EMGPUFreqDomain(S, K, idx.astype(np.int32), table, 1.0, 0.0, 2 * np.pi / 0.5)
-    ref = oracle.field_paths(pos, nrm, w, idx, S, K, 0.0, 2 * np.pi / 0.5)
+    ref = oracle.field_paths(pos, nrm, w, idx, S, K, 0.0, 2 * np.pi / 0.5, eta=eta)
     scale = np.abs(ref).max()
     assert np.abs(net - ref).max() < FIELD_RTOL * scale
     assert net.dtype == np.complex128 and net.shape == (S, K, 3)

@@ -34,14 +34,17 @@ def _index(rng, n, V, ncol):
     return idx
 
 
-@pytest.mark.parametrize("ncol", [2, 3, 4])
-def test_time_paths_match_oracle(native, oracle, ncol):
-    print(f"time paths ncol={ncol}", flush=True)
+@pytest.mark.parametrize("ncol,magnetic", [(2, False), (3, False), (4, False), (3, True)])
+def test_time_paths_match_oracle(native, oracle, ncol, magnetic):
+    """Per-path amplitudes and arrivals; with magnetic=True every third point
+    is magnetic (impedance 376.73 ohm), launching Re(w) x d / eta."""
+    print(f"time paths ncol={ncol} magnetic={magnetic}", flush=True)
     rng = np.random.default_rng(ncol)
     pos, nrm, w = _table(rng, 50)
     idx = _index(rng, 50, 3000, ncol)
-    amp, arr = native.time_paths(pos, nrm, w, idx, 0.02, 2 * np.pi / 0.03)
-    amp_o, arr_o = oracle.time_paths(pos, nrm, w, idx, 0.02, 2 * np.pi / 0.03)
+    eta = np.where(np.arange(50) % 3 == 0, np.float32(376.73), 0.0) if magnetic else None
+    amp, arr = native.time_paths(pos, nrm, w, idx, 0.02, 2 * np.pi / 0.03, eta=eta)
+    amp_o, arr_o = oracle.time_paths(pos, nrm, w, idx, 0.02, 2 * np.pi / 0.03, eta=eta)
     amp, arr = amp.cpu().numpy(), arr.cpu().numpy()
     assert np.array_equal(np.isnan(amp), np.isnan(amp_o))
     scale = np.nanmax(np.abs(amp_o))
@@ -181,5 +184,16 @@ def test_timedomainv3_dropin(native, oracle):
     s, k = EM.path_slots(fi, S, K)
     ref, wake_o, _ = oracle.time_place(amp, arr, s * K + k, S * K, exc, 2e9, 400)
     assert tm.shape == (S, K, 400, 3) and wake == wake_o
+    np.testing.assert_allclose(tm, ref.reshape(S, K, 400, 3), rtol=1e-9,
+                               atol=1e-12 * np.abs(ref).max())
+    # magnetic sources 0 and 2 (free-space impedance of the table's constants)
+    table["Electric"][[0, 2]] = False
+    eta = np.zeros(12)
+    eta[[0, 2]] = np.float32(np.sqrt(np.float64(table["permeability"][0].real)
+                                     / np.float64(table["permittivity"][0].real)))
+    tm, wake = EM.TimeDomainv3(S, K, table, fi, 0.25, 0.1, exc, 2e9, 400, 0.0, 20.0)
+    amp, arr = oracle.time_paths(pos, nrm, w, fi, 0.0, 20.0, eta=eta)
+    ref, wake_o, _ = oracle.time_place(amp, arr, s * K + k, S * K, exc, 2e9, 400)
+    assert wake == wake_o and np.abs(ref).max() > 0
     np.testing.assert_allclose(tm, ref.reshape(S, K, 400, 3), rtol=1e-9,
                                atol=1e-12 * np.abs(ref).max())

@@ -182,8 +182,8 @@ def test_library_rejects_bad_arguments_without_gpu():
     lib = _native.load_library()
     rc = lib.lem_visibility_dense(None, 4, None, 4, None, 0, 7, 0, None, None, None)
     assert rc == -1 and b"semantics" in lib.lem_last_error()
-    rc = lib.lem_field_paths(None, None, None, 1, None, 5, None, None, None, 1, 0.0, 1.0, None,
-                             None, None)
+    rc = lib.lem_field_paths(None, None, None, None, 1, None, 5, None, None, None, 1, 0.0, 1.0,
+                             None, None, None)
     assert rc == -1 and b"columns" in lib.lem_last_error()
     assert lib.lem_field_los_workspace(100, 1000, 0) > 0
     assert lib.lem_field_los_workspace(100, 1000, _native.FIELD_ELEMENTS) == 0

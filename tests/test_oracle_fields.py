@@ -315,3 +315,54 @@ def test_four_column_paths_three_leg_factorisation(oracle):
     total = net.sum(0)
     # H and G are rounded to complex64 between the legs -> ~1e-6 relative
     assert np.allclose(total, E0 + E1 + E2, rtol=5e-6, atol=5e-6 * np.abs(total).max())
+
+
+ETA0 = float(np.sqrt(1.25663706212e-6 / 8.8541878187e-12))
+
+
+def test_magnetic_source_closed_form(oracle):
+    """A path from a magnetic point launches (w x d) / eta instead of w
+    (empropagation.py:1111-1146).  w = z with the sink on +x gives the field of
+    an electric source of weight y / eta (z x x = y); for random complex w and
+    random 2- and 3-point paths the field equals that of electric sources of
+    weight numpy.cross(w, d) / eta, d the unit direction of the first leg."""
+    # the closed-form case: z x x = +y (a transposed cross product gives -y)
+    pos = np.array([[0, 0, 0], [200, 0, 0]], np.float32)
+    nrm = np.array([[0, 0, 1], [1, 0, 0]], np.float32)
+    w = np.array([[0, 0, 1], [1, 1, 1]], np.complex64)
+    idx = np.array([[1, 2]])
+    eta = np.array([ETA0, 0.0])
+    Em = oracle.field_paths(pos, nrm, w, idx, 1, 1, 0.0, BETA, eta=eta)
+    we = np.array([[0, 1 / ETA0, 0], [1, 1, 1]], np.complex64)
+    Ee = oracle.field_paths(pos, nrm, we.astype(np.complex128), idx, 1, 1, 0.0, BETA)
+    assert np.abs(Em[0, 0, 1]) > 0 and Em[0, 0, 0] == 0 and Em[0, 0, 2] == 0
+    np.testing.assert_allclose(Em, Ee, rtol=1e-6)
+    # random paths: S sources, K sinks, P scatterers, half the sources magnetic
+    rng = np.random.default_rng(17)
+    S, K, P = 8, 5, 6
+    N = S + K + P
+    pos = rng.uniform(-30, 30, (N, 3)).astype(np.float32)
+    nrm = rng.normal(size=(N, 3)).astype(np.float32)
+    w = (rng.normal(size=(N, 3)) + 1j * rng.normal(size=(N, 3))).astype(np.complex64)
+    eta = np.where(np.arange(N) < S, np.where(np.arange(N) % 2 == 0, ETA0, 0.0), 0.0)
+    rows = [[s + 1, S + k + 1, 0] for s in range(S) for k in range(K)]
+    rows += [[s + 1, S + K + p + 1, S + k + 1] for s in range(S) for p in range(P)
+             for k in range(K)]
+    idx = np.array(rows)
+    Em = oracle.field_paths(pos, nrm, w, idx, S, K, 0.0, BETA, eta=eta)
+    Ee = np.zeros_like(Em)
+    for row in idx:
+        a, b = row[0] - 1, row[1] - 1
+        d = pos[b].astype(np.float64) - pos[a]
+        d /= np.linalg.norm(d)
+        wr = w.astype(np.complex128)
+        if eta[a] > 0:
+            wr[a] = np.cross(wr[a], d) / eta[a]
+        Ee += oracle.field_paths(pos, nrm, wr, row[None, :], S, K, 0.0, BETA)
+    # (the oracle takes weights as complex64: the cross product is rounded)
+    np.testing.assert_allclose(Em, Ee, rtol=1e-6, atol=1e-6 * np.abs(Ee).max())
+    # electric rows are unchanged; magnetic line-of-sight fields are orthogonal
+    # to the leg (the launch transform leaves w x d as it is)
+    el = oracle.field_paths(pos, nrm, w, idx, S, K, 0.0, BETA)
+    assert np.array_equal(Em[1::2], el[1::2])
+    assert not np.allclose(Em[0::2], el[0::2])

@@ -148,3 +148,33 @@ def test_wake_starts_at_one_second(oracle):
     out, wake, bins = oracle.time_place(amp, arrival, [0, 0], 1, [1.0], 1e9, 10)
     assert wake == 1.0
     assert (bins > 10).all() and not out.any()
+
+
+def test_magnetic_first_point(oracle):
+    """A magnetic first point launches Re(w) x d / eta (empropagation.py:
+    1395-1419): equal to an electric point of weight numpy.cross(Re w, d) / eta
+    with the same delay; w = z towards +x gives +y."""
+    eta0 = float(np.sqrt(1.25663706212e-6 / 8.8541878187e-12))
+    pos = np.array([[0, 0, 0], [200, 0, 0]], np.float32)
+    nrm = np.array([[0, 0, 1], [1, 0, 0]], np.float32)
+    w = np.array([[0, 0, 1], [1, 1, 1]], np.complex64)
+    amp, _ = oracle.time_paths(pos, nrm, w, np.array([[1, 2]], np.int32), 0.0, 2 * np.pi / 0.05,
+                               eta=np.array([eta0, 0.0]))
+    assert amp[0, 1] > 0 and amp[0, 0] == 0 and amp[0, 2] == 0
+    rng = np.random.default_rng(5)
+    pos, nrm, w = _table(rng, 12)
+    eta = np.where(np.arange(12) % 3 == 0, eta0, 0.0)
+    idx = np.array([[a, b, c] for a in range(1, 5) for b in range(5, 9) for c in (0, 10)],
+                   np.int32)
+    beta = 2 * np.pi / 0.05
+    am, arm = oracle.time_paths(pos, nrm, w, idx, 0.01, beta, eta=eta)
+    for v, row in enumerate(idx):
+        a, b = row[0] - 1, row[1] - 1
+        we = w.copy()
+        if eta[a] > 0:
+            d = pos[b].astype(np.float64) - pos[a]
+            d /= np.linalg.norm(d)
+            we[a] = (np.cross(w[a].real.astype(np.float64), d) / eta[a]) + 1j * w[a].imag
+        ae, are = oracle.time_paths(pos, nrm, we, row[None, :], 0.01, beta)
+        np.testing.assert_allclose(am[v], ae[0], rtol=1e-6, atol=1e-9 * np.abs(ae).max())
+        assert arm[v] == are[0]

@@ -43,19 +43,21 @@ def _noalloc(native, src, w, snk, nrm, wk, bits, alpha, beta, flags):
     return out.cpu().numpy()
 
 
-def _compare(got, ref, exact):
+def _compare(got, ref, exact, bound):
+    """Bit-identical when the chunking is lem_field_los's own; regrouped, the
+    per-sink difference is float32 summation rounding: below 1e-5 of (a bound
+    of) the sum of the sink's |terms|."""
     assert np.isfinite(got).all()
     if exact:
         assert np.array_equal(got, ref)
         return
     err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
-    big = np.abs(ref) >= 1e-3 * np.abs(ref).max()
-    per = (np.abs(got - ref)[big] / np.abs(ref[big])).max()
-    print(f"  regrouped partial sums: normwise {err:.2e}, per-entry {per:.2e}", flush=True)
-    assert err < 1e-5 and per < 1e-4
+    per = (np.abs(got - ref).max(axis=-1) / bound).max()
+    print(f"  regrouped partial sums: normwise {err:.2e}, per-sink {per:.2e}", flush=True)
+    assert err < 1e-5 and per < 1e-5
 
 
-def test_noalloc_equals_field_los(native):
+def test_noalloc_equals_field_los(native, term_bound):
     lib = native.load_library()
     kmax = lib.lem_field_los_noalloc_max_sinks()
     # (S, K): the first call is too wide for lem_field_los's chunk count to fit
@@ -78,7 +80,8 @@ def test_noalloc_equals_field_los(native):
                                    bits_sink_major=sm).cpu().numpy()
             got = _noalloc(native, src, w, snk, nrm, wk if use_wk else None, b, alpha,
                            2 * np.pi / 0.2, flags)
-            _compare(got, ref, exact)
+            _compare(got, ref, exact, None if exact else term_bound(
+                src, w, snk, nrm, wk if use_wk else None, alpha, 2 * np.pi / 0.2))
     # the per-element network and an empty source set pass through
     src, snk, nrm, w, wk, bits, _ = _inputs(native, 64, 100, 4)
     got = _noalloc(native, src[:0], w[:0], snk, nrm, None, None, 0.0, 1.0, 0)
