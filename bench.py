@@ -320,10 +320,44 @@ class C2:
         hit, dist = orc.visibility_sampled(sc["src"], sc["snk"], pairs, sc["tris"], nthreads=nth)
         dt = time.perf_counter() - t0
         self.cpu_sample = (pairs, hit, dist)
+        self.cpu_tiles = self._cpu_tiles(budget_s / 2, nth)
         return {"value": n / dt / 1e6, "unit": "Mrays/s", "cores": nth, "kind": "port",
                 "sample": f"{n} random source->sink rays of the 1.718e10, brute force over all "
                           f"{self.T} triangles (kernel1Dv3 closest hit, the reference "
                           f"algorithm), {dt:.1f} s"}
+
+    cpu_tiles = None
+
+    def _cpu_tiles(self, budget_s, nth):
+        """The reference's own acceleration structure on the CPU: the oracle's
+        restatement of Tile_acceleration_structure (vertex binning into 2 m
+        tiles over x, y; the tile rectangle between where each ray enters and
+        leaves the terrain's height slab scanned until a triangle hits), any-hit
+        masks with the numba predicate, on random rays of the launch; masks
+        compared with the GPU's."""
+        from oracle import oracle as orc
+
+        sc, torch = self.sc, self.torch
+        t0 = time.perf_counter()
+        tiles = orc.Tiles(sc["tris"], up=2, n_cells=250)
+        build_s = time.perf_counter() - t0
+        rng = np.random.default_rng(1)
+        n = 256
+        pairs = np.stack([rng.integers(0, self.S, n), rng.integers(0, self.K, n)], 1)
+        t0 = time.perf_counter()
+        tiles.cast(sc["src"], sc["snk"], pairs, nthreads=nth)
+        dt = time.perf_counter() - t0
+        n = int(max(256, min(10 ** 7, n * budget_s / max(dt, 1e-6))))
+        pairs = np.stack([rng.integers(0, self.S, n), rng.integers(0, self.K, n)], 1)
+        t0 = time.perf_counter()
+        hit = tiles.cast(sc["src"], sc["snk"], pairs, nthreads=nth)
+        dt = time.perf_counter() - t0
+        m_g = _bits_at(torch, self.bits, pairs[:, 1], pairs[:, 0])
+        return {"value": n / dt / 1e6, "unit": "Mrays/s", "cores": nth, "kind": "port",
+                "sample": f"{n} random rays of the launch, {dt:.1f} s; tiles "
+                          f"{tiles.na} x {tiles.nb} of {tiles.tile:.3f} m, {tiles.refs} "
+                          f"references, built in {build_s:.2f} s",
+                "mask_mismatches_vs_gpu": int((m_g != hit).sum())}
 
     cpu_sample = None
 
@@ -1053,6 +1087,8 @@ def main(argv=None) -> int:
         cpu = cfg.cpu_baseline(a.cpu_seconds)
         print(f"[bench] cpu baseline: {cpu['value']:.3f} {cpu['unit']}", file=sys.stderr,
               flush=True)
+        if getattr(cfg, "cpu_tiles", None) is not None:
+            rec["extra"]["cpu_baseline_tiles"] = cfg.cpu_tiles
     parity = None
     if not a.no_parity:
         t0 = time.perf_counter()

@@ -275,6 +275,215 @@ ORC_API void orc_visibility_sampled(const float *src, const float *snk, const in
 }
 
 /* ------------------------------------------------------------------------ */
+/* Tile acceleration (the reference's cuda branch), as a CPU baseline         */
+/* ------------------------------------------------------------------------ */
+
+/* Tile_acceleration_structure (models/acceleration_structures.py:57-70) bins
+ * each triangle into the 2-D tiles of its three vertices over the two
+ * horizontal axes, plus the fourth tile of the 2 x 2 block when the vertices
+ * fall in three different tiles (terrain_acceleration_build.cuh:57-112; bins
+ * of one triangle must be adjacent, :114-133).  The reference takes x as the
+ * vertical axis and tiles (y, z); `up` names the vertical axis here and the
+ * tiles span the other two (a < b).  Tiles are `tile` wide from lo_a, lo_b.
+ * The triangle ids of tile (i, j) are id[start[i * nb + j] .. start[... + 1]). */
+typedef struct {
+    int up, a, b;
+    float lo_a, lo_b, tile;
+    int na, nb;
+    float bottom, top;  /* extent of the structure along `up` */
+    int64_t *start;
+    int32_t *id;
+    const float *tris;
+    int64_t T;
+} orc_tiles;
+
+static inline int tile_coord(float p, float lo, float tile) { return (int)floorf((p - lo) / tile); }
+
+ORC_API void *orc_tiles_build(const float *tris, int64_t T, int up, float lo_a, float lo_b,
+                              float tile, int na, int nb, float bottom, float top)
+{
+    orc_tiles *g = (orc_tiles *)calloc(1, sizeof(orc_tiles));
+    g->up = up;
+    g->a = up == 0 ? 1 : 0;
+    g->b = up == 2 ? 1 : 2;
+    g->lo_a = lo_a;
+    g->lo_b = lo_b;
+    g->tile = tile;
+    g->na = na;
+    g->nb = nb;
+    g->bottom = bottom;
+    g->top = top;
+    g->tris = tris;
+    g->T = T;
+    const int64_t nt = (int64_t)na * nb;
+    int64_t *cnt = (int64_t *)calloc((size_t)nt + 1, sizeof(int64_t));
+    int32_t *bins = (int32_t *)malloc(sizeof(int32_t) * 4 * (size_t)(T > 0 ? T : 1));
+    for (int64_t t = 0; t < T; ++t) {
+        int ia[3], ib[3];
+        for (int v = 0; v < 3; ++v) {
+            ia[v] = tile_coord(tris[9 * t + 3 * v + g->a], lo_a, tile);
+            ib[v] = tile_coord(tris[9 * t + 3 * v + g->b], lo_b, tile);
+        }
+        int n = 0;
+        int32_t *bt = bins + 4 * t;
+        for (int v = 0; v < 3; ++v) {
+            int c = ia[v] * nb + ib[v];
+            int dup = 0;
+            for (int u = 0; u < n; ++u)
+                dup |= bt[u] == c;
+            if (!dup)
+                bt[n++] = c;
+        }
+        if (n == 3) {
+            /* the corner that completes the 2 x 2 block: on each axis the
+             * coordinate that occurs once among the three tiles */
+            int fa = ia[0] == ia[1] ? ia[2] : (ia[0] == ia[2] ? ia[1] : ia[0]);
+            int fb = ib[0] == ib[1] ? ib[2] : (ib[0] == ib[2] ? ib[1] : ib[0]);
+            bt[n++] = fa * nb + fb;
+        }
+        for (int u = n; u < 4; ++u)
+            bt[u] = -1;
+        for (int u = 0; u < n; ++u)
+            if (bt[u] >= 0 && bt[u] < nt)
+                cnt[bt[u] + 1]++;
+    }
+    for (int64_t c = 0; c < nt; ++c)
+        cnt[c + 1] += cnt[c];
+    g->start = cnt;
+    g->id = (int32_t *)malloc(sizeof(int32_t) * (size_t)(cnt[nt] > 0 ? cnt[nt] : 1));
+    int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nt > 0 ? nt : 1));
+    memcpy(fill, cnt, sizeof(int64_t) * (size_t)nt);
+    for (int64_t t = 0; t < T; ++t)
+        for (int u = 0; u < 4; ++u) {
+            int32_t c = bins[4 * t + u];
+            if (c >= 0 && c < nt)
+                g->id[fill[c]++] = (int32_t)t;
+        }
+    free(fill);
+    free(bins);
+    return g;
+}
+
+ORC_API int64_t orc_tiles_refs(const void *h)
+{
+    const orc_tiles *g = (const orc_tiles *)h;
+    return g->start[(int64_t)g->na * g->nb];
+}
+
+ORC_API void orc_tiles_free(void *h)
+{
+    orc_tiles *g = (orc_tiles *)h;
+    if (!g)
+        return;
+    free(g->start);
+    free(g->id);
+    free(g);
+}
+
+/* ray_plane_intersection (raycasting_accelerated.cuh:20-26) with the plane
+ * up = level, float32: the point o - d / t * dir with d = o.up - level, t = dir.up */
+static inline void plane_point(const orc_tiles *g, const float *o, const float *dir, float level,
+                               float *pa, float *pb)
+{
+    const float d = o[g->up] - level;
+    const float t = dir[g->up];
+    *pa = o[g->a] - d / t * dir[g->a];
+    *pb = o[g->b] - d / t * dir[g->b];
+}
+
+/* One ray of the tile DDA (raycasting_accelerated.cuh:106-268): the tiles
+ * spanned by where the segment enters and leaves the structure's slab along
+ * `up` -- the whole rectangle between them, row by row in the direction of
+ * travel -- are scanned and every triangle binned there is tested until one
+ * hits.  A segment entirely above or below the structure tests nothing.
+ * sem 0: numba predicate with the ray's length as window; 1: native culled. */
+static int tiles_cast(const orc_tiles *g, const float *o, const float *e, int sem)
+{
+    float dir[3], L;
+    charge_one(o, e, dir, &L);
+    const int up = g->up;
+    const int o_above = o[up] > g->top, o_below = o[up] < g->bottom;
+    const int o_in = !o_above && !o_below;
+    const int e_above = e[up] > g->top, e_below = e[up] < g->bottom;
+    const int e_in = !e_above && !e_below;
+    float sa, sb, da, db;
+    if (o[up] > e[up]) {
+        if (o_above && e_below) {
+            plane_point(g, o, dir, g->top, &sa, &sb);
+            plane_point(g, o, dir, g->bottom, &da, &db);
+        } else if (o_above && e_in) {
+            plane_point(g, o, dir, g->top, &sa, &sb);
+            da = e[g->a];
+            db = e[g->b];
+        } else if (e_below && o_in) {
+            sa = o[g->a];
+            sb = o[g->b];
+            plane_point(g, o, dir, g->bottom, &da, &db);
+        } else if (e_in && o_in) {
+            sa = o[g->a];
+            sb = o[g->b];
+            da = e[g->a];
+            db = e[g->b];
+        } else {
+            return 0;
+        }
+    } else {
+        if (o_below && e_above) {
+            plane_point(g, o, dir, g->bottom, &sa, &sb);
+            plane_point(g, o, dir, g->top, &da, &db);
+        } else if (o_below && e_in) {
+            plane_point(g, o, dir, g->bottom, &sa, &sb);
+            da = e[g->a];
+            db = e[g->b];
+        } else if (e_above && o_in) {
+            sa = o[g->a];
+            sb = o[g->b];
+            plane_point(g, o, dir, g->top, &da, &db);
+        } else if (e_in && o_in) {
+            sa = o[g->a];
+            sb = o[g->b];
+            da = e[g->a];
+            db = e[g->b];
+        } else {
+            return 0;
+        }
+    }
+    const int ta = tile_coord(sa, g->lo_a, g->tile), tb = tile_coord(sb, g->lo_b, g->tile);
+    const int ua = tile_coord(da, g->lo_a, g->tile), ub = tile_coord(db, g->lo_b, g->tile);
+    /* step_direction (:44-48): the sign of the ray's direction on each axis */
+    const int step_a = dir[g->a] < 0 ? -1 : 1, step_b = dir[g->b] < 0 ? -1 : 1;
+    const int ra = abs(ua - ta) + 1, rb = abs(ub - tb) + 1;
+    for (int i = 0; i < ra; ++i)
+        for (int j = 0; j < rb; ++j) {
+            const int ca = ta + i * step_a, cb = tb + j * step_b;
+            if (ca < 0 || ca >= g->na || cb < 0 || cb >= g->nb)
+                continue;
+            const int64_t c = (int64_t)ca * g->nb + cb;
+            for (int64_t q = g->start[c]; q < g->start[c + 1]; ++q) {
+                const float *tri = g->tris + 9 * (int64_t)g->id[q];
+                double t;
+                if (sem == 1 ? orc_hit_native(o, dir, L, tri) : orc_hit_numba(o, dir, L, tri, &t))
+                    return 1;
+            }
+        }
+    return 0;
+}
+
+/* Any-hit masks of explicit (source row, sink row) pairs through the tiles. */
+ORC_API void orc_tiles_cast(const void *h, const float *src, const float *snk,
+                            const int64_t *pairs, int64_t n, int sem, uint8_t *hit, int nthreads)
+{
+    const orc_tiles *g = (const orc_tiles *)h;
+#ifdef _OPENMP
+    if (nthreads > 0)
+        omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i)
+        hit[i] = (uint8_t)tiles_cast(g, src + 3 * pairs[2 * i], snk + 3 * pairs[2 * i + 1], sem);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Fields (float64)                                                          */
 /* ------------------------------------------------------------------------ */
 

@@ -75,6 +75,14 @@ def lib():
                                     ctypes.c_double, ctypes.c_int, P, ctypes.c_int]
         L.orc_field_paths.argtypes = [P, P, P, P, i64, P, i64, ctypes.c_int, i64, i64,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, P]
+        f32 = ctypes.c_float
+        L.orc_tiles_build.argtypes = [P, i64, ctypes.c_int, f32, f32, f32, ctypes.c_int,
+                                      ctypes.c_int, f32, f32]
+        L.orc_tiles_build.restype = P
+        L.orc_tiles_refs.argtypes = [P]
+        L.orc_tiles_refs.restype = i64
+        L.orc_tiles_free.argtypes = [P]
+        L.orc_tiles_cast.argtypes = [P, P, P, P, i64, ctypes.c_int, P, ctypes.c_int]
         L.orc_py_floordiv.argtypes = [ctypes.c_double, ctypes.c_double]
         L.orc_py_floordiv.restype = ctypes.c_double
         L.orc_time_paths.argtypes = [P, P, P, P, i64, P, i64, ctypes.c_int, ctypes.c_double,
@@ -165,6 +173,48 @@ def visibility_sampled(src, snk, pairs, tris9, any_hit_only=False, nthreads=None
                                  int(any_hit_only), _p(hit), _p(dist),
                                  nthreads or nthreads_default())
     return hit.astype(bool), dist
+
+
+class Tiles:
+    """The reference's tile acceleration (Tile_acceleration_structure,
+    models/acceleration_structures.py:57-70, with the binning of
+    terrain_acceleration_build.cuh:57-133 and the tile DDA of
+    raycasting_accelerated.cuh:106-268), restated in C: triangles binned into
+    2-D tiles over the two axes other than `up` by their vertices, and for each
+    ray every tile of the rectangle between where it enters and leaves the
+    structure's slab scanned until a triangle hits.  Bounds carry the
+    reference's 0.1 % padding; the tile is min(horizontal extents) / n_cells.
+    Any-hit masks only (the reference's structure returns no distances)."""
+
+    def __init__(self, tris9, up=2, n_cells=256):
+        self.tr = _f32(tris9).reshape(-1, 9)
+        pts = self.tr.reshape(-1, 3).astype(np.float64)
+        mx, mn = pts.max(0), pts.min(0)
+        hi, lo = mx + np.abs(mx) * 0.001, mn - np.abs(mn) * 0.001
+        a, b = [x for x in range(3) if x != up]
+        tile = min(hi[a] - lo[a], hi[b] - lo[b]) / n_cells
+        self.na = int(np.ceil((hi[a] - lo[a]) / tile))
+        self.nb = int(np.ceil((hi[b] - lo[b]) / tile))
+        self.tile = tile
+        self.h = lib().orc_tiles_build(_p(self.tr), self.tr.shape[0], int(up), float(lo[a]),
+                                       float(lo[b]), float(tile), self.na, self.nb,
+                                       float(lo[up]), float(hi[up]))
+        self.refs = int(lib().orc_tiles_refs(self.h))
+
+    def cast(self, src, snk, pairs, sem=0, nthreads=None):
+        """Blocked flags of (source row, sink row) pairs (sem 0: numba
+        predicate, 1: native culled)."""
+        s, k = _f32(src).reshape(-1, 3), _f32(snk).reshape(-1, 3)
+        pr = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
+        hit = np.zeros(pr.shape[0], np.uint8)
+        lib().orc_tiles_cast(self.h, _p(s), _p(k), _p(pr), pr.shape[0], int(sem), _p(hit),
+                             nthreads or nthreads_default())
+        return hit.astype(bool)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_tiles_free(self.h)
+            self.h = None
 
 
 # --------------------------------------------------------------------------
