@@ -620,6 +620,7 @@ class C4:
                          "frac": pairs * PAIR_FLOPS / (fwd * 1e-3) / PEAK_FP32, "traffic": None},
             "extra": {"forward_ms": fwd, "loss_and_adjoint_ms": bwd,
                       "adjoint_over_forward": bwd / fwd,
+                      "gradient_step_over_forward": (fwd + bwd) / fwd,
                       "forward_gpairs_per_s": pairs / (fwd * 1e-3) / 1e9},
         }
 
@@ -813,7 +814,7 @@ class TD:
         st = nat.stream_ptr()
         if ev is not None:
             ev[0].record()
-        nat.check(lib.lem_time_paths(p(self.pos), p(self.nrm), p(self.w), self.pos.shape[0],
+        nat.check(lib.lem_time_paths(p(self.pos), p(self.nrm), p(self.w), None, self.pos.shape[0],
                                      p(self.index), self.V, 2, 0.0, float(self.sc["beta"]),
                                      p(self.amp), p(self.arr), p(self.err), st), "time_paths")
         if ev is not None:
