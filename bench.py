@@ -588,6 +588,9 @@ class C4:
         self.snk_n = torch.from_numpy(sc["snk_n"]).to(dev)
         self.in_beam = torch.from_numpy(sc["in_beam"]).to(dev)
         self.S, self.K = len(sc["src"]), len(sc["snk"])
+        # the forward over a spatial order of the sinks (fixed sinks and mask)
+        from lyceanem.differentiable import FarFieldPlan
+        self.plan = FarFieldPlan(self.snk, self.snk_n, self.bits, center=sc["src"].mean(0))
 
     def rays(self):
         return self.S * self.K
@@ -599,7 +602,8 @@ class C4:
         pos = self.pos.clone().requires_grad_(True)
         if ev is not None:
             ev[0].record()
-        E = far_field(w, pos, self.snk, self.snk_n, hit_bits=self.bits, beta=self.sc["beta"])
+        E = far_field(w, pos, self.snk, self.snk_n, hit_bits=self.bits, beta=self.sc["beta"],
+                      plan=self.plan)
         if ev is not None:
             ev[1].record()
         loss = db_mask_loss(E, self.in_beam)
@@ -644,15 +648,16 @@ class C4:
                                           f"elements x {self.K} sinks, {dt:.1f} s"}
 
     def parity(self, n_sinks=4096):
-        """The forward field on n_sinks random sinks against the float64 oracle
-        over every element, given the frozen mask."""
+        """The forward field (through the plan, as timed) on n_sinks random
+        sinks against the float64 oracle over every element, given the frozen
+        mask."""
         from oracle import oracle as orc
         from lyceanem.differentiable import far_field
 
         sc = self.sc
         with self.torch.no_grad():
             E = far_field(self.w, self.pos, self.snk, self.snk_n, hit_bits=self.bits,
-                          beta=sc["beta"]).cpu().numpy()
+                          beta=sc["beta"], plan=self.plan).cpu().numpy()
         ks = np.sort(np.random.default_rng(777).choice(self.K, n_sinks, replace=False))
         hit = self.nat.unpack_bits(self.bits, self.K)[:, ks]
         ref = orc.field_los(sc["src"], sc["w"], sc["snk"][ks], sc["snk_n"][ks], 0.0, sc["beta"],

@@ -1,6 +1,6 @@
 """Differentiable line-of-sight far field for gradient-based beam synthesis.
 
-`FarField.apply(w, pos, sinks, sink_normals, hit_bits, alpha, beta)` returns the
+`FarField.apply(w, pos, sinks, sink_normals, hit_bits, alpha, beta, plan)` returns the
 source-summed field E (K, 3) complex64 of `lem_field_los` and differentiates it
 with the hand-written adjoint kernel `lem_field_los_adjoint`: gradients flow to
 the complex excitation weights w (S, 3) and the source positions pos (S, 3);
@@ -16,11 +16,77 @@ import torch
 from . import _native
 
 
+def sink_tile_order(sinks, center=None):
+    """A spatial order of the sinks: 2-D Morton order of their directions seen
+    from `center` (default: the origin), polar and azimuth angle quantised on
+    one angular step (2 pi / 1023), so that 64 consecutive sinks of this order
+    form a compact, roughly square patch of directions and the 64 lanes of a
+    wavefront see nearly the same sources blocked.  (On a 1-degree
+    elevation / azimuth grid seen by a 4,096-element cylinder, 6.8 % of
+    (wavefront, element) pairs have lanes that disagree about visibility,
+    against 35 % in elevation-major order and 10.5 % when the polar angle gets
+    twice the azimuth's resolution.)"""
+    p = torch.as_tensor(sinks).to(torch.float64)
+    if center is not None:
+        p = p - torch.as_tensor(center, dtype=torch.float64, device=p.device)
+    r = torch.linalg.vector_norm(p, dim=1).clamp_min(1e-300)
+    th = torch.acos((p[:, 2] / r).clamp(-1.0, 1.0)) / (2 * np.pi)
+    ph = (torch.atan2(p[:, 1], p[:, 0]) + np.pi) / (2 * np.pi)
+    qa = (th * 1023.0).round().to(torch.int64)
+    qb = (ph * 1023.0).round().to(torch.int64)
+    key = torch.zeros_like(qa)
+    for b in range(10):
+        key |= ((qa >> b) & 1) << (2 * b + 1)
+        key |= ((qb >> b) & 1) << (2 * b)
+    return torch.argsort(key, stable=True)
+
+
+def permute_mask_sinks(hit_bits, K, order):
+    """The source-major mask (S, ceil(K/64)) int64 with its sink columns taken
+    in `order`: bit j of the result's row s is bit order[j] of row s."""
+    S, W = hit_bits.shape
+    sh = torch.arange(64, device=hit_bits.device, dtype=torch.int64)
+    m = ((hit_bits.unsqueeze(-1) >> sh) & 1).reshape(S, W * 64)[:, :K]
+    m = m[:, order.to(hit_bits.device)]
+    out = torch.zeros((S, W * 64), dtype=torch.int64, device=hit_bits.device)
+    out[:, :K] = m
+    return (out.reshape(S, W, 64) << sh).sum(dim=-1)
+
+
+class FarFieldPlan:
+    """A fixed sink set and mask prepared for repeated far-field evaluations
+    (beam synthesis): the forward kernel runs over the sinks in
+    `sink_tile_order` with the mask's columns permuted to match, and the field
+    comes back in the caller's order.  Each sink's sum covers the same sources
+    in the same order, so the field is bit-identical to the unplanned one; it
+    is faster because a wave's lanes agree on the mask (on the 4,096-element
+    cylinder and a 181 x 361 far-field grid: 0.566 -> 0.457 ms).  The adjoint
+    keeps the caller's order (its lanes are sources)."""
+
+    def __init__(self, sinks, sink_normals, hit_bits=None, center=None):
+        self.K = sinks.shape[0]
+        self.order = sink_tile_order(sinks, center).to(sinks.device)
+        self.sinks = sinks[self.order].contiguous()
+        self.sink_normals = sink_normals[self.order].contiguous()
+        self.hit_bits = None if hit_bits is None else permute_mask_sinks(hit_bits, self.K,
+                                                                         self.order)
+        self.inverse = torch.empty_like(self.order)
+        self.inverse[self.order] = torch.arange(self.K, device=self.order.device)
+
+    def forward_field(self, w, pos, alpha, beta):
+        E = _native.field_los(pos, w, self.sinks, self.sink_normals, hit_bits=self.hit_bits,
+                              alpha=alpha, beta=beta)
+        return E[self.inverse.to(E.device)]
+
+
 class FarField(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, w, pos, sinks, sink_normals, hit_bits, alpha, beta):
-        E = _native.field_los(pos.detach(), w.detach(), sinks, sink_normals, hit_bits=hit_bits,
-                              alpha=alpha, beta=beta)
+    def forward(ctx, w, pos, sinks, sink_normals, hit_bits, alpha, beta, plan=None):
+        if plan is not None:
+            E = plan.forward_field(w.detach(), pos.detach(), alpha, beta)
+        else:
+            E = _native.field_los(pos.detach(), w.detach(), sinks, sink_normals,
+                                  hit_bits=hit_bits, alpha=alpha, beta=beta)
         ctx.save_for_backward(w.detach(), pos.detach(), sinks, sink_normals)
         ctx.hit_bits = hit_bits
         ctx.alpha, ctx.beta = alpha, beta
@@ -33,11 +99,13 @@ class FarField(torch.autograd.Function):
             pos, w, sinks, sink_normals, grad_E.contiguous(), hit_bits=ctx.hit_bits,
             alpha=ctx.alpha, beta=ctx.beta, want_w=ctx.needs_input_grad[0],
             want_p=ctx.needs_input_grad[1])
-        return gw, gp, None, None, None, None, None
+        return gw, gp, None, None, None, None, None, None
 
 
-def far_field(w, pos, sinks, sink_normals, hit_bits=None, alpha=0.0, beta=2 * np.pi):
-    return FarField.apply(w, pos, sinks, sink_normals, hit_bits, float(alpha), float(beta))
+def far_field(w, pos, sinks, sink_normals, hit_bits=None, alpha=0.0, beta=2 * np.pi, plan=None):
+    """Differentiable source-summed field (K, 3).  plan: a FarFieldPlan of the
+    same sinks, normals and mask (forward in a spatial sink order)."""
+    return FarField.apply(w, pos, sinks, sink_normals, hit_bits, float(alpha), float(beta), plan)
 
 
 def db_mask_loss(E, in_beam, sidelobe_db=-25.0, ref_power=None, eps=1e-30):
