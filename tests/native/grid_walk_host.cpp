@@ -300,6 +300,38 @@ Scene terrain(uint64_t seed, int nqx, int nqy, int ns, int nk)
     return sc;
 }
 
+// The terrain under a tall, mostly empty sky (a few small triangles float
+// 30-40 m up, so the grid's upper blocks are empty): sources high above the
+// surface, sinks near it, some on the vertex columns and rows -- rays cross
+// long runs of empty macro blocks before they reach the surface, and some skim
+// just above it.
+Scene terrain_sky(uint64_t seed, int nqx, int nqy, int ns, int nk)
+{
+    Scene sc = terrain(seed, nqx, nqy, 0, 0);
+    sc.name = "terrain-sky";
+    Rng g(seed + 1000);
+    for (int a = 0; a < 6; ++a) {
+        const double x = g.uni(0, nqx), y = g.uni(0, nqy), z = g.uni(30, 40);
+        push3(sc.tris, x, y, z);
+        push3(sc.tris, x + 0.5, y, z);
+        push3(sc.tris, x, y + 0.5, z + 0.2);
+    }
+    const int cols[3] = {nqx / 4, nqx / 2, 3 * nqx / 4};
+    for (int a = 0; a < ns; ++a) {
+        double x = g.uni(0, nqx), y = g.uni(0, nqy);
+        if (a % 4 == 0)
+            x = cols[(int)g.uni(0, 3)];
+        push3(sc.src, x, y, g.uni(6.0, 36.0));
+    }
+    for (int a = 0; a < nk; ++a) {
+        double x = g.uni(0, nqx), y = g.uni(0, nqy);
+        if (a % 4 == 0)
+            x = cols[(int)g.uni(0, 3)];
+        push3(sc.snk, x, y, a % 2 ? g.uni(-4.0, 6.0) : g.uni(6.0, 36.0));
+    }
+    return sc;
+}
+
 // Points exactly on the grid's own cell boundaries (the float32 values the
 // walk steps on, lo + cell * i): rays along cell planes, along cell edge lines
 // and through cell corners, over the terrain of `base`.
@@ -414,6 +446,7 @@ int main(int argc, char **argv)
     const Scene clu = clutter(12, 1500, 70 / q, 70 / q, 0.3);
     const Scene pla = planes(13, 1200, 60 / q, 60 / q);
     const Scene far = far_scene(14, 800, 50 / q, 60 / q);
+    const Scene sky = terrain_sky(15, 64, 48, 110 / q, 110 / q);
     for (float cpt : {1.0f, 2.0f, 4.0f})
         check_scene(ter, cpt, -1);
     check_scene(cb1, 2.0f, -1);
@@ -426,6 +459,8 @@ int main(int argc, char **argv)
     check_scene(pla, 1.0f, 1);
     check_scene(far, 2.0f, -1);
     check_scene(far, 4.0f, 3);
+    for (int ms = 0; ms <= 2; ++ms)
+        check_scene(sky, 2.0f, ms);
     std::printf("total: %lld rays cast, %lld mismatches\n", (long long)g_rays,
                 (long long)g_bad);
     return g_bad == 0 ? 0 : 1;

@@ -9,7 +9,8 @@ rays, the walk from the sink end, both predicates), on terrains whose rays lie
 in vertex planes (shared-edge near-ties that the kernel1Dv3 distance replay
 must resolve), points exactly on the grid's float32 cell boundaries, rays
 along grid planes and zero-length rays, origins 300 m to 30 km from a 60 m
-scene, macro blocks of 1 to 16 cells and 1 to 4 cells per triangle.  A second
+scene, a terrain under a tall empty sky (long runs of empty macro blocks),
+macro blocks of 1 to 16 cells and 1 to 4 cells per triangle.  A second
 build runs under AddressSanitizer and UndefinedBehaviorSanitizer (including
 float-to-int conversion overflow)."""
 import os
