@@ -591,15 +591,29 @@ class C4:
         # the forward over a spatial order of the sinks (fixed sinks and mask)
         from lyceanem.differentiable import FarFieldPlan
         self.plan = FarFieldPlan(self.snk, self.snk_n, self.bits, center=sc["src"].mean(0))
+        # the gradient step (forward, loss, adjoint: all on the device) captured
+        # once in a HIP graph; each timed step copies the current weights and
+        # positions into the captured leaves and replays it
+        self.wg = self.w.clone().requires_grad_(True)
+        self.pg = self.pos.clone().requires_grad_(True)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(3):
+                self._eager(self.wg, self.pg)
+        torch.cuda.current_stream().wait_stream(side)
+        self.wg.grad = None
+        self.pg.grad = None
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.g_loss = self._eager(self.wg, self.pg)
 
     def rays(self):
         return self.S * self.K
 
-    def step(self, ev=None):
+    def _eager(self, w, pos, ev=None):
         from lyceanem.differentiable import far_field, db_mask_loss
 
-        w = self.w.clone().requires_grad_(True)
-        pos = self.pos.clone().requires_grad_(True)
         if ev is not None:
             ev[0].record()
         E = far_field(w, pos, self.snk, self.snk_n, hit_bits=self.bits, beta=self.sc["beta"],
@@ -610,11 +624,31 @@ class C4:
         loss.backward()
         if ev is not None:
             ev[2].record()
-        self.loss = loss
+        return loss.detach()
+
+    def step(self, ev=None):
+        if ev is not None:
+            ev[0].record()
+        with self.torch.no_grad():
+            self.wg.copy_(self.w)
+            self.pg.copy_(self.pos)
+        self.graph.replay()
+        if ev is not None:
+            ev[1].record()
+            ev[2].record()
+        self.loss = self.g_loss
 
     def record(self, times_ms):
-        fwd = float(np.mean([a for a, _ in times_ms]))
-        bwd = float(np.mean([b for _, b in times_ms]))
+        torch = self.torch
+        graph_ms = float(np.mean([a for a, _ in times_ms]))
+        # forward / loss + adjoint split from eager steps timed with events
+        evs = [_events(torch, 3) for _ in range(10)]
+        for e in evs:
+            self._eager(self.w.clone().requires_grad_(True), self.pos.clone().requires_grad_(True),
+                        e)
+        torch.cuda.synchronize()
+        fwd = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+        bwd = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
         pairs = self.rays()
         return {
             "kernel": "k_field_los_partial (forward) / k_adjoint_partial (backward)",
@@ -622,7 +656,8 @@ class C4:
             "roofline": {"bound": "valu", "achieved": pairs * PAIR_FLOPS / (fwd * 1e-3) / 1e12,
                          "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                          "frac": pairs * PAIR_FLOPS / (fwd * 1e-3) / PEAK_FP32, "traffic": None},
-            "extra": {"forward_ms": fwd, "loss_and_adjoint_ms": bwd,
+            "extra": {"graph_step_ms": graph_ms,
+                      "forward_ms": fwd, "loss_and_adjoint_ms": bwd,
                       "adjoint_over_forward": bwd / fwd,
                       "gradient_step_over_forward": (fwd + bwd) / fwd,
                       "forward_gpairs_per_s": pairs / (fwd * 1e-3) / 1e9},
