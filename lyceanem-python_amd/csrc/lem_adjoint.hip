@@ -113,95 +113,105 @@ __global__ __launch_bounds__(AB) void k_adjoint_partial(
         if (!active)
             continue;
         unsigned long long word = 0;
+        for (int j0 = 0; j0 < n; j0 += SUM_GROUP) {
+            const int j1 = (j0 + SUM_GROUP) < n ? (j0 + SUM_GROUP) : n;
+            float h[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 2
-        for (int j = 0; j < n; ++j) {
-            const int64_t k = base + j;
-            if (hit_bits) {
-                if (j == 0 || (k & 63) == 0)
-                    word = hit_bits[s * W64 + (k >> 6)];
-                if ((word >> (k & 63)) & 1ull)
-                    continue;
-            }
-            const SnkLds &L = s_k[j];
-            double dx = L.x - xs, dy = L.y - ys, dz = L.z - zs;
-            Phase p = phase_of(dx, dy, dz, beta);
-            const float ir = p.inv_r;
-            const float ux = (float)dx * ir, uy = (float)dy * ir, uz = (float)dz * ir;
-            const float cs = ux * L.nx + uy * L.ny + uz * L.nz;
-            // G = e^{-alpha r}/r e^{-j beta r}
-            float amp = ir;
-            if (LOSSY)
-                amp *= expf(-alpha * (float)p.r);
-            const float Gr = amp * p.c, Gi = amp * p.s;
-            // F = FRONT (-g - 1/r) G ; F' = FRONT (g^2 + 2g/r + 2/r^2) G
-            float2 F = cmulf(-alpha - ir, -beta_f, Gr, Gi);
-            F.x *= FRONT;
-            F.y *= FRONT;
-            const float b_r = alpha * alpha - beta_f * beta_f + 2.0f * alpha * ir + 2.0f * ir * ir;
-            const float b_i = 2.0f * alpha * beta_f + 2.0f * beta_f * ir;
-            float2 dF = cmulf(b_r, b_i, Gr, Gi);
-            dF.x *= FRONT;
-            dF.y *= FRONT;
-            // Q = P(d) w,  uw = d.w
-            const float uwr = ux * w.xr + uy * w.yr + uz * w.zr;
-            const float uwi = ux * w.xi + uy * w.yi + uz * w.zi;
-            const float Qxr = w.xr - ux * uwr, Qxi = w.xi - ux * uwi;
-            const float Qyr = w.yr - uy * uwr, Qyi = w.yi - uy * uwi;
-            const float Qzr = w.zr - uz * uwr, Qzi = w.zi - uz * uwi;
-            // gv = gE_k, U = conj(gv)
-            const float gxr = L.g[0], gxi = L.g[1], gyr = L.g[2], gyi = L.g[3], gzr = L.g[4],
-                        gzi = L.g[5];
-            // weight gradient: conj(F cos) (gv - d (d.gv))
-            {
-                const float ugr = ux * gxr + uy * gyr + uz * gzr;
-                const float ugi = ux * gxi + uy * gyi + uz * gzi;
-                const float Pxr = gxr - ux * ugr, Pxi = gxi - ux * ugi;
-                const float Pyr = gyr - uy * ugr, Pyi = gyi - uy * ugi;
-                const float Pzr = gzr - uz * ugr, Pzi = gzi - uz * ugi;
-                const float Lr = F.x * cs, Li = -F.y * cs;  // conj(F cos)
-                float v[6];
-                v[0] = Lr * Pxr - Li * Pxi; v[1] = Lr * Pxi + Li * Pxr;
-                v[2] = Lr * Pyr - Li * Pyi; v[3] = Lr * Pyi + Li * Pyr;
-                v[4] = Lr * Pzr - Li * Pzi; v[5] = Lr * Pzi + Li * Pzr;
-                gw.add(v);
-            }
-            // position gradient
-            {
-                // U.Q and U.d with U = conj(g)
-                const float UQr = gxr * Qxr + gxi * Qxi + gyr * Qyr + gyi * Qyi + gzr * Qzr +
-                                  gzi * Qzi;
-                const float UQi = gxr * Qxi - gxi * Qxr + gyr * Qyi - gyi * Qyr + gzr * Qzi -
-                                  gzi * Qzr;
-                const float Udr = gxr * ux + gyr * uy + gzr * uz;
-                const float Udi = -(gxi * ux + gyi * uy + gzi * uz);
-                // M U = U - d (U.d)
-                const float MUxr = gxr - ux * Udr, MUxi = -gxi - ux * Udi;
-                const float MUyr = gyr - uy * Udr, MUyi = -gyi - uy * Udi;
-                const float MUzr = gzr - uz * Udr, MUzi = -gzi - uz * Udi;
-                // term1 coefficient on d: -F' cos (U.Q)
-                const float2 t1 = cmulf(dF.x, dF.y, UQr, UQi);
-                const float a1 = -t1.x * cs;
-                // term2: -F (U.Q) (n - d cos)/r  (real part only)
-                const float2 t2 = cmulf(F.x, F.y, UQr, UQi);
-                const float a2 = -t2.x * ir;
-                // term3: F cos ((d.w) M U + (U.d) Q)/r
-                const float2 Fc = make_float2(F.x * cs * ir, F.y * cs * ir);
-                const float2 c1 = cmulf(Fc.x, Fc.y, uwr, uwi);
-                const float2 c2 = cmulf(Fc.x, Fc.y, Udr, Udi);
-                float v[3];
-                v[0] = a1 * ux + a2 * (L.nx - ux * cs) + (c1.x * MUxr - c1.y * MUxi) +
-                       (c2.x * Qxr - c2.y * Qxi);
-                v[1] = a1 * uy + a2 * (L.ny - uy * cs) + (c1.x * MUyr - c1.y * MUyi) +
-                       (c2.x * Qyr - c2.y * Qyi);
-                v[2] = a1 * uz + a2 * (L.nz - uz * cs) + (c1.x * MUzr - c1.y * MUzi) +
-                       (c2.x * Qzr - c2.y * Qzi);
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    float y = v[i] - gpc[i];
-                    float t = gp[i] + y;
-                    gpc[i] = (t - gp[i]) - y;
-                    gp[i] = t;
+            for (int j = j0; j < j1; ++j) {
+                const int64_t k = base + j;
+                if (hit_bits) {
+                    if (j == 0 || (k & 63) == 0)
+                        word = hit_bits[s * W64 + (k >> 6)];
+                    if ((word >> (k & 63)) & 1ull)
+                        continue;
                 }
+                const SnkLds &L = s_k[j];
+                double dx = L.x - xs, dy = L.y - ys, dz = L.z - zs;
+                Phase p = phase_of(dx, dy, dz, beta);
+                const float ir = p.inv_r;
+                const float ux = (float)dx * ir, uy = (float)dy * ir, uz = (float)dz * ir;
+                const float cs = ux * L.nx + uy * L.ny + uz * L.nz;
+                // G = e^{-alpha r}/r e^{-j beta r}
+                float amp = ir;
+                if (LOSSY)
+                    amp *= expf(-alpha * (float)p.r);
+                const float Gr = amp * p.c, Gi = amp * p.s;
+                // F = FRONT (-g - 1/r) G ; F' = FRONT (g^2 + 2g/r + 2/r^2) G
+                float2 F = cmulf(-alpha - ir, -beta_f, Gr, Gi);
+                F.x *= FRONT;
+                F.y *= FRONT;
+                const float b_r = alpha * alpha - beta_f * beta_f + 2.0f * alpha * ir + 2.0f * ir * ir;
+                const float b_i = 2.0f * alpha * beta_f + 2.0f * beta_f * ir;
+                float2 dF = cmulf(b_r, b_i, Gr, Gi);
+                dF.x *= FRONT;
+                dF.y *= FRONT;
+                // Q = P(d) w,  uw = d.w
+                const float uwr = ux * w.xr + uy * w.yr + uz * w.zr;
+                const float uwi = ux * w.xi + uy * w.yi + uz * w.zi;
+                const float Qxr = w.xr - ux * uwr, Qxi = w.xi - ux * uwi;
+                const float Qyr = w.yr - uy * uwr, Qyi = w.yi - uy * uwi;
+                const float Qzr = w.zr - uz * uwr, Qzi = w.zi - uz * uwi;
+                // gv = gE_k, U = conj(gv)
+                const float gxr = L.g[0], gxi = L.g[1], gyr = L.g[2], gyi = L.g[3], gzr = L.g[4],
+                            gzi = L.g[5];
+                // weight gradient: conj(F cos) (gv - d (d.gv))
+                {
+                    const float ugr = ux * gxr + uy * gyr + uz * gzr;
+                    const float ugi = ux * gxi + uy * gyi + uz * gzi;
+                    const float Pxr = gxr - ux * ugr, Pxi = gxi - ux * ugi;
+                    const float Pyr = gyr - uy * ugr, Pyi = gyi - uy * ugi;
+                    const float Pzr = gzr - uz * ugr, Pzi = gzi - uz * ugi;
+                    const float Lr = F.x * cs, Li = -F.y * cs;  // conj(F cos)
+                    float v[6];
+                    v[0] = Lr * Pxr - Li * Pxi; v[1] = Lr * Pxi + Li * Pxr;
+                    v[2] = Lr * Pyr - Li * Pyi; v[3] = Lr * Pyi + Li * Pyr;
+                    v[4] = Lr * Pzr - Li * Pzi; v[5] = Lr * Pzi + Li * Pzr;
+#pragma unroll
+                    for (int i = 0; i < 6; ++i)
+                        h[i] += v[i];
+                }
+                // position gradient
+                {
+                    // U.Q and U.d with U = conj(g)
+                    const float UQr = gxr * Qxr + gxi * Qxi + gyr * Qyr + gyi * Qyi + gzr * Qzr +
+                                      gzi * Qzi;
+                    const float UQi = gxr * Qxi - gxi * Qxr + gyr * Qyi - gyi * Qyr + gzr * Qzi -
+                                      gzi * Qzr;
+                    const float Udr = gxr * ux + gyr * uy + gzr * uz;
+                    const float Udi = -(gxi * ux + gyi * uy + gzi * uz);
+                    // M U = U - d (U.d)
+                    const float MUxr = gxr - ux * Udr, MUxi = -gxi - ux * Udi;
+                    const float MUyr = gyr - uy * Udr, MUyi = -gyi - uy * Udi;
+                    const float MUzr = gzr - uz * Udr, MUzi = -gzi - uz * Udi;
+                    // term1 coefficient on d: -F' cos (U.Q)
+                    const float2 t1 = cmulf(dF.x, dF.y, UQr, UQi);
+                    const float a1 = -t1.x * cs;
+                    // term2: -F (U.Q) (n - d cos)/r  (real part only)
+                    const float2 t2 = cmulf(F.x, F.y, UQr, UQi);
+                    const float a2 = -t2.x * ir;
+                    // term3: F cos ((d.w) M U + (U.d) Q)/r
+                    const float2 Fc = make_float2(F.x * cs * ir, F.y * cs * ir);
+                    const float2 c1 = cmulf(Fc.x, Fc.y, uwr, uwi);
+                    const float2 c2 = cmulf(Fc.x, Fc.y, Udr, Udi);
+                    float v[3];
+                    v[0] = a1 * ux + a2 * (L.nx - ux * cs) + (c1.x * MUxr - c1.y * MUxi) +
+                           (c2.x * Qxr - c2.y * Qxi);
+                    v[1] = a1 * uy + a2 * (L.ny - uy * cs) + (c1.x * MUyr - c1.y * MUyi) +
+                           (c2.x * Qyr - c2.y * Qyi);
+                    v[2] = a1 * uz + a2 * (L.nz - uz * cs) + (c1.x * MUzr - c1.y * MUzi) +
+                           (c2.x * Qzr - c2.y * Qzi);
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+                        h[6 + i] += v[i];
+                }
+            }
+            gw.add(h);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                float y = h[6 + i] - gpc[i];
+                float t = gp[i] + y;
+                gpc[i] = (t - gp[i]) - y;
+                gp[i] = t;
             }
         }
     }

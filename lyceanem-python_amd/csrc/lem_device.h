@@ -216,10 +216,13 @@ struct C3 {
 };
 
 // e^{-j beta r} and 1/r for a path of float64 length components.
-// r = |delta| by one float64 Newton step on the float32 square root;
-// phase reduced modulo pi/2 in float64 (two-constant Cody-Waite), then
-// sine / cosine of the |x| <= pi/4 remainder from their Taylor series to x^9 /
-// x^10 (truncation error < 2e-9 on that interval) and the quadrant swap.
+// r = |delta| by one float64 Newton step on the float32 square root; 1/r is the
+// float32 reciprocal of that square root.  The phase is carried in quarter
+// turns, t = -beta r (2/pi): adding and removing 1.5 * 2^52 rounds t to the
+// nearest integer q (exact for |t| < 2^51) and leaves q mod 4 in the low
+// mantissa bits, and the remainder x = t - q (|x| <= 1/2) is exact in float64.
+// Sine / cosine of (pi/2) x come from their Taylor series to x^9 / x^10
+// (truncation error < 2e-9 on |x| <= pi/4), then the quadrant swap.
 struct Phase {
     float c, s;     // cos(-beta r), sin(-beta r)
     float inv_r;    // 1/r (float32)
@@ -245,23 +248,22 @@ LEM_DEV Phase phase_of(double dx, double dy, double dz, double beta)
     Phase p;
     double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
     float r0 = sqrtf_estimate((float)r2);   // ~1 ulp seed
-    float h = 0.5f * rcpf_estimate(r0);
+    float ir0 = rcpf_estimate(r0);
     double r0d = (double)r0;
     double e = __builtin_fma(-r0d, r0d, r2);
-    double r = __builtin_fma(e, (double)h, r0d);
+    double r = __builtin_fma(e, (double)(0.5f * ir0), r0d);
     p.r = r;
-    p.inv_r = rcpf_estimate((float)r);
-    double ph = -beta * r;
-    const double TWO_OVER_PI = 0.63661977236758134308;
-    const double PIO2_HI = 1.5707963267948966;     // float64(pi/2)
-    const double PIO2_LO = 6.123233995736766e-17;  // pi/2 - PIO2_HI
-    double q = __builtin_rint(ph * TWO_OVER_PI);
-    double xr = __builtin_fma(-q, PIO2_HI, ph);
-    xr = __builtin_fma(-q, PIO2_LO, xr);
+    p.inv_r = ir0;
+    const double NEG_TWO_OVER_PI = -0.63661977236758134308;
+    const double ROUND = 6755399441055744.0;  // 1.5 * 2^52
+    double t = r * (beta * NEG_TWO_OVER_PI);  // quarter turns
+    double m = t + ROUND;
+    double q = m - ROUND;
+    float x = (float)(t - q) * 1.57079632679489662f;
     float s, c;
-    sincos_reduced((float)xr, s, c);
-    // quadrant q mod 4, exact for |q| < 2^53 (q / 4 and its floor are exact)
-    const int qi = (int)__builtin_fma(-4.0, __builtin_floor(q * 0.25), q);
+    sincos_reduced(x, s, c);
+    // q mod 4 (two's complement of q in the low word of m)
+    const int qi = (int)(unsigned)__builtin_bit_cast(unsigned long long, m) & 3;
     // (cos, sin) of x + qi pi/2: quadrants 0..3 give (c, s), (-s, c), (-c, -s),
     // (s, -c) -- selected, not branched on
     const float cs = (qi & 1) ? s : c, sn = (qi & 1) ? c : s;
@@ -312,7 +314,13 @@ LEM_DEV C3 project(const C3 &w, float dx, float dy, float dz)
     return o;
 }
 
-// Kahan-compensated float32 accumulator for one complex 3-vector.
+// Kahan-compensated float32 accumulator for one complex 3-vector.  The source
+// and sink loops add plain float32 partials of SUM_GROUP consecutive terms and
+// fold each partial into the accumulator (a group's rounding is at most
+// (SUM_GROUP - 1) ulp of its terms' magnitudes; the compensated sum keeps the
+// long loops from growing it).
+constexpr int SUM_GROUP = 16;
+
 struct KahanC3 {
     float s[6];
     float c[6];

@@ -11,9 +11,10 @@
 // then multiplied componentwise by the sink weight and (for the 3-column index
 // of calculate_scattering with scattering > 0) projected onto the sink tangent
 // plane -- both linear and source-independent, so the source sum applies them
-// once per sink.  r and the phase are float64; the rest is float32 with Kahan
-// accumulation over sources; chunk partials merge in a fixed order, so results
-// are deterministic (the reference adds with float64 atomics in arbitrary order).
+// once per sink.  r and the phase are float64; the rest is float32, summed over
+// groups of 16 sources whose partials are Kahan-accumulated; chunk partials
+// merge in a fixed order, so results are deterministic (the reference adds
+// with float64 atomics in arbitrary order).
 #include "lem_device.h"
 #include "lem_field_dev.h"
 #include "lem_internal.h"
@@ -77,28 +78,34 @@ __global__ __launch_bounds__(FB) void k_field_los_partial(
         __syncthreads();
         int n = (int)((s1 - base) < FS_TILE ? (s1 - base) : FS_TILE);
         if (active) {
+            for (int j0 = 0; j0 < n; j0 += SUM_GROUP) {
+                const int j1 = (j0 + SUM_GROUP) < n ? (j0 + SUM_GROUP) : n;
+                float g[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 2
-            for (int j = 0; j < n; ++j) {
-                const int64_t s = base + j;
-                bool bl = false;
-                if (sink_major) {
-                    if ((s >> 6) != word_at) {
-                        word_at = s >> 6;
-                        word = hit_bits ? hit_bits[k * W64 + word_at] : 0ull;
+                for (int j = j0; j < j1; ++j) {
+                    const int64_t s = base + j;
+                    bool bl = false;
+                    if (sink_major) {
+                        if ((s >> 6) != word_at) {
+                            word_at = s >> 6;
+                            word = hit_bits ? hit_bits[k * W64 + word_at] : 0ull;
+                        }
+                        bl = (word >> (s & 63)) & 1ull;
+                    } else if (src_major_bits) {
+                        bl = (s_mask[j][wave] >> lane) & 1ull;
                     }
-                    bl = (word >> (s & 63)) & 1ull;
-                } else if (src_major_bits) {
-                    bl = (s_mask[j][wave] >> lane) & 1ull;
+                    if (bl)
+                        continue;
+                    const SrcLds &L = s_src[j];
+                    C3 w;
+                    w.xr = L.w[0]; w.xi = L.w[1]; w.yr = L.w[2]; w.yi = L.w[3]; w.zr = L.w[4];
+                    w.zi = L.w[5];
+                    C3 t = los_term<LOSSY>(L.x, L.y, L.z, w, xk, yk, zk, nkx, nky, nkz, alpha,
+                                           beta);
+                    g[0] += t.xr; g[1] += t.xi; g[2] += t.yr;
+                    g[3] += t.yi; g[4] += t.zr; g[5] += t.zi;
                 }
-                if (bl)
-                    continue;
-                const SrcLds &L = s_src[j];
-                C3 w;
-                w.xr = L.w[0]; w.xi = L.w[1]; w.yr = L.w[2]; w.yi = L.w[3]; w.zr = L.w[4];
-                w.zi = L.w[5];
-                C3 t = los_term<LOSSY>(L.x, L.y, L.z, w, xk, yk, zk, nkx, nky, nkz, alpha, beta);
-                float v[6] = {t.xr, t.xi, t.yr, t.yi, t.zr, t.zi};
-                acc.add(v);
+                acc.add(g);
             }
         }
     }

@@ -761,18 +761,25 @@ __global__ __launch_bounds__(FB) void k_field_los_grid_partial(
         __syncthreads();
         int n = (int)((s1 - base) < FS_TILE ? (s1 - base) : FS_TILE);
         if (active) {
-            for (int j = 0; j < n; ++j) {
-                const SrcLds &L = s_src[j];
-                Ray r = charge((float)L.x, (float)L.y, (float)L.z, xkf, ykf, zkf);
-                float dist = r.len;
-                if (dda_cast<true>(r, dist, G, cell_start, recs, box))
-                    continue;
-                C3 w;
-                w.xr = L.w[0]; w.xi = L.w[1]; w.yr = L.w[2]; w.yi = L.w[3]; w.zr = L.w[4];
-                w.zi = L.w[5];
-                C3 t = los_term<LOSSY>(L.x, L.y, L.z, w, xk, yk, zk, nkx, nky, nkz, alpha, beta);
-                float v[6] = {t.xr, t.xi, t.yr, t.yi, t.zr, t.zi};
-                acc.add(v);
+            // same groups of sources as k_field_los_partial: identical sums
+            for (int j0 = 0; j0 < n; j0 += SUM_GROUP) {
+                const int j1 = (j0 + SUM_GROUP) < n ? (j0 + SUM_GROUP) : n;
+                float g[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                for (int j = j0; j < j1; ++j) {
+                    const SrcLds &L = s_src[j];
+                    Ray r = charge((float)L.x, (float)L.y, (float)L.z, xkf, ykf, zkf);
+                    float dist = r.len;
+                    if (dda_cast<true>(r, dist, G, cell_start, recs, box))
+                        continue;
+                    C3 w;
+                    w.xr = L.w[0]; w.xi = L.w[1]; w.yr = L.w[2]; w.yi = L.w[3]; w.zr = L.w[4];
+                    w.zi = L.w[5];
+                    C3 t = los_term<LOSSY>(L.x, L.y, L.z, w, xk, yk, zk, nkx, nky, nkz, alpha,
+                                           beta);
+                    g[0] += t.xr; g[1] += t.xi; g[2] += t.yr;
+                    g[3] += t.yi; g[4] += t.zr; g[5] += t.zi;
+                }
+                acc.add(g);
             }
         }
     }
