@@ -11,8 +11,9 @@
 //                             + F cos ((d.w) M U + (U.d) Q)/r ]
 // with U = conj(gE_k), Q = P(d) w_s, M = I - d d^T and
 // F' = dF/dr = -(1/2pi) e^{-g r}/r (g^2 + 2g/r + 2/r^2); bilinear dots (no
-// conjugation).  Sinks are staged through LDS, lanes run over sources, chunk
-// partials are Kahan-summed and merged in a fixed order.  The sink-weight and
+// conjugation).  Sinks are staged through LDS, lanes run over sources; the
+// terms of every 16 sinks are summed in float32, those partials are
+// Kahan-accumulated, and chunk partials merge in a fixed order.  The sink-weight and
 // sink-tangent steps of the forward are applied to gE on entry (their
 // transposes), so the adjoint matches lem_field_los for every flag.
 #include "lem_device.h"
@@ -55,11 +56,6 @@ __global__ __launch_bounds__(AB) void k_adjoint_prep(const float *__restrict__ g
     o[2] = make_float2(g.zr, g.zi);
 }
 
-LEM_DEV float2 cmulf(float ar, float ai, float br, float bi)
-{
-    return make_float2(ar * br - ai * bi, ar * bi + ai * br);
-}
-
 // Partials: block b covers sources [(b % nbs)*256, +256), sink chunk c = b / nbs.
 template <bool LOSSY>
 __global__ __launch_bounds__(AB) void k_adjoint_partial(
@@ -85,6 +81,8 @@ __global__ __launch_bounds__(AB) void k_adjoint_partial(
     const int64_t k0 = c * CH;
     const int64_t k1 = (k0 + CH) < K ? (k0 + CH) : K;
     const float beta_f = (float)beta;
+    const float beta2 = beta_f * beta_f;
+    const float al = LOSSY ? alpha : 0.0f;  // lossless: the alpha terms fold away
     const float FRONT = -0.15915494309189535f;
     KahanC3 gw;
     gw.zero();
@@ -131,78 +129,67 @@ __global__ __launch_bounds__(AB) void k_adjoint_partial(
                 const float ir = p.inv_r;
                 const float ux = (float)dx * ir, uy = (float)dy * ir, uz = (float)dz * ir;
                 const float cs = ux * L.nx + uy * L.ny + uz * L.nz;
-                // G = e^{-alpha r}/r e^{-j beta r}
-                float amp = ir;
+                // FRONT G = FRONT e^{-alpha r}/r e^{-j beta r}
+                float amp = ir * FRONT;
                 if (LOSSY)
                     amp *= expf(-alpha * (float)p.r);
                 const float Gr = amp * p.c, Gi = amp * p.s;
                 // F = FRONT (-g - 1/r) G ; F' = FRONT (g^2 + 2g/r + 2/r^2) G
-                float2 F = cmulf(-alpha - ir, -beta_f, Gr, Gi);
-                F.x *= FRONT;
-                F.y *= FRONT;
-                const float b_r = alpha * alpha - beta_f * beta_f + 2.0f * alpha * ir + 2.0f * ir * ir;
-                const float b_i = 2.0f * alpha * beta_f + 2.0f * beta_f * ir;
-                float2 dF = cmulf(b_r, b_i, Gr, Gi);
-                dF.x *= FRONT;
-                dF.y *= FRONT;
-                // Q = P(d) w,  uw = d.w
-                const float uwr = ux * w.xr + uy * w.yr + uz * w.zr;
-                const float uwi = ux * w.xi + uy * w.yi + uz * w.zi;
-                const float Qxr = w.xr - ux * uwr, Qxi = w.xi - ux * uwi;
-                const float Qyr = w.yr - uy * uwr, Qyi = w.yi - uy * uwi;
-                const float Qzr = w.zr - uz * uwr, Qzi = w.zi - uz * uwi;
-                // gv = gE_k, U = conj(gv)
+                const float ar = -al - ir;
+                const float Fr = ar * Gr + beta_f * Gi, Fi = ar * Gi - beta_f * Gr;
+                const float br = (al * al - beta2) + 2.0f * ir * (al + ir);
+                const float bi = 2.0f * beta_f * (al + ir);
+                const float dFr = br * Gr - bi * Gi, dFi = br * Gi + bi * Gr;
+                // g = gE_k; bilinear dots (no conjugation) of d, w and U = conj(g)
                 const float gxr = L.g[0], gxi = L.g[1], gyr = L.g[2], gyi = L.g[3], gzr = L.g[4],
                             gzi = L.g[5];
-                // weight gradient: conj(F cos) (gv - d (d.gv))
+                const float uwr = ux * w.xr + uy * w.yr + uz * w.zr;
+                const float uwi = ux * w.xi + uy * w.yi + uz * w.zi;
+                const float ugr = ux * gxr + uy * gyr + uz * gzr;  // d.g; U.d = (ugr, -ugi)
+                const float ugi = ux * gxi + uy * gyi + uz * gzi;
+                const float Uwr = gxr * w.xr + gxi * w.xi + gyr * w.yr + gyi * w.yi +
+                                  gzr * w.zr + gzi * w.zi;
+                const float Uwi = gxr * w.xi - gxi * w.xr + gyr * w.yi - gyi * w.yr +
+                                  gzr * w.zi - gzi * w.zr;
+                // U.Q = U.w - (U.d)(d.w),  Q = P(d) w
+                const float UQr = Uwr - (ugr * uwr + ugi * uwi);
+                const float UQi = Uwi - (ugr * uwi - ugi * uwr);
+                // weight gradient: conj(F cos) P(d) g
                 {
-                    const float ugr = ux * gxr + uy * gyr + uz * gzr;
-                    const float ugi = ux * gxi + uy * gyi + uz * gzi;
+                    const float Lr = Fr * cs, Li = Fi * cs;
                     const float Pxr = gxr - ux * ugr, Pxi = gxi - ux * ugi;
                     const float Pyr = gyr - uy * ugr, Pyi = gyi - uy * ugi;
                     const float Pzr = gzr - uz * ugr, Pzi = gzi - uz * ugi;
-                    const float Lr = F.x * cs, Li = -F.y * cs;  // conj(F cos)
-                    float v[6];
-                    v[0] = Lr * Pxr - Li * Pxi; v[1] = Lr * Pxi + Li * Pxr;
-                    v[2] = Lr * Pyr - Li * Pyi; v[3] = Lr * Pyi + Li * Pyr;
-                    v[4] = Lr * Pzr - Li * Pzi; v[5] = Lr * Pzi + Li * Pzr;
-#pragma unroll
-                    for (int i = 0; i < 6; ++i)
-                        h[i] += v[i];
+                    h[0] = h[0] + Lr * Pxr; h[0] = h[0] + Li * Pxi;
+                    h[1] = h[1] + Lr * Pxi; h[1] = h[1] - Li * Pxr;
+                    h[2] = h[2] + Lr * Pyr; h[2] = h[2] + Li * Pyi;
+                    h[3] = h[3] + Lr * Pyi; h[3] = h[3] - Li * Pyr;
+                    h[4] = h[4] + Lr * Pzr; h[4] = h[4] + Li * Pzi;
+                    h[5] = h[5] + Lr * Pzi; h[5] = h[5] - Li * Pzr;
                 }
-                // position gradient
+                // position gradient: with M U = U - d (U.d) and Q = w - d (d.w),
+                //   (d.w) M U + (U.d) Q = (d.w) U + (U.d) w - 2 (d.w)(U.d) d
+                // so the sum is Re[coef d + a2 n + c1 U + c2 w] with
+                //   a1 = -F' cos (U.Q), a2 = -F (U.Q)/r, c1 = F cos (d.w)/r,
+                //   c2 = F cos (U.d)/r, coef = a1 - a2 cos - 2 c1 (U.d)
                 {
-                    // U.Q and U.d with U = conj(g)
-                    const float UQr = gxr * Qxr + gxi * Qxi + gyr * Qyr + gyi * Qyi + gzr * Qzr +
-                                      gzi * Qzi;
-                    const float UQi = gxr * Qxi - gxi * Qxr + gyr * Qyi - gyi * Qyr + gzr * Qzi -
-                                      gzi * Qzr;
-                    const float Udr = gxr * ux + gyr * uy + gzr * uz;
-                    const float Udi = -(gxi * ux + gyi * uy + gzi * uz);
-                    // M U = U - d (U.d)
-                    const float MUxr = gxr - ux * Udr, MUxi = -gxi - ux * Udi;
-                    const float MUyr = gyr - uy * Udr, MUyi = -gyi - uy * Udi;
-                    const float MUzr = gzr - uz * Udr, MUzi = -gzi - uz * Udi;
-                    // term1 coefficient on d: -F' cos (U.Q)
-                    const float2 t1 = cmulf(dF.x, dF.y, UQr, UQi);
-                    const float a1 = -t1.x * cs;
-                    // term2: -F (U.Q) (n - d cos)/r  (real part only)
-                    const float2 t2 = cmulf(F.x, F.y, UQr, UQi);
-                    const float a2 = -t2.x * ir;
-                    // term3: F cos ((d.w) M U + (U.d) Q)/r
-                    const float2 Fc = make_float2(F.x * cs * ir, F.y * cs * ir);
-                    const float2 c1 = cmulf(Fc.x, Fc.y, uwr, uwi);
-                    const float2 c2 = cmulf(Fc.x, Fc.y, Udr, Udi);
-                    float v[3];
-                    v[0] = a1 * ux + a2 * (L.nx - ux * cs) + (c1.x * MUxr - c1.y * MUxi) +
-                           (c2.x * Qxr - c2.y * Qxi);
-                    v[1] = a1 * uy + a2 * (L.ny - uy * cs) + (c1.x * MUyr - c1.y * MUyi) +
-                           (c2.x * Qyr - c2.y * Qyi);
-                    v[2] = a1 * uz + a2 * (L.nz - uz * cs) + (c1.x * MUzr - c1.y * MUzi) +
-                           (c2.x * Qzr - c2.y * Qzi);
-#pragma unroll
-                    for (int i = 0; i < 3; ++i)
-                        h[6 + i] += v[i];
+                    const float a1 = -(dFr * UQr - dFi * UQi) * cs;
+                    const float a2 = -(Fr * UQr - Fi * UQi) * ir;
+                    const float csr = cs * ir;
+                    const float Fcr = Fr * csr, Fci = Fi * csr;
+                    const float c1r = Fcr * uwr - Fci * uwi, c1i = Fcr * uwi + Fci * uwr;
+                    const float c2r = Fcr * ugr + Fci * ugi, c2i = Fci * ugr - Fcr * ugi;
+                    const float coef = (a1 - a2 * cs) - 2.0f * (c1r * ugr + c1i * ugi);
+                    // Re(c1 U_x) = c1r gxr + c1i gxi ; Re(c2 w_x) = c2r wxr - c2i wxi
+                    h[6] = h[6] + coef * ux; h[6] = h[6] + a2 * L.nx;
+                    h[6] = h[6] + c1r * gxr; h[6] = h[6] + c1i * gxi;
+                    h[6] = h[6] + c2r * w.xr; h[6] = h[6] - c2i * w.xi;
+                    h[7] = h[7] + coef * uy; h[7] = h[7] + a2 * L.ny;
+                    h[7] = h[7] + c1r * gyr; h[7] = h[7] + c1i * gyi;
+                    h[7] = h[7] + c2r * w.yr; h[7] = h[7] - c2i * w.yi;
+                    h[8] = h[8] + coef * uz; h[8] = h[8] + a2 * L.nz;
+                    h[8] = h[8] + c1r * gzr; h[8] = h[8] + c1i * gzi;
+                    h[8] = h[8] + c2r * w.zr; h[8] = h[8] - c2i * w.zi;
                 }
             }
             gw.add(h);

@@ -220,7 +220,8 @@ struct C3 {
 // float32 reciprocal of that square root.  The phase is carried in quarter
 // turns, t = -beta r (2/pi): adding and removing 1.5 * 2^52 rounds t to the
 // nearest integer q (exact for |t| < 2^51) and leaves q mod 4 in the low
-// mantissa bits, and the remainder x = t - q (|x| <= 1/2) is exact in float64.
+// mantissa bits; the remainder x = t - q (|x| <= 1/2 + 2^-52) is the fused
+// r (-beta 2/pi) - q, rounded once.
 // Sine / cosine of (pi/2) x come from their Taylor series to x^9 / x^10
 // (truncation error < 2e-9 on |x| <= pi/4), then the quadrant swap.
 struct Phase {
@@ -256,10 +257,11 @@ LEM_DEV Phase phase_of(double dx, double dy, double dz, double beta)
     p.inv_r = ir0;
     const double NEG_TWO_OVER_PI = -0.63661977236758134308;
     const double ROUND = 6755399441055744.0;  // 1.5 * 2^52
-    double t = r * (beta * NEG_TWO_OVER_PI);  // quarter turns
-    double m = t + ROUND;
+    const double b = beta * NEG_TWO_OVER_PI;  // quarter turns per unit length
+    // t = r b; q = rint(t) from the fused product, remainder t - q rounded once
+    double m = __builtin_fma(r, b, ROUND);
     double q = m - ROUND;
-    float x = (float)(t - q) * 1.57079632679489662f;
+    float x = (float)__builtin_fma(r, b, -q) * 1.57079632679489662f;
     float s, c;
     sincos_reduced(x, s, c);
     // q mod 4 (two's complement of q in the low word of m)
@@ -283,17 +285,16 @@ LEM_DEV void leg_factor(const Phase &p, float alpha, float beta_f, float cosang,
                         float &lr, float &li)
 {
 #pragma clang fp contract(fast)
-    float amp = p.inv_r;
+    const float FRONT = -0.15915494309189535f;  // -(1/(2 pi))
+    // FRONT cos e^{-alpha r}/r e^{-j beta r}
+    float amp = p.inv_r * (FRONT * cosang);
     if (LOSSY)
         amp *= expf(-alpha * (float)p.r);
-    float Gr = amp * p.c, Gi = amp * p.s;
-    float ar = -alpha - p.inv_r, ai = -beta_f;
-    float dGr = ar * Gr - ai * Gi;
-    float dGi = ar * Gi + ai * Gr;
-    const float FRONT = -0.15915494309189535f;  // -(1/(2 pi))
-    float f = FRONT * cosang;
-    lr = dGr * f;
-    li = dGi * f;
+    const float Gr = amp * p.c, Gi = amp * p.s;
+    // times (-alpha - 1/r) - j beta
+    const float ar = LOSSY ? -alpha - p.inv_r : -p.inv_r;
+    lr = ar * Gr + beta_f * Gi;
+    li = ar * Gi - beta_f * Gr;
 }
 
 // (I - d d^T) w for a complex 3-vector: the closed form of the u/v launch

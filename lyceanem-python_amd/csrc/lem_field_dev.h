@@ -52,20 +52,43 @@ LEM_DEV C3 project_normal(const C3 &e, float nx, float ny, float nz)
     return project(e, nx * inv, ny * inv, nz * inv);
 }
 
-// Contribution of one source->sink leg before the sink weight / tangent step.
+// One source->sink leg before the sink weight / tangent step: the projected
+// source weight e = (I - d d^T) w_s and the scalar leg factor (lr, li); the
+// contribution is (lr + j li) e.
 template <bool LOSSY = true>
-LEM_DEV C3 los_term(double xs, double ys, double zs, const C3 &ws, double xk, double yk,
-                    double zk, float nkx, float nky, float nkz, float alpha, double beta)
+LEM_DEV void los_leg(double xs, double ys, double zs, const C3 &ws, double xk, double yk,
+                     double zk, float nkx, float nky, float nkz, float alpha, double beta, C3 &e,
+                     float &lr, float &li)
 {
 #pragma clang fp contract(fast)
     double dx = xk - xs, dy = yk - ys, dz = zk - zs;
     Phase p = phase_of(dx, dy, dz, beta);
     float ux = (float)dx * p.inv_r, uy = (float)dy * p.inv_r, uz = (float)dz * p.inv_r;
     float cosang = ux * nkx + uy * nky + uz * nkz;
-    float lr, li;
     leg_factor<LOSSY>(p, alpha, (float)beta, cosang, lr, li);
-    C3 e = project(ws, ux, uy, uz);
+    e = project(ws, ux, uy, uz);
+}
+
+template <bool LOSSY = true>
+LEM_DEV C3 los_term(double xs, double ys, double zs, const C3 &ws, double xk, double yk,
+                    double zk, float nkx, float nky, float nkz, float alpha, double beta)
+{
+    C3 e;
+    float lr, li;
+    los_leg<LOSSY>(xs, ys, zs, ws, xk, yk, zk, nkx, nky, nkz, alpha, beta, e, lr, li);
     return scale(e, lr, li);
+}
+
+// g += (lr + j li) e, each product fused into the running partial (the same
+// operations in every kernel that sums legs, so their sums agree bit for bit).
+LEM_DEV void add_leg(float g[6], const C3 &e, float lr, float li)
+{
+    g[0] = __builtin_fmaf(e.xr, lr, g[0]); g[0] = __builtin_fmaf(-e.xi, li, g[0]);
+    g[1] = __builtin_fmaf(e.xr, li, g[1]); g[1] = __builtin_fmaf(e.xi, lr, g[1]);
+    g[2] = __builtin_fmaf(e.yr, lr, g[2]); g[2] = __builtin_fmaf(-e.yi, li, g[2]);
+    g[3] = __builtin_fmaf(e.yr, li, g[3]); g[3] = __builtin_fmaf(e.yi, lr, g[3]);
+    g[4] = __builtin_fmaf(e.zr, lr, g[4]); g[4] = __builtin_fmaf(-e.zi, li, g[4]);
+    g[5] = __builtin_fmaf(e.zr, li, g[5]); g[5] = __builtin_fmaf(e.zi, lr, g[5]);
 }
 
 LEM_DEV bool blocked(const unsigned long long *__restrict__ hit_bits, int64_t W64, int64_t s,

@@ -774,10 +774,11 @@ __global__ __launch_bounds__(FB) void k_field_los_grid_partial(
                     C3 w;
                     w.xr = L.w[0]; w.xi = L.w[1]; w.yr = L.w[2]; w.yi = L.w[3]; w.zr = L.w[4];
                     w.zi = L.w[5];
-                    C3 t = los_term<LOSSY>(L.x, L.y, L.z, w, xk, yk, zk, nkx, nky, nkz, alpha,
-                                           beta);
-                    g[0] += t.xr; g[1] += t.xi; g[2] += t.yr;
-                    g[3] += t.yi; g[4] += t.zr; g[5] += t.zi;
+                    C3 e;
+                    float lr, li;
+                    los_leg<LOSSY>(L.x, L.y, L.z, w, xk, yk, zk, nkx, nky, nkz, alpha, beta, e,
+                                   lr, li);
+                    add_leg(g, e, lr, li);
                 }
                 acc.add(g);
             }
