@@ -13,9 +13,9 @@
 // F' = dF/dr = -(1/2pi) e^{-g r}/r (g^2 + 2g/r + 2/r^2); bilinear dots (no
 // conjugation).  Sinks are staged through LDS, lanes run over sources; the
 // terms of every 16 sinks are summed in float32, those partials are
-// Kahan-accumulated, and chunk partials merge in a fixed order.  The sink-weight and
-// sink-tangent steps of the forward are applied to gE on entry (their
-// transposes), so the adjoint matches lem_field_los for every flag.
+// Kahan-accumulated, and chunk partials merge in a fixed order.  The
+// sink-weight and sink-tangent steps of the forward are applied to gE on entry
+// (their transposes), so the adjoint matches lem_field_los for every flag.
 #include "lem_device.h"
 #include "lem_field_dev.h"
 #include "lem_internal.h"
@@ -243,9 +243,14 @@ __global__ __launch_bounds__(AB) void k_adjoint_finish(const float *__restrict__
 
 // Sink chunks: enough (source, chunk) threads to fill the chip, chunk length a
 // multiple of the LDS tile (keeps 64-bit mask words aligned).
+// (source, chunk) threads per launch: 2^LEM_ADJ_THREADS_LOG2
+#ifndef LEM_ADJ_THREADS_LOG2
+#define LEM_ADJ_THREADS_LOG2 19
+#endif
+
 inline int64_t adj_chunks(int64_t S, int64_t K)
 {
-    int64_t want = ceil_div(1 << 19, S > 0 ? S : 1);
+    int64_t want = ceil_div(1ll << LEM_ADJ_THREADS_LOG2, S > 0 ? S : 1);
     int64_t cap = ceil_div(K, AK_TILE);
     int64_t nc = want < cap ? want : cap;
     return nc < 1 ? 1 : nc;

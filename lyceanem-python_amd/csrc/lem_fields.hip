@@ -271,9 +271,14 @@ __global__ void k_fill_zero(float *__restrict__ p, int64_t n)
 // >= 64 sources per thread.  The mask lets whole waves skip sources, so the
 // work per workgroup is uneven and small workgroups balance it (the 64x64
 // conformal array forward: 0.66 ms with 2^19 threads, 0.575 ms with 2^21).
+// (sink, chunk) threads per launch: 2^LEM_LOS_THREADS_LOG2
+#ifndef LEM_LOS_THREADS_LOG2
+#define LEM_LOS_THREADS_LOG2 21
+#endif
+
 inline int64_t los_chunks(int64_t S, int64_t K)
 {
-    int64_t want = ceil_div(1ll << 21, K > 0 ? K : 1);
+    int64_t want = ceil_div(1ll << LEM_LOS_THREADS_LOG2, K > 0 ? K : 1);
     int64_t cap = ceil_div(S, 64);
     int64_t nc = want < cap ? want : cap;
     return nc < 1 ? 1 : nc;

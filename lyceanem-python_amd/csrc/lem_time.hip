@@ -293,9 +293,11 @@ __global__ __launch_bounds__(TB) void k_trace_keys(const int64_t *__restrict__ b
     const int64_t p = (int64_t)blockIdx.x * TB + threadIdx.x;
     if (p >= V)
         return;
+    // keys 0 .. N - 1 (a 2^16-sample trace sorts on 16 bits); a path that
+    // starts at or past the trace's end keeps key N - 1 and is marked -1
     const int64_t b = bin[p];
-    key[p] = (int32_t)(b < 0 ? 0 : (b > N ? N : b));
-    val[p] = (int32_t)p;
+    key[p] = (int32_t)(b < 0 ? 0 : (b >= N ? N - 1 : b));
+    val[p] = b >= N ? -1 : (int32_t)p;
 }
 
 // start[b] = first sorted position with key >= b, for b in [0, N]
@@ -333,6 +335,8 @@ __global__ __launch_bounds__(TB) void k_trace_hist(const double *__restrict__ am
     const int32_t e = start[b + 1];
     for (int32_t i = start[b] + lane; i < e; i += 64) {
         const int64_t p = val[i];
+        if (p < 0)
+            continue;  // past the trace's end
         h0 += amp[3 * p];
         h1 += amp[3 * p + 1];
         h2 += amp[3 * p + 2];
@@ -422,7 +426,7 @@ inline int64_t cub_sort_bytes(int64_t V, int64_t N)
     if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const int32_t *)nullptr,
                                            (int32_t *)nullptr, (const int32_t *)nullptr,
                                            (int32_t *)nullptr, (int)V, 0,
-                                           key_bits(N)) != hipSuccess)
+                                           key_bits(N - 1)) != hipSuccess)
         return -1;
     return (int64_t)bytes;
 }
@@ -545,7 +549,7 @@ extern "C" int lem_time_trace(const double *amp, const int64_t *bin, int64_t V,
             return rc;
         size_t tb = L.cub_bytes;
         if (hipcub::DeviceRadixSort::SortPairs(L.cub, tb, L.k0, L.k1, L.v0, L.v1, (int)V, 0,
-                                               key_bits(N), st) != hipSuccess) {
+                                               key_bits(N - 1), st) != hipSuccess) {
             set_error("lem_time_trace: radix sort failed");
             return E_HIP;
         }
