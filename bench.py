@@ -642,7 +642,11 @@ class C4:
         torch = self.torch
         graph_ms = float(np.mean([a for a, _ in times_ms]))
         # forward / loss + adjoint split from eager steps timed with events
-        evs = [_events(torch, 3) for _ in range(10)]
+        # (after two untimed eager steps, which set up the allocator's blocks)
+        for _ in range(2):
+            self._eager(self.w.clone().requires_grad_(True),
+                        self.pos.clone().requires_grad_(True))
+        evs = [_events(torch, 3) for _ in range(20)]
         for e in evs:
             self._eager(self.w.clone().requires_grad_(True), self.pos.clone().requires_grad_(True),
                         e)

@@ -210,6 +210,30 @@ def test_field_long_paths_phase(native, oracle, extent):
     assert np.abs(E[big] - ref[big]).max() / np.abs(ref[big]).max() < FIELD_RTOL
 
 
+@pytest.mark.parametrize("r_lo,r_hi", [(1e5, 1e6), (3e6, 3e7)])
+def test_field_phase_very_long_paths(native, oracle, r_lo, r_hi):
+    """Paths of 100 km - 30,000 km at 10 GHz (beta r up to 6e9 rad, 4e9 quarter
+    turns): the quarter-turn rounding and quadrant of the phase reduction hold
+    far beyond 2^31 quarter turns.  Per element against the float64 oracle, on
+    the same float32 positions."""
+    rng = np.random.default_rng(int(r_lo))
+    beta = 2 * np.pi / 0.03
+    src = rng.uniform(-0.2, 0.2, (32, 3)).astype(np.float32)
+    u = rng.normal(size=(200, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    snk = (u * rng.uniform(r_lo, r_hi, (200, 1))).astype(np.float32)
+    w = (rng.normal(size=(32, 3)) + 1j * rng.normal(size=(32, 3))).astype(np.complex64)
+    E = native.field_los(src, w, snk, u.astype(np.float32), beta=beta,
+                         elements=True).cpu().numpy()
+    ref = oracle.field_los(src, w, snk, u.astype(np.float32), 0.0, beta, elements=True)
+    big = np.abs(ref) > 1e-3 * np.abs(ref).max()
+    dph = np.abs(np.angle(E[big] / ref[big]))
+    rel = np.abs(E[big] - ref[big]) / np.abs(ref[big])
+    print(f"paths {r_lo:.0e}-{r_hi:.0e} m: max phase err {dph.max():.2e} rad, max rel "
+          f"{rel.max():.2e}", flush=True)
+    assert dph.max() < 1e-4 and rel.max() < FIELD_RTOL
+
+
 def test_field_los_mixed_source_extents(native, oracle):
     """One launch over compact and spread source chunks: 2048 sources in a
     0.2 m cube followed by 2048 spread over 40 m, sinks from 30 m to 10 km at
