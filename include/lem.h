@@ -251,6 +251,21 @@ int lem_field_los_adjoint(const float *src, const float *src_w, int64_t S, const
                           const float *grad_out, float *grad_w, float *grad_p, void *workspace,
                           int64_t workspace_bytes, void *stream);
 
+/* dB mask loss of a source-summed far field and its gradient, in one pass:
+ * P_k = sum_c |E_kc|^2, R = ref_power if > 0 else max_k P_k (a constant of the
+ * loss), db_k = 10 log10(P_k / R + eps), err_k = db_k for in_beam[k] != 0 and
+ * max(db_k - sidelobe_db, 0) otherwise; *loss = mean_k err_k^2 (one float on
+ * the device).  grad_E (K x 3 complex64, or NULL) receives dloss/dE in
+ * PyTorch's convention (dL/dRe E + j dL/dIm E).  E: K x 3 complex64; in_beam:
+ * K bytes.  workspace: lem_db_mask_loss_workspace(K) bytes.  Deterministic,
+ * no host synchronisation and no memset (capturable in a HIP graph).
+ * No counterpart in the reference (it has no gradients): the beam-synthesis
+ * objective for lem_field_los / lem_field_los_adjoint. */
+int64_t lem_db_mask_loss_workspace(int64_t K);
+int lem_db_mask_loss(const float *E, const uint8_t *in_beam, int64_t K, double sidelobe_db,
+                     double ref_power, double eps, float *loss, float *grad_E, void *workspace,
+                     int64_t workspace_bytes, void *stream);
+
 /* Batched-excitation contraction on FP32 MFMA: for F wavenumbers k_f and B = 64
  * excitation sets W at once,
  *   E[f,c,k,b] = sum_s G_c[k,s,f] W[f,s,b],

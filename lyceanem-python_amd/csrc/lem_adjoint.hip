@@ -216,28 +216,28 @@ __global__ __launch_bounds__(AB) void k_adjoint_partial(
     }
 }
 
+// Fixed-order merge of the chunk partials: one thread per (source, gradient
+// component) -- 9 S threads, each summing its component over the chunks in
+// chunk order.
 __global__ __launch_bounds__(AB) void k_adjoint_finish(const float *__restrict__ ws_sum,
                                                        const float *__restrict__ ws_cmp,
                                                        int64_t NC, int64_t S,
                                                        float *__restrict__ grad_w,
                                                        float *__restrict__ grad_p)
 {
-    const int64_t s = (int64_t)blockIdx.x * AB + threadIdx.x;
-    if (s >= S)
+    const int64_t t = (int64_t)blockIdx.x * AB + threadIdx.x;
+    if (t >= 9 * S)
         return;
-    double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t c = 0; c < NC; ++c) {
-#pragma unroll
-        for (int i = 0; i < 9; ++i)
-            a[i] += (double)ws_sum[(c * 9 + i) * S + s] - (double)ws_cmp[(c * 9 + i) * S + s];
-    }
-    if (grad_w) {
-        for (int i = 0; i < 6; ++i)
-            grad_w[6 * s + i] = (float)a[i];
-    }
-    if (grad_p) {
-        for (int i = 0; i < 3; ++i)
-            grad_p[3 * s + i] = (float)a[6 + i];
+    const int i = (int)(t / S);
+    const int64_t s = t - i * S;
+    double a = 0.0;
+    for (int64_t c = 0; c < NC; ++c)
+        a += (double)ws_sum[(c * 9 + i) * S + s] - (double)ws_cmp[(c * 9 + i) * S + s];
+    if (i < 6) {
+        if (grad_w)
+            grad_w[6 * s + i] = (float)a;
+    } else if (grad_p) {
+        grad_p[3 * s + (i - 6)] = (float)a;
     }
 }
 
@@ -325,7 +325,7 @@ extern "C" int lem_field_los_adjoint(const float *src, const float *src_w, int64
     int rc = check_launch("k_adjoint_partial");
     if (rc)
         return rc;
-    hipLaunchKernelGGL(k_adjoint_finish, dim3((unsigned)nbs), dim3(AB), 0, st, ws_sum, ws_cmp, NCe,
-                       S, grad_w, grad_p);
+    hipLaunchKernelGGL(k_adjoint_finish, dim3((unsigned)ceil_div(9 * S, AB)), dim3(AB), 0, st,
+                       ws_sum, ws_cmp, NCe, S, grad_w, grad_p);
     return check_launch("k_adjoint_finish");
 }

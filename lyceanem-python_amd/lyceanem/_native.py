@@ -68,6 +68,8 @@ SIGNATURES = {
     "lem_field_los_noalloc_max_sinks": ([], _I64),
     "lem_field_los_noalloc": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _D, _D, _INT, _P, _P], _INT),
     "lem_field_los_adjoint_workspace": ([_I64, _I64], _I64),
+    "lem_db_mask_loss_workspace": ([_I64], _I64),
+    "lem_db_mask_loss": ([_P, _P, _I64, _D, _D, _D, _P, _P, _P, _I64, _P], _INT),
     "lem_field_los_adjoint": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _D, _D, _INT, _P, _P, _P, _P,
                                _I64, _P], _INT),
 }
@@ -279,6 +281,28 @@ def field_los_adjoint(src, src_w, snk, snk_n, grad_out, hit_bits=None, snk_w=Non
                                     ptr(gw), ptr(gp), ptr(ws), wsb, stream_ptr()),
           "lem_field_los_adjoint")
     return gw, gp
+
+
+def db_mask_loss(E, in_beam, sidelobe_db=-25.0, ref_power=0.0, eps=1e-30, want_grad=True):
+    """lem_db_mask_loss: (loss (0-dim float32), dloss/dE (K,3) complex64 or
+    None).  ref_power <= 0: the current largest sink power."""
+    torch = _torch()
+    lib = load_library()
+    E = to_dev(E, torch.complex64).reshape(-1, 3)
+    K = E.shape[0]
+    beam = to_dev(in_beam, torch.bool).reshape(-1)
+    if beam.shape[0] != K:
+        raise LemError("db_mask_loss: one in_beam flag per sink")
+    loss = torch.empty((), dtype=torch.float32, device=E.device)
+    grad = torch.empty((K, 3), dtype=torch.complex64, device=E.device) if want_grad else None
+    wsb = int(lib.lem_db_mask_loss_workspace(K))
+    if wsb < 0:
+        raise LemError("lem_db_mask_loss_workspace: size query failed")
+    ws = torch.empty(wsb, dtype=torch.uint8, device=E.device)
+    check(lib.lem_db_mask_loss(ptr(E), ptr(beam), K, float(sidelobe_db), float(ref_power),
+                               float(eps), ptr(loss), ptr(grad), ptr(ws), wsb, stream_ptr()),
+          "lem_db_mask_loss")
+    return loss, grad
 
 
 def _eta(eta, N):

@@ -108,12 +108,27 @@ def far_field(w, pos, sinks, sink_normals, hit_bits=None, alpha=0.0, beta=2 * np
     return FarField.apply(w, pos, sinks, sink_normals, hit_bits, float(alpha), float(beta), plan)
 
 
+class DbMaskLoss(torch.autograd.Function):
+    """lem_db_mask_loss: the loss and dloss/dE in one native pass; the
+    gradient is kept from the forward and scaled by the incoming one."""
+
+    @staticmethod
+    def forward(ctx, E, in_beam, sidelobe_db, ref_power, eps):
+        loss, grad = _native.db_mask_loss(E.detach(), in_beam, sidelobe_db, ref_power, eps,
+                                          want_grad=ctx.needs_input_grad[0])
+        ctx.save_for_backward(grad if grad is not None else loss)
+        return loss
+
+    @staticmethod
+    def backward(ctx, grad_loss):
+        (grad,) = ctx.saved_tensors
+        return grad * grad_loss, None, None, None, None
+
+
 def db_mask_loss(E, in_beam, sidelobe_db=-25.0, ref_power=None, eps=1e-30):
     """Squared dB mismatch to a mask: 0 dB inside the main beam, at most
-    `sidelobe_db` outside (excess only), relative to `ref_power` (default: the
-    current peak power, treated as a constant).  Mean over sinks."""
-    P = (E.real ** 2 + E.imag ** 2).sum(dim=-1)
-    peak = P.detach().max() if ref_power is None else ref_power
-    db = 10.0 * torch.log10(P / peak + eps)
-    err = torch.where(in_beam, db, torch.clamp(db - sidelobe_db, min=0.0))
-    return (err ** 2).mean()
+    `sidelobe_db` outside (excess only), relative to `ref_power` (a number;
+    default: the current peak power, treated as a constant).  Mean over sinks.
+    One native pass (lem_db_mask_loss) computes the loss and its gradient."""
+    return DbMaskLoss.apply(E, in_beam, float(sidelobe_db),
+                            0.0 if ref_power is None else float(ref_power), float(eps))

@@ -22,13 +22,54 @@ def _mask(native, sc):
     return bits, ~native.unpack_bits(bits, len(sc["snk"]))
 
 
-def _loss64(oracle, w, pos, sc, visible, in_beam, ref_power=None):
-    from lyceanem.differentiable import db_mask_loss
+def _db_mask_loss_torch(E, in_beam, sidelobe_db=-25.0, ref_power=None, eps=1e-30):
+    """The dB mask loss as a composition of torch operations (any dtype and
+    device, differentiated by autograd): the reference for lem_db_mask_loss."""
+    P = (E.real ** 2 + E.imag ** 2).sum(dim=-1)
+    peak = P.detach().max() if ref_power is None else ref_power
+    db = 10.0 * torch.log10(P / peak + eps)
+    err = torch.where(in_beam, db, torch.clamp(db - sidelobe_db, min=0.0))
+    return (err ** 2).mean()
 
+
+def _loss64(oracle, w, pos, sc, visible, in_beam, ref_power=None):
     E = oracle.field_los_torch64(w, pos, torch.from_numpy(sc["snk"].astype(np.float64)),
                                  torch.from_numpy(sc["snk_n"].astype(np.float64)),
                                  torch.from_numpy(visible), sc["alpha"], sc["beta"])
-    return db_mask_loss(E, torch.from_numpy(in_beam), ref_power=ref_power)
+    return _db_mask_loss_torch(E, torch.from_numpy(in_beam), ref_power=ref_power)
+
+
+@pytest.mark.parametrize("ref_power", [None, 3.0])
+def test_db_mask_loss_matches_torch_reference(native, ref_power):
+    """lem_db_mask_loss (loss and dloss/dE in one pass) against autograd of the
+    torch composition in float64 on the same complex64 field: a beam of 300
+    sinks, side lobes on both sides of the -25 dB bar, a zero-power sink."""
+    from lyceanem.differentiable import db_mask_loss
+
+    rng = np.random.default_rng(31)
+    K = 5000
+    amp = 10.0 ** (rng.uniform(-4.5, 0.0, K) / 2)
+    E0 = (amp[:, None] * (rng.normal(size=(K, 3)) + 1j * rng.normal(size=(K, 3)))).astype(
+        np.complex64)
+    E0[17] = 0
+    in_beam = np.zeros(K, bool)
+    in_beam[:300] = True
+    dev = native.device()
+    E = torch.from_numpy(E0).to(dev).requires_grad_(True)
+    loss = db_mask_loss(E, torch.from_numpy(in_beam).to(dev), ref_power=ref_power)
+    loss.backward()
+    E64 = torch.from_numpy(E0.astype(np.complex128)).requires_grad_(True)
+    l64 = _db_mask_loss_torch(E64, torch.from_numpy(in_beam), ref_power=ref_power)
+    l64.backward()
+    g, g64 = E.grad.cpu().numpy(), E64.grad.numpy()
+    rl = abs(loss.item() - l64.item()) / abs(l64.item())
+    rg = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+    live = np.abs(g64).sum(-1) > 0
+    print(f"dB mask loss: loss {rl:.2e}, gradient {rg:.2e} normwise, {int(live.sum())} of {K} "
+          f"sinks with a gradient", flush=True)
+    assert 0.1 < live.mean() < 0.9
+    assert (g[~live] == 0).all()
+    assert rl < 1e-5 and rg < 1e-5
 
 
 @pytest.mark.parametrize("radius", [0.5, 3.0])
