@@ -346,7 +346,7 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_par(
     const float4 *__restrict__ box, const uint2 *__restrict__ mq, int nmacro,
     const int32_t *__restrict__ order, unsigned long long *__restrict__ bits,
     float *__restrict__ dist_out, int skip_self, const unsigned long long *__restrict__ par_count,
-    const uint2 *__restrict__ par_queue, int64_t par_cap)
+    const uint2 *__restrict__ par_queue, int64_t par_cap, int perm)
 {
     const unsigned long long n = *par_count;
     if (n == 0)
@@ -394,8 +394,10 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_par(
         float &dd = REV ? dw : dist;
         const bool hit = dda_cast<ANY, SEM, true, true>(rw, dd, G, cell_start, recs, box, nullptr,
                                                         nullptr, nullptr, smq, REV ? &r : nullptr);
+        // perm: the rows are in lane-slot order until k_unpermute_rows
+        const int64_t at = perm ? slot : inner;
         if (hit)
-            atomicOr(&bits[outer * W + (inner >> 6)], 1ull << (inner & 63));
+            atomicOr(&bits[outer * W + (at >> 6)], 1ull << (at & 63));
         if (dist_out)
             dist_out[outer * NI + inner] = dist;
     }
@@ -424,7 +426,8 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
     const float4 *__restrict__ box, const uint2 *__restrict__ mq, int nmacro,
     const int32_t *__restrict__ order, unsigned long long *__restrict__ bits,
     float *__restrict__ dist_out, unsigned long long *__restrict__ next_task, int skip_self,
-    unsigned long long *__restrict__ par_count, uint2 *__restrict__ par_queue, int64_t par_cap)
+    unsigned long long *__restrict__ par_count, uint2 *__restrict__ par_queue, int64_t par_cap,
+    int perm)
 {
     __shared__ uint32_t smqa[MACRO_LDS_MAX];
     __shared__ uint16_t smqb[MACRO_LDS_MAX];
@@ -504,11 +507,13 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
                                                           nullptr, nullptr, nullptr, smq, rt);
                 }
             }
-            if (order) {
+            if (order && !perm) {
                 // the lanes' bits belong to several words (zeroed beforehand)
                 if (active && hit)
                     atomicOr(&bits[outer * W + (inner >> 6)], 1ull << (inner & 63));
             } else {
+                // one word per wave: in input order, or (perm) in lane-slot
+                // order, rewritten by k_unpermute_rows afterwards
                 const unsigned long long bw = __ballot(active && hit);
                 if (lane == 0)
                     bits[outer * W + word] = bw;
@@ -584,6 +589,49 @@ __global__ __launch_bounds__(GB) void k_morton_keys(const float *__restrict__ p,
 // Workspace of lem_visibility_grid: the task counter and point bounds, then
 // (unless the input order is kept) keys, values and order of the lane-side
 // points and the radix sort's temporary storage.
+// inv[order[slot]] = slot
+__global__ __launch_bounds__(GB) void k_invert_order(const int32_t *__restrict__ order,
+                                                     int64_t n, int32_t *__restrict__ inv)
+{
+    const int64_t slot = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (slot < n)
+        inv[order[slot]] = (int32_t)slot;
+}
+
+// Rows written in lane-slot order (bit j of word w = the ray of slot 64 w + j)
+// rewritten in place in the order of the lane-side points: a workgroup copies
+// one row (W words) into LDS, then each wave builds output words by ballot,
+// bit i = slot inv[i] of the copy.  Replaces one device-scope atomic OR per
+// blocked ray with one plain store per 64 rays.
+__global__ __launch_bounds__(GB) void k_unpermute_rows(unsigned long long *__restrict__ bits,
+                                                       int64_t NO, int64_t W, int64_t NI,
+                                                       const int32_t *__restrict__ inv)
+{
+    extern __shared__ unsigned long long srow[];  // W words
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int64_t row = blockIdx.x; row < NO; row += gridDim.x) {
+        unsigned long long *b = bits + row * W;
+        __syncthreads();  // the previous row's reads of srow are done
+        for (int64_t w = threadIdx.x; w < W; w += GB)
+            srow[w] = b[w];
+        __syncthreads();
+        for (int64_t w = wave; w < W; w += GB / 64) {
+            const int64_t i = w * 64 + lane;
+            bool bit = false;
+            if (i < NI) {
+                const int32_t sl = inv[i];
+                bit = (srow[sl >> 6] >> (sl & 63)) & 1ull;
+            }
+            const unsigned long long bw = __ballot(bit);
+            if (lane == 0)
+                b[w] = bw;
+        }
+    }
+}
+
+// largest row (in words) k_unpermute_rows stages in LDS
+constexpr int64_t UNPERM_MAX_WORDS = 8192;  // 64 KB
+
 struct VisWs {
     unsigned long long *counter;
     unsigned int *bounds;
@@ -1005,15 +1053,29 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
     const int64_t cub = (flags & LEM_VIS_INPUT_ORDER) ? 0 : vis_sort_bytes(NI);
     vis_layout(NI, (size_t)std::max<int64_t>(cub, 0), reinterpret_cast<char *>(workspace), &L);
     const int32_t *order = nullptr;
+    // persistent walk over a spatial order with rows that fit in LDS: each wave
+    // stores its 64 bits in lane-slot order and the rows are reordered after
+    // the walk (no per-ray atomics, no clearing); otherwise bits are ORed per
+    // lane into cleared rows
+    const int64_t Wn = ceil_div(NI, 64);
+    const bool perm = !(flags & LEM_VIS_INPUT_ORDER) && info->nmacro <= MACRO_LDS_MAX &&
+                      Wn <= UNPERM_MAX_WORDS;
     if (!(flags & LEM_VIS_INPUT_ORDER)) {
-        // spatial order of the lane-side points; bits are then set per lane
+        // spatial order of the lane-side points
         int rc = morton_order(sink_major ? src : snk, NI, L, st);
         if (rc)
             return rc;
-        const int64_t nw = NO * ceil_div(NI, 64);
-        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<int64_t>(ceil_div(nw, GB), 65536)),
-                           dim3(GB), 0, st, hb, nw);
-        rc = check_launch("k_zero_words");
+        if (perm) {
+            // the sort's value input (L.v0) is free again: the inverse order
+            hipLaunchKernelGGL(k_invert_order, dim3((unsigned)ceil_div(NI, GB)), dim3(GB), 0, st,
+                               L.order, NI, L.v0);
+        } else {
+            const int64_t nw = NO * Wn;
+            hipLaunchKernelGGL(k_zero_words,
+                               dim3((unsigned)std::min<int64_t>(ceil_div(nw, GB), 65536)), dim3(GB),
+                               0, st, hb, nw);
+        }
+        rc = check_launch("k_zero_words / k_invert_order");
         if (rc)
             return rc;
         order = L.order;
@@ -1046,10 +1108,11 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
     do {                                                                                      \
         hipLaunchKernelGGL((k_visibility_grid_lds<A, M, SM, R>), dim3((unsigned)nwg), dim3(GB), 0, \
                            st, src, S, snk, K, G, cell_start, rc4, bx4, mq, info->nmacro, order, \
-                           hb, dist, L.counter, skip_self, pcount, pq, PAR_QUEUE_CAP);         \
+                           hb, dist, L.counter, skip_self, pcount, pq, PAR_QUEUE_CAP,          \
+                           perm ? 1 : 0);                                                      \
         hipLaunchKernelGGL((k_visibility_grid_par<A, M, SM, R>), dim3((unsigned)nwg), dim3(GB), 0, \
                            st, src, S, snk, K, G, cell_start, rc4, bx4, mq, info->nmacro, order, \
-                           hb, dist, skip_self, pcount, pq, PAR_QUEUE_CAP);                    \
+                           hb, dist, skip_self, pcount, pq, PAR_QUEUE_CAP, perm ? 1 : 0);      \
     } while (0)
 #define LEM_VGL2(A, SM, R)                                                                    \
     do {                                                                                      \
@@ -1073,7 +1136,13 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
         }
 #undef LEM_VGL2
 #undef LEM_VGL
-        return check_launch("k_visibility_grid_lds / k_visibility_grid_par");
+        int rc = check_launch("k_visibility_grid_lds / k_visibility_grid_par");
+        if (rc || !perm)
+            return rc;
+        const int64_t nub = std::max<int64_t>(1, std::min<int64_t>(NO, (int64_t)cus * 8));
+        hipLaunchKernelGGL(k_unpermute_rows, dim3((unsigned)nub), dim3(GB),
+                           (unsigned)(Wn * sizeof(unsigned long long)), st, hb, NO, Wn, NI, L.v0);
+        return check_launch("k_unpermute_rows");
     }
     LEM_REQUIRE(NO * nbl <= 2147483647LL, "lem_visibility_grid: S*K too large for one launch");
     dim3 grid((unsigned)(NO * nbl));
