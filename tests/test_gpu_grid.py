@@ -106,6 +106,27 @@ def test_grid_from_sink_matches_brute_force(native):
             assert np.array_equal(mb, m), f"sem={sem} sm={sm}: {(mb != m).sum()} mismatches"
 
 
+@pytest.mark.parametrize("S", [524288, 524289])
+def test_grid_mask_rows_at_the_lds_limit(native, S):
+    """Sink-major rows of 8,192 words (64 KB: lane-order words, reordered in
+    LDS) and of 8,193 words (per-ray atomic ORs into cleared rows), any-hit
+    and closest-hit, against the brute-force kernel; two sinks, one of them
+    also a source (row boundaries off the word grid)."""
+    src, snk, tris = _small_tri_scene(12, n_tris=500, S=S, K=2)
+    snk[1] = src[S // 3]
+    g = native.Grid(tris)
+    print(f"grid rows of {S} lane-side points", flush=True)
+    ref, ref_d = native.visibility_dense(src, snk, tris, any_hit=False, want_dist=True)
+    mb = native.unpack_bits(ref, len(snk))
+    assert 0.05 < mb.mean() < 0.95
+    for any_hit in (True, False):
+        b, d = g.visibility(src, snk, any_hit=any_hit, sink_major=True, want_dist=not any_hit)
+        m = native.unpack_bits(b, S).T
+        assert np.array_equal(m, mb), f"any_hit={any_hit}: {(m != mb).sum()} mismatches"
+        if not any_hit:
+            assert np.array_equal(d.cpu().numpy().T, ref_d.cpu().numpy())
+
+
 def test_grid_sink_major_layout(native):
     src, snk, tris = _small_tri_scene(3, S=130, K=70)
     g = native.Grid(tris)
