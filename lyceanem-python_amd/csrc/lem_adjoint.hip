@@ -147,19 +147,18 @@ __global__ __launch_bounds__(AB) void k_adjoint_partial(
                 const float uwi = ux * w.xi + uy * w.yi + uz * w.zi;
                 const float ugr = ux * gxr + uy * gyr + uz * gzr;  // d.g; U.d = (ugr, -ugi)
                 const float ugi = ux * gxi + uy * gyi + uz * gzi;
-                const float Uwr = gxr * w.xr + gxi * w.xi + gyr * w.yr + gyi * w.yi +
-                                  gzr * w.zr + gzi * w.zi;
-                const float Uwi = gxr * w.xi - gxi * w.xr + gyr * w.yi - gyi * w.yr +
-                                  gzr * w.zi - gzi * w.zr;
-                // U.Q = U.w - (U.d)(d.w),  Q = P(d) w
-                const float UQr = Uwr - (ugr * uwr + ugi * uwi);
-                const float UQi = Uwi - (ugr * uwi - ugi * uwr);
+                // P(d) g
+                const float Pxr = gxr - ux * ugr, Pxi = gxi - ux * ugi;
+                const float Pyr = gyr - uy * ugr, Pyi = gyi - uy * ugi;
+                const float Pzr = gzr - uz * ugr, Pzi = gzi - uz * ugi;
+                // U.Q with Q = P(d) w: P is real and symmetric, so U.Q = conj(P(d) g).w
+                const float UQr = Pxr * w.xr + Pxi * w.xi + Pyr * w.yr + Pyi * w.yi +
+                                  Pzr * w.zr + Pzi * w.zi;
+                const float UQi = Pxr * w.xi - Pxi * w.xr + Pyr * w.yi - Pyi * w.yr +
+                                  Pzr * w.zi - Pzi * w.zr;
                 // weight gradient: conj(F cos) P(d) g
                 {
                     const float Lr = Fr * cs, Li = Fi * cs;
-                    const float Pxr = gxr - ux * ugr, Pxi = gxi - ux * ugi;
-                    const float Pyr = gyr - uy * ugr, Pyi = gyi - uy * ugi;
-                    const float Pzr = gzr - uz * ugr, Pzi = gzi - uz * ugi;
                     h[0] = h[0] + Lr * Pxr; h[0] = h[0] + Li * Pxi;
                     h[1] = h[1] + Lr * Pxi; h[1] = h[1] - Li * Pxr;
                     h[2] = h[2] + Lr * Pyr; h[2] = h[2] + Li * Pyi;
