@@ -229,9 +229,24 @@ __global__ __launch_bounds__(AB) void k_adjoint_finish(const float *__restrict__
         return;
     const int i = (int)(t / S);
     const int64_t s = t - i * S;
+    // eight chunks' loads issued together, added in chunk order
+    const float *ps = ws_sum + (int64_t)i * S + s, *pc = ws_cmp + (int64_t)i * S + s;
+    const int64_t step = 9 * S;
     double a = 0.0;
-    for (int64_t c = 0; c < NC; ++c)
-        a += (double)ws_sum[(c * 9 + i) * S + s] - (double)ws_cmp[(c * 9 + i) * S + s];
+    int64_t c = 0;
+    for (; c + 8 <= NC; c += 8) {
+        float vs[8], vc[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            vs[u] = ps[(c + u) * step];
+            vc[u] = pc[(c + u) * step];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            a += (double)vs[u] - (double)vc[u];
+    }
+    for (; c < NC; ++c)
+        a += (double)ps[c * step] - (double)pc[c * step];
     if (i < 6) {
         if (grad_w)
             grad_w[6 * s + i] = (float)a;
