@@ -560,14 +560,10 @@ LEM_DEV float fkey_dec(unsigned int k)
     return __builtin_bit_cast(float, u);
 }
 
-__global__ __launch_bounds__(GB) void k_morton_keys(const float *__restrict__ p, int64_t n,
-                                                    const unsigned int *__restrict__ b,
-                                                    uint32_t *__restrict__ keys,
-                                                    int32_t *__restrict__ vals)
+// 30-bit Morton key of point i in the box of the bound keys b (min x, y, z,
+// max x, y, z as fkey values)
+LEM_DEV uint32_t morton_key(const float *__restrict__ p, int64_t i, const unsigned int *b)
 {
-    const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
-    if (i >= n)
-        return;
     float lo[3], ext = 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -582,13 +578,58 @@ __global__ __launch_bounds__(GB) void k_morton_keys(const float *__restrict__ p,
         const uint32_t q = f >= 1023.0f ? 1023u : (f > 0.0f ? (uint32_t)f : 0u);  // NaN -> 0
         key |= spread3(q) << a;
     }
-    keys[i] = key;
+    return key;
+}
+
+__global__ __launch_bounds__(GB) void k_morton_keys(const float *__restrict__ p, int64_t n,
+                                                    const unsigned int *__restrict__ b,
+                                                    uint32_t *__restrict__ keys,
+                                                    int32_t *__restrict__ vals)
+{
+    const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (i >= n)
+        return;
+    keys[i] = morton_key(p, i, b);
     vals[i] = (int32_t)i;
 }
 
-// Workspace of lem_visibility_grid: the task counter and point bounds, then
-// (unless the input order is kept) keys, values and order of the lane-side
-// points and the radix sort's temporary storage.
+// Bounds and keys of up to MORTON_SMALL points in one workgroup (one launch
+// where k_bounds_init, k_bounds and k_morton_keys take three); the same keys.
+constexpr int MORTON_SMALL = 65536;
+constexpr int MB = 1024;
+
+__global__ __launch_bounds__(MB) void k_morton_small(const float *__restrict__ p, int64_t n,
+                                                     uint32_t *__restrict__ keys,
+                                                     int32_t *__restrict__ vals)
+{
+    __shared__ unsigned int sb[6];
+    if (threadIdx.x < 3) {
+        sb[threadIdx.x] = 0xffffffffu;
+        sb[3 + threadIdx.x] = 0u;
+    }
+    __syncthreads();
+    unsigned int lmin[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
+    unsigned int lmax[3] = {0u, 0u, 0u};
+    for (int64_t i = threadIdx.x; i < n; i += MB) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const unsigned int k = fkey(p[3 * i + a]);
+            lmin[a] = k < lmin[a] ? k : lmin[a];
+            lmax[a] = k > lmax[a] ? k : lmax[a];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        atomicMin(&sb[a], lmin[a]);
+        atomicMax(&sb[3 + a], lmax[a]);
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < n; i += MB) {
+        keys[i] = morton_key(p, i, sb);
+        vals[i] = (int32_t)i;
+    }
+}
+
 // inv[order[slot]] = slot
 __global__ __launch_bounds__(GB) void k_invert_order(const int32_t *__restrict__ order,
                                                      int64_t n, int32_t *__restrict__ inv)
@@ -632,6 +673,9 @@ __global__ __launch_bounds__(GB) void k_unpermute_rows(unsigned long long *__res
 // largest row (in words) k_unpermute_rows stages in LDS
 constexpr int64_t UNPERM_MAX_WORDS = 8192;  // 64 KB
 
+// Workspace of lem_visibility_grid: the task counter and point bounds, then
+// (unless the input order is kept) keys, values and order of the lane-side
+// points and the radix sort's temporary storage.
 struct VisWs {
     unsigned long long *counter;
     unsigned int *bounds;
@@ -681,11 +725,15 @@ inline int64_t vis_layout(int64_t n, size_t cub_bytes, char *base, VisWs *L)
 // L.order = the Morton order of n points (the workspace laid out by vis_layout).
 int morton_order(const float *pts, int64_t n, const VisWs &L, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, st, L.bounds);
-    hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(ceil_div(n, GB), 1024)),
-                       dim3(GB), 0, st, pts, 3 * n, L.bounds);
-    hipLaunchKernelGGL(k_morton_keys, dim3((unsigned)ceil_div(n, GB)), dim3(GB), 0, st, pts, n,
-                       L.bounds, L.k0, L.v0);
+    if (n <= MORTON_SMALL) {
+        hipLaunchKernelGGL(k_morton_small, dim3(1), dim3(MB), 0, st, pts, n, L.k0, L.v0);
+    } else {
+        hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, st, L.bounds);
+        hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(ceil_div(n, GB), 1024)),
+                           dim3(GB), 0, st, pts, 3 * n, L.bounds);
+        hipLaunchKernelGGL(k_morton_keys, dim3((unsigned)ceil_div(n, GB)), dim3(GB), 0, st, pts,
+                           n, L.bounds, L.k0, L.v0);
+    }
     int rc = check_launch("k_morton_keys");
     if (rc)
         return rc;
