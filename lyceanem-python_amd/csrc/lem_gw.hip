@@ -64,8 +64,10 @@ __global__ __launch_bounds__(256) void k_gw_mfma(
 
     for (int64_t s0 = 0; s0 < S; s0 += GW_KC) {
         __syncthreads();
-        // --- G tile: 32 sources x 32 sinks, 4 pairs per lane
-#pragma unroll
+        // --- G tile: 32 sources x 32 sinks, 4 pairs per lane, one at a time:
+        // 78 VGPRs + 48 accumulation registers fit 4 waves per SIMD (unrolled,
+        // 97 + 48 allowed 3; 2.6 % slower)
+#pragma unroll 1
         for (int i = 0; i < 4; ++i) {
             const int sl = (tid >> 5) + 8 * i;
             const int64_t s = s0 + sl;
