@@ -80,10 +80,21 @@ __global__ __launch_bounds__(GB) void k_bounds(const float *__restrict__ tris, i
             lmax[a] = k > lmax[a] ? k : lmax[a];
         }
     }
+    // wave minima / maxima first, then one LDS atomic per wave and bound
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        atomicMin(&smin[a], lmin[a]);
-        atomicMax(&smax[a], lmax[a]);
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned int m0 = __shfl_xor(lmin[a], off), m1 = __shfl_xor(lmax[a], off);
+            lmin[a] = m0 < lmin[a] ? m0 : lmin[a];
+            lmax[a] = m1 > lmax[a] ? m1 : lmax[a];
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&smin[a], lmin[a]);
+            atomicMax(&smax[a], lmax[a]);
+        }
     }
     __syncthreads();
     if (threadIdx.x < 3) {
@@ -618,10 +629,21 @@ __global__ __launch_bounds__(MB) void k_morton_small(const float *__restrict__ p
             lmax[a] = k > lmax[a] ? k : lmax[a];
         }
     }
+    // wave minima / maxima first: 16 LDS atomics per bound instead of 1024
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        atomicMin(&sb[a], lmin[a]);
-        atomicMax(&sb[3 + a], lmax[a]);
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned int m0 = __shfl_xor(lmin[a], off), m1 = __shfl_xor(lmax[a], off);
+            lmin[a] = m0 < lmin[a] ? m0 : lmin[a];
+            lmax[a] = m1 > lmax[a] ? m1 : lmax[a];
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&sb[a], lmin[a]);
+            atomicMax(&sb[3 + a], lmax[a]);
+        }
     }
     __syncthreads();
     for (int64_t i = threadIdx.x; i < n; i += MB) {
