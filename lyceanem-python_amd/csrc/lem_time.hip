@@ -74,8 +74,9 @@ LEM_DEV double loss_mag(const float *__restrict__ pos, const float *__restrict__
     const double inv = 1.0 / (double)len;
     const double inv32 = (double)(float)inv;
     const double re = alpha + inv32, im = beta;
-    return (1.0 / (2.0 * M_PI)) * sqrt(re * re + im * im) * exp(-alpha * (double)len) * inv *
-           fabs((double)dot);
+    // lossless: e^{-0 len} = 1 without evaluating the exponential
+    const double att = alpha != 0.0 ? exp(-alpha * (double)len) : 1.0;
+    return (1.0 / (2.0 * M_PI)) * sqrt(re * re + im * im) * att * inv * fabs((double)dot);
 }
 
 // (I - d d^T) v for the unit float64 direction a -> b (calc_dv into a float64
@@ -85,8 +86,8 @@ LEM_DEV void project_dir(const float *__restrict__ pos, int64_t a, int64_t b, do
     const double dx = (double)(pos[3 * b] - pos[3 * a]);
     const double dy = (double)(pos[3 * b + 1] - pos[3 * a + 1]);
     const double dz = (double)(pos[3 * b + 2] - pos[3 * a + 2]);
-    const double n = sqrt(dx * dx + dy * dy + dz * dz);
-    const double ux = dx / n, uy = dy / n, uz = dz / n;
+    const double in = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+    const double ux = dx * in, uy = dy * in, uz = dz * in;
     const double p = v[0] * ux + v[1] * uy + v[2] * uz;
     v[0] -= p * ux;
     v[1] -= p * uy;
@@ -116,8 +117,8 @@ LEM_DEV void magnetic_launch(const float *__restrict__ pos, int64_t a, int64_t b
 LEM_DEV void project_normal64(const float *__restrict__ nrm, int64_t a, double v[3])
 {
     const double mx = nrm[3 * a], my = nrm[3 * a + 1], mz = nrm[3 * a + 2];
-    const double m = sqrt(mx * mx + my * my + mz * mz);
-    const double nx = mx / m, ny = my / m, nz = mz / m;
+    const double im = 1.0 / sqrt(mx * mx + my * my + mz * mz);
+    const double nx = mx * im, ny = my * im, nz = mz * im;
     const double p = v[0] * nx + v[1] * ny + v[2] * nz;
     v[0] -= p * nx;
     v[1] -= p * ny;
