@@ -39,21 +39,22 @@ def _loss64(oracle, w, pos, sc, visible, in_beam, ref_power=None):
     return _db_mask_loss_torch(E, torch.from_numpy(in_beam), ref_power=ref_power)
 
 
-@pytest.mark.parametrize("ref_power", [None, 3.0])
-def test_db_mask_loss_matches_torch_reference(native, ref_power):
+@pytest.mark.parametrize("K,n_beam,ref_power", [(5000, 300, None), (5000, 300, 3.0),
+                                                 (7, 2, None), (300000, 900, None)])
+def test_db_mask_loss_matches_torch_reference(native, K, n_beam, ref_power):
     """lem_db_mask_loss (loss and dloss/dE in one pass) against autograd of the
-    torch composition in float64 on the same complex64 field: a beam of 300
-    sinks, side lobes on both sides of the -25 dB bar, a zero-power sink."""
+    torch composition in float64 on the same complex64 field: a beam, side lobes
+    on both sides of the -25 dB bar, a zero-power sink; from one workgroup
+    (K = 7) to more sinks than the 1,024 workgroups cover in one pass."""
     from lyceanem.differentiable import db_mask_loss
 
-    rng = np.random.default_rng(31)
-    K = 5000
+    rng = np.random.default_rng(31 + K)
     amp = 10.0 ** (rng.uniform(-4.5, 0.0, K) / 2)
     E0 = (amp[:, None] * (rng.normal(size=(K, 3)) + 1j * rng.normal(size=(K, 3)))).astype(
         np.complex64)
-    E0[17] = 0
+    E0[min(17, K - 1)] = 0
     in_beam = np.zeros(K, bool)
-    in_beam[:300] = True
+    in_beam[:n_beam] = True
     dev = native.device()
     E = torch.from_numpy(E0).to(dev).requires_grad_(True)
     loss = db_mask_loss(E, torch.from_numpy(in_beam).to(dev), ref_power=ref_power)
@@ -67,7 +68,7 @@ def test_db_mask_loss_matches_torch_reference(native, ref_power):
     live = np.abs(g64).sum(-1) > 0
     print(f"dB mask loss: loss {rl:.2e}, gradient {rg:.2e} normwise, {int(live.sum())} of {K} "
           f"sinks with a gradient", flush=True)
-    assert 0.1 < live.mean() < 0.9
+    assert 0.1 < live.mean() < 0.95
     assert (g[~live] == 0).all()
     assert rl < 1e-5 and rg < 1e-5
 

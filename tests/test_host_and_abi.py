@@ -196,6 +196,15 @@ def test_library_rejects_bad_arguments_without_gpu():
     assert rc == -1
     rc = lib.lem_field_los_noalloc(1, 1, 10, 1, 1, None, kmax + 1, None, 0.0, 1.0, 0, 1, None)
     assert rc == -1 and b"scratch" in lib.lem_last_error()
+    # the one-pass loss: no sinks, missing pointers, a workspace below its size
+    rc = lib.lem_db_mask_loss(None, None, 0, -25.0, 0.0, 1e-30, None, None, None, 0, None)
+    assert rc == -1 and b"no sinks" in lib.lem_last_error()
+    rc = lib.lem_db_mask_loss(None, None, 10, -25.0, 0.0, 1e-30, None, None, None, 0, None)
+    assert rc == -1 and b"null pointer" in lib.lem_last_error()
+    wsb = lib.lem_db_mask_loss_workspace(10)
+    assert wsb >= 256 + 4
+    rc = lib.lem_db_mask_loss(1, 1, 10, -25.0, 0.0, 1e-30, 1, None, 1, wsb - 1, None)
+    assert rc != 0 and b"workspace" in lib.lem_last_error()
 
 
 def test_product_fails_loudly_without_gpu():
