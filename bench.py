@@ -590,7 +590,8 @@ class C4:
         self.S, self.K = len(sc["src"]), len(sc["snk"])
         # the forward over a spatial order of the sinks (fixed sinks and mask)
         from lyceanem.differentiable import FarFieldPlan
-        self.plan = FarFieldPlan(self.snk, self.snk_n, self.bits, center=sc["src"].mean(0))
+        self.plan = FarFieldPlan(self.snk, self.snk_n, self.bits, center=sc["src"].mean(0),
+                                 sources=self.pos)
         # the gradient step (forward, loss, adjoint: all on the device) captured
         # once in a HIP graph; each timed step copies the current weights and
         # positions into the captured leaves and replays it
@@ -664,7 +665,10 @@ class C4:
                       "forward_ms": fwd, "loss_and_adjoint_ms": bwd,
                       "adjoint_over_forward": bwd / fwd,
                       "gradient_step_over_forward": (fwd + bwd) / fwd,
-                      "forward_gpairs_per_s": pairs / (fwd * 1e-3) / 1e9},
+                      "forward_gpairs_per_s": pairs / (fwd * 1e-3) / 1e9,
+                      "adjoint_source_order": "mask" if self.plan.src_order is not None
+                      else "kernel",
+                      "adjoint_wave_sink_pairs": self.plan.adjoint_costs},
         }
 
     def cpu_baseline(self, budget_s):

@@ -194,8 +194,10 @@ def test_adjoint_flags_match_autograd(native, oracle):
 
 def test_far_field_plan_is_bit_identical(native):
     """FarFieldPlan runs the forward over a spatial sink order with the mask's
-    columns permuted: each sink's sum is over the same sources in the same
-    order, so field and gradients equal the unplanned ones bit for bit."""
+    columns permuted, and the adjoint over a mask-driven source order with the
+    mask's rows permuted: each sink's sum is over the same sources, and each
+    source's over the same sinks, in the same order, so field and gradients
+    equal the unplanned ones bit for bit."""
     from lyceanem.differentiable import FarFieldPlan, far_field, db_mask_loss
 
     sc = _small_c4()
@@ -206,16 +208,24 @@ def test_far_field_plan_is_bit_identical(native):
     in_beam = torch.from_numpy(sc["in_beam"]).to(dev)
     plan = FarFieldPlan(snk, snk_n, bits, center=sc["src"].mean(0))
     assert not torch.equal(plan.order.cpu(), torch.arange(len(snk)))
+    # the adjoint over the mask-driven source order (forced) and the automatic choice
+    plan_m = FarFieldPlan(snk, snk_n, bits, center=sc["src"].mean(0), adjoint_order="mask")
+    assert plan_m.src_order is not None
+    assert not torch.equal(plan_m.src_order.cpu(), torch.arange(len(sc["src"])))
+    plan_a = FarFieldPlan(snk, snk_n, bits, center=sc["src"].mean(0),
+                          sources=torch.from_numpy(sc["src"]).to(dev))
+    print(f"adjoint wave-sink pairs per order: {plan_a.adjoint_costs}", flush=True)
     out = []
-    for p in (None, plan):
+    for p in (None, plan, plan_m, plan_a):
         w = torch.from_numpy(sc["w"]).to(dev).requires_grad_(True)
         pos = torch.from_numpy(sc["src"]).to(dev).requires_grad_(True)
         E = far_field(w, pos, snk, snk_n, hit_bits=bits, beta=sc["beta"], plan=p)
         loss = db_mask_loss(E, in_beam)
         loss.backward()
         out.append((E.detach().cpu(), w.grad.cpu(), pos.grad.cpu()))
-    for a, b in zip(out[0], out[1]):
-        assert torch.equal(a, b)
+    for o in out[1:]:
+        for a, b in zip(out[0], o):
+            assert torch.equal(a, b)
 
 
 def test_gradient_step_replays_in_a_graph(native):
@@ -231,7 +241,7 @@ def test_gradient_step_replays_in_a_graph(native):
     snk = torch.from_numpy(sc["snk"]).to(dev)
     snk_n = torch.from_numpy(sc["snk_n"]).to(dev)
     in_beam = torch.from_numpy(sc["in_beam"]).to(dev)
-    plan = FarFieldPlan(snk, snk_n, bits)
+    plan = FarFieldPlan(snk, snk_n, bits, adjoint_order="mask")
     w0 = torch.from_numpy(sc["w"]).to(dev)
     p0 = torch.from_numpy(sc["src"]).to(dev)
 
