@@ -666,9 +666,10 @@ class C4:
                       "adjoint_over_forward": bwd / fwd,
                       "gradient_step_over_forward": (fwd + bwd) / fwd,
                       "forward_gpairs_per_s": pairs / (fwd * 1e-3) / 1e9,
-                      "adjoint_source_order": "mask" if self.plan.src_order is not None
-                      else "kernel",
-                      "adjoint_wave_sink_pairs": self.plan.adjoint_costs},
+                      "plan_sink_order": self.plan.sink_order,
+                      "plan_forward_ms": self.plan.forward_times,
+                      "plan_source_order": self.plan.adjoint_order,
+                      "plan_adjoint_ms": self.plan.adjoint_times},
         }
 
     def cpu_baseline(self, budget_s):

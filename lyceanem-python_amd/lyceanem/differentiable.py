@@ -78,19 +78,6 @@ def _wave_sink_cost(hit_bits, K, order, chunk=512):
     return total
 
 
-def morton_order_points(p):
-    """A 3-D Morton order of points (10 bits per axis over their bounding box)."""
-    p = torch.as_tensor(p).to(torch.float64)
-    lo, hi = p.min(dim=0).values, p.max(dim=0).values
-    ext = float((hi - lo).max().clamp_min(1e-300))
-    q = ((p - lo) / ext * 1023.0).clamp(0, 1023).to(torch.int64)
-    key = torch.zeros(p.shape[0], dtype=torch.int64, device=p.device)
-    for b in range(10):
-        for a in range(3):
-            key |= ((q[:, a] >> b) & 1) << (3 * b + a)
-    return torch.argsort(key, stable=True)
-
-
 def source_mask_order(hit_bits, K, sink_dirs, chunk=512):
     """An order of the sources that puts sources with similar mask rows in the
     same wavefront: by the direction of the mean of the sink directions each
@@ -115,54 +102,132 @@ def source_mask_order(hit_bits, K, sink_dirs, chunk=512):
     return torch.argsort(key)
 
 
+def sink_azimuth_order(sinks, center=None):
+    """The sinks by the azimuth of their direction seen from `center` (1023
+    levels), then its polar angle, then index: a wavefront's lanes are one
+    azimuth column of a far-field grid."""
+    p = torch.as_tensor(sinks).to(torch.float64)
+    if center is not None:
+        p = p - torch.as_tensor(center, dtype=torch.float64, device=p.device)
+    r = torch.linalg.vector_norm(p, dim=1).clamp_min(1e-300)
+    th = torch.acos((p[:, 2] / r).clamp(-1.0, 1.0))
+    ph = torch.atan2(p[:, 1], p[:, 0])
+    qa = ((ph + np.pi) / (2 * np.pi) * 1023.0).round().to(torch.int64)
+    qe = (th / np.pi * 1048575.0).round().to(torch.int64)
+    key = (qa << 52) | (qe << 31) | torch.arange(p.shape[0], device=p.device)
+    return torch.argsort(key)
+
+
+def _timed(fn, reps=3):
+    """Mean device time of fn() in ms (one untimed call first)."""
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
 class FarFieldPlan:
     """A fixed sink set and mask prepared for repeated far-field evaluations
-    (beam synthesis).  Forward: the kernel runs over the sinks in
-    `sink_tile_order` with the mask's columns permuted to match, and the field
-    comes back in the caller's order.  Each sink's sum covers the same sources
-    in the same order, so the field is bit-identical to the unplanned one; it
-    is faster because a wave's lanes agree on the mask (on the 4,096-element
-    cylinder and a 181 x 361 far-field grid: 0.566 -> 0.457 ms).  Adjoint (lanes
-    over sources): the sources go in `source_mask_order` when that leaves the
-    waves fewer (wave, sink) pairs to walk than the kernel's own Morton order of
-    the source positions (given as `sources`) and the input order; the
-    gradients come back in the caller's order, bit-identical (each source's sum
-    over the sinks is unchanged).  adjoint_order: "auto" (that choice), "mask"
-    (always the mask order) or "kernel" (the kernel's own order)."""
+    (beam synthesis).
+
+    Forward (lanes over sinks): the kernel runs over a spatial order of the
+    sinks with the mask's columns permuted to match, and the field comes back
+    in the caller's order.  Each sink's sum covers the same sources in the same
+    order, so the field is bit-identical to the unplanned one; it is faster
+    because a wave's lanes agree on the mask and the workgroups get even work.
+    Candidates: `sink_tile_order` (2-D Morton of the directions),
+    `sink_azimuth_order` and the input order.  Adjoint (lanes over sources):
+    the sources go in `source_mask_order`, the kernel's own Morton order of the
+    positions, or the input order; the gradients come back in the caller's
+    order, bit-identical (each source's sum over the sinks is unchanged).
+
+    With `sources` (the element positions) the plan times each candidate once
+    with unit weights and keeps the fastest (on the 4,096-element cylinder and
+    a 181 x 361 grid: the azimuth order for the forward, the mask order for the
+    adjoint); without, it keeps the Morton sink order and the mask order when
+    that leaves fewer (wave, sink) pairs to walk than the input order.
+    sink_order / adjoint_order force a candidate ("tile", "azimuth", "input" /
+    "mask", "kernel", "input")."""
 
     def __init__(self, sinks, sink_normals, hit_bits=None, center=None, sources=None,
-                 adjoint_order="auto"):
-        if adjoint_order not in ("auto", "mask", "kernel"):
-            raise ValueError("adjoint_order: 'auto', 'mask' or 'kernel'")
+                 adjoint_order="auto", sink_order="auto"):
+        if adjoint_order not in ("auto", "mask", "kernel", "input"):
+            raise ValueError("adjoint_order: 'auto', 'mask', 'kernel' or 'input'")
+        if sink_order not in ("auto", "tile", "azimuth", "input"):
+            raise ValueError("sink_order: 'auto', 'tile', 'azimuth' or 'input'")
         self.K = sinks.shape[0]
-        self.order = sink_tile_order(sinks, center).to(sinks.device)
-        self.sinks = sinks[self.order].contiguous()
-        self.sink_normals = sink_normals[self.order].contiguous()
-        self.hit_bits = None if hit_bits is None else permute_mask_sinks(hit_bits, self.K,
-                                                                         self.order)
+        dev = sinks.device
+        self.forward_times = self.adjoint_times = self.adjoint_costs = None
+        timing = sources is not None and dev.type == "cuda"
+        if timing:
+            pos = torch.as_tensor(sources, dtype=torch.float32).to(dev).reshape(-1, 3)
+            w1 = torch.ones((pos.shape[0], 3), dtype=torch.complex64, device=dev)
+        # --- forward: sink order
+        cands = {"tile": lambda: sink_tile_order(sinks, center),
+                 "azimuth": lambda: sink_azimuth_order(sinks, center),
+                 "input": lambda: torch.arange(self.K, device=dev)}
+        names = [sink_order] if sink_order != "auto" else (
+            list(cands) if timing and hit_bits is not None else ["tile"])
+        best = None
+        for name in names:
+            order = cands[name]().to(dev)
+            snk_p = sinks[order].contiguous()
+            nrm_p = sink_normals[order].contiguous()
+            bits_p = None if hit_bits is None else permute_mask_sinks(hit_bits, self.K, order)
+            t = 0.0
+            if len(names) > 1:
+                t = _timed(lambda: _native.field_los(pos, w1, snk_p, nrm_p, hit_bits=bits_p))
+                self.forward_times = dict(self.forward_times or {}, **{name: t})
+            if best is None or t < best[0]:
+                best = (t, name, order, snk_p, nrm_p, bits_p)
+        _, self.sink_order, self.order, self.sinks, self.sink_normals, self.hit_bits = best
         self.inverse = torch.empty_like(self.order)
-        self.inverse[self.order] = torch.arange(self.K, device=self.order.device)
+        self.inverse[self.order] = torch.arange(self.K, device=dev)
+        # --- adjoint: source order
         self.src_order = self.src_inverse = self.src_bits = None
-        self.adjoint_costs = None
-        if hit_bits is not None and adjoint_order != "kernel":
-            dev = hit_bits.device
-            S = hit_bits.shape[0]
-            c = torch.zeros(3, dtype=torch.float64, device=sinks.device) if center is None \
-                else torch.as_tensor(center, dtype=torch.float64, device=sinks.device)
-            d = sinks.to(torch.float64) - c
-            d = d / torch.linalg.vector_norm(d, dim=1, keepdim=True).clamp_min(1e-300)
-            cand = {"mask": source_mask_order(hit_bits, self.K, d.to(dev))}
-            if adjoint_order == "auto":
-                cand["input"] = torch.arange(S, device=dev)
-                if sources is not None:
-                    cand["morton"] = morton_order_points(sources).to(dev)
-            costs = {k: _wave_sink_cost(hit_bits, self.K, o) for k, o in cand.items()}
-            self.adjoint_costs = costs
-            if costs["mask"] < min([v for k, v in costs.items() if k != "mask"] + [float("inf")]):
-                self.src_order = cand["mask"]
-                self.src_inverse = torch.empty_like(self.src_order)
-                self.src_inverse[self.src_order] = torch.arange(S, device=dev)
-                self.src_bits = hit_bits[self.src_order].contiguous()
+        self.adjoint_order = "kernel"
+        if hit_bits is None or adjoint_order == "kernel":
+            return
+        S = hit_bits.shape[0]
+        c = torch.zeros(3, dtype=torch.float64, device=dev) if center is None \
+            else torch.as_tensor(center, dtype=torch.float64, device=dev)
+        d = sinks.to(torch.float64) - c
+        d = d / torch.linalg.vector_norm(d, dim=1, keepdim=True).clamp_min(1e-300)
+        orders = {"mask": source_mask_order(hit_bits, self.K, d.to(hit_bits.device)),
+                  "input": torch.arange(S, device=hit_bits.device)}
+        if adjoint_order != "auto":
+            choice = adjoint_order
+        elif timing:
+            g1 = torch.ones((self.K, 3), dtype=torch.complex64, device=dev)
+
+            def run(name):
+                if name == "kernel":
+                    return lambda: _native.field_los_adjoint(pos, w1, sinks, sink_normals, g1,
+                                                             hit_bits=hit_bits)
+                o = orders[name]
+                bo = hit_bits[o].contiguous()
+                return lambda: _native.field_los_adjoint(pos[o], w1, sinks, sink_normals, g1,
+                                                         hit_bits=bo, input_order=True)
+
+            self.adjoint_times = {n: _timed(run(n)) for n in ("kernel", "mask", "input")}
+            choice = min(self.adjoint_times, key=self.adjoint_times.get)
+        else:
+            self.adjoint_costs = {k: _wave_sink_cost(hit_bits, self.K, o)
+                                  for k, o in orders.items()}
+            choice = "mask" if self.adjoint_costs["mask"] < self.adjoint_costs["input"] \
+                else "kernel"
+        self.adjoint_order = choice
+        if choice in orders:
+            self.src_order = orders[choice]
+            self.src_inverse = torch.empty_like(self.src_order)
+            self.src_inverse[self.src_order] = torch.arange(S, device=self.src_order.device)
+            self.src_bits = hit_bits[self.src_order].contiguous()
 
     def forward_field(self, w, pos, alpha, beta):
         E = _native.field_los(pos, w, self.sinks, self.sink_normals, hit_bits=self.hit_bits,

@@ -214,9 +214,13 @@ def test_far_field_plan_is_bit_identical(native):
     assert not torch.equal(plan_m.src_order.cpu(), torch.arange(len(sc["src"])))
     plan_a = FarFieldPlan(snk, snk_n, bits, center=sc["src"].mean(0),
                           sources=torch.from_numpy(sc["src"]).to(dev))
-    print(f"adjoint wave-sink pairs per order: {plan_a.adjoint_costs}", flush=True)
+    print(f"plan with sources: forward {plan_a.sink_order} {plan_a.forward_times}, adjoint "
+          f"{plan_a.adjoint_order} {plan_a.adjoint_times}", flush=True)
+    plan_z = FarFieldPlan(snk, snk_n, bits, center=sc["src"].mean(0), sink_order="azimuth",
+                          adjoint_order="input")
+    assert plan_z.sink_order == "azimuth" and plan_z.src_order is not None
     out = []
-    for p in (None, plan, plan_m, plan_a):
+    for p in (None, plan, plan_m, plan_a, plan_z):
         w = torch.from_numpy(sc["w"]).to(dev).requires_grad_(True)
         pos = torch.from_numpy(sc["src"]).to(dev).requires_grad_(True)
         E = far_field(w, pos, snk, snk_n, hit_bits=bits, beta=sc["beta"], plan=p)
