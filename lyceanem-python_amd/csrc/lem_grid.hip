@@ -438,7 +438,7 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
     const int32_t *__restrict__ order, unsigned long long *__restrict__ bits,
     float *__restrict__ dist_out, unsigned long long *__restrict__ next_task, int skip_self,
     unsigned long long *__restrict__ par_count, uint2 *__restrict__ par_queue, int64_t par_cap,
-    int perm)
+    int perm, const int32_t *__restrict__ outer_order)
 {
     __shared__ uint32_t smqa[MACRO_LDS_MAX];
     __shared__ uint16_t smqb[MACRO_LDS_MAX];
@@ -473,13 +473,16 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
         // scalar and the walk keeps its vector registers
         t = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned int)(t >> 32)) << 32) |
             (unsigned long long)__builtin_amdgcn_readfirstlane((unsigned int)t);
-        const int64_t outer = (int64_t)(t / (unsigned long long)UPR);
-        if (outer >= NO) {
+        const int64_t oc = (int64_t)(t / (unsigned long long)UPR);
+        if (oc >= NO) {
             if (++tried == 8)
                 break;
             owner = (owner + 1) & 7;
             continue;
         }
+        // claims walk the outer points in a spatial order (consecutive claims
+        // share cells in L2); rows are written at the point's own index
+        const int64_t outer = outer_order ? (int64_t)outer_order[oc] : oc;
         const int64_t w0 = owner * CH + (int64_t)(t % (unsigned long long)UPR) * CLAIM_WORDS;
         const int64_t end = (int64_t)(owner + 1) * CH < W ? (int64_t)(owner + 1) * CH : W;
         const int64_t w1 = w0 + CLAIM_WORDS < end ? w0 + CLAIM_WORDS : end;
@@ -1061,11 +1064,28 @@ extern "C" int lem_grid_fill(const float *tris, int64_t T, const lem_grid_info *
     return check_launch("lem_grid_fill");
 }
 
+// Workspace region of the outer points' order (after the lane-side layout):
+// keys, values, order and the sort's storage for NO points.
+inline int64_t outer_region(int64_t NO, size_t cub_bytes, char *base, VisWs *L)
+{
+    auto al = [](int64_t b) { return ((b + 255) / 256) * 256; };
+    if (L) {
+        L->k0 = reinterpret_cast<uint32_t *>(base);
+        L->k1 = reinterpret_cast<uint32_t *>(base + al(4 * NO));
+        L->v0 = reinterpret_cast<int32_t *>(base + 2 * al(4 * NO));
+        L->order = reinterpret_cast<int32_t *>(base + 3 * al(4 * NO));
+        L->cub = base + 4 * al(4 * NO);
+        L->cub_bytes = cub_bytes;
+    }
+    return 4 * al(4 * NO) + al((int64_t)cub_bytes);
+}
+
 extern "C" int64_t lem_visibility_grid_workspace(int64_t S, int64_t K, int flags)
 {
     if (S < 0 || K < 0)
         return -1;
     const int64_t NI = (flags & LEM_VIS_SINK_MAJOR) ? S : K;
+    const int64_t NO = (flags & LEM_VIS_SINK_MAJOR) ? K : S;
     int64_t base = VIS_HEAD;
     if (!(flags & LEM_VIS_INPUT_ORDER) && NI > 0) {
         if (NI > 2147483647LL)
@@ -1073,7 +1093,13 @@ extern "C" int64_t lem_visibility_grid_workspace(int64_t S, int64_t K, int flags
         const int64_t cub = vis_sort_bytes(NI);
         if (cub < 0)
             return -1;
-        base = vis_layout(NI, (size_t)cub, nullptr, nullptr);
+        base = ceil_div(vis_layout(NI, (size_t)cub, nullptr, nullptr), 256) * 256;
+        if (NO > 0 && NO <= 2147483647LL) {
+            const int64_t cubo = vis_sort_bytes(NO);
+            if (cubo < 0)
+                return -1;
+            base += outer_region(NO, (size_t)cubo, nullptr, nullptr);
+        }
     }
     // the queue of rays along grid planes (k_visibility_grid_par) comes last
     return ceil_div(base, 256) * 256 + PAR_QUEUE_BYTES;
@@ -1122,7 +1148,7 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
     VisWs L;
     const int64_t cub = (flags & LEM_VIS_INPUT_ORDER) ? 0 : vis_sort_bytes(NI);
     vis_layout(NI, (size_t)std::max<int64_t>(cub, 0), reinterpret_cast<char *>(workspace), &L);
-    const int32_t *order = nullptr;
+    const int32_t *order = nullptr, *outer_order = nullptr;
     // persistent walk over a spatial order with rows that fit in LDS: each wave
     // stores its 64 bits in lane-slot order and the rows are reordered after
     // the walk (no per-ray atomics, no clearing); otherwise bits are ORed per
@@ -1149,6 +1175,24 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
         if (rc)
             return rc;
         order = L.order;
+        if (info->nmacro <= MACRO_LDS_MAX && NO <= 2147483647LL) {
+            // the persistent walk's claims also follow a spatial order of the
+            // outer points
+            const int64_t cubo = vis_sort_bytes(NO);
+            if (cubo < 0) {
+                set_error("lem_visibility_grid: radix sort size query failed");
+                return E_HIP;
+            }
+            const int64_t off =
+                ceil_div(vis_layout(NI, (size_t)std::max<int64_t>(cub, 0), nullptr, nullptr), 256) *
+                256;
+            VisWs Lo = L;
+            outer_region(NO, (size_t)cubo, reinterpret_cast<char *>(workspace) + off, &Lo);
+            rc = morton_order(sink_major ? snk : src, NO, Lo, st);
+            if (rc)
+                return rc;
+            outer_order = Lo.order;
+        }
     }
     if (info->nmacro <= MACRO_LDS_MAX) {
         // persistent launch: as many workgroups as are resident at once.  The
@@ -1179,7 +1223,7 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
         hipLaunchKernelGGL((k_visibility_grid_lds<A, M, SM, R>), dim3((unsigned)nwg), dim3(GB), 0, \
                            st, src, S, snk, K, G, cell_start, rc4, bx4, mq, info->nmacro, order, \
                            hb, dist, L.counter, skip_self, pcount, pq, PAR_QUEUE_CAP,          \
-                           perm ? 1 : 0);                                                      \
+                           perm ? 1 : 0, outer_order);                                         \
         hipLaunchKernelGGL((k_visibility_grid_par<A, M, SM, R>), dim3((unsigned)nwg), dim3(GB), 0, \
                            st, src, S, snk, K, G, cell_start, rc4, bx4, mq, info->nmacro, order, \
                            hb, dist, skip_self, pcount, pq, PAR_QUEUE_CAP, perm ? 1 : 0);      \
