@@ -210,11 +210,15 @@ int lem_grid_fill(const float *tris, int64_t T, const lem_grid_info *info, int32
  * LEM_SEM_NATIVE is any-hit (no distance).  Unless LEM_VIS_INPUT_ORDER is
  * set, the lane-side points are first sorted into a 3-D Morton order so that
  * each wavefront's 64 rays are spatially compact; results are written at the
- * original indices (hit_bits is cleared and bits are set atomically).  The
- * workspace (lem_visibility_grid_workspace(S, K, flags) bytes) holds the task
- * counters of the persistent traversal (one per XCD: each XCD walks a
+ * original indices.  With the persistent traversal (macro table in LDS) and
+ * rows of at most 524,288 lane-side points, each wavefront stores its 64 bits
+ * as one word in lane order and the rows are reordered in place afterwards;
+ * otherwise hit_bits is cleared and bits are set atomically.  The persistent
+ * traversal also claims rows in a Morton order of the other side's points.
+ * The workspace (lem_visibility_grid_workspace(S, K, flags) bytes) holds the
+ * task counters of the persistent traversal (one per XCD: each XCD walks a
  * contiguous eighth of every row of rays, so its L2 keeps one region's cells)
- * and the sort: calls that overlap on one workspace race. */
+ * and both orders and their sorts: calls that overlap on one workspace race. */
 int64_t lem_visibility_grid_workspace(int64_t S, int64_t K, int flags);
 int lem_visibility_grid(const float *src, int64_t S, const float *snk, int64_t K,
                         const lem_grid_info *info, const int32_t *cell_start, const float *recs,
