@@ -343,6 +343,12 @@ __global__ __launch_bounds__(GB) void k_visibility_grid(
 }
 
 constexpr int LDS_WG_PER_CU = 6;  // 6 x 25.5 KB of the 160 KB LDS of a CU
+// Any-hit walks whose macro table has at most MACRO_LDS_MAX7 blocks run 7
+// workgroups per CU (7 x 21.8 KB of LDS, 72 registers): the C3 one-bounce
+// masks gain 4 %.  The closest-hit walk keeps 6: at 72 registers it spills
+// 14 and loses 6 %.
+constexpr int MACRO_LDS_MAX7 = 3712;
+constexpr int lds_table(int wpc) { return wpc >= 7 ? MACRO_LDS_MAX7 : MACRO_LDS_MAX; }
 
 // Rays of the persistent traversal with a zero direction component (along a
 // grid plane), queued by k_visibility_grid_lds as (outer, lane slot): walked
@@ -425,13 +431,13 @@ constexpr int COUNTER_STRIDE = 512;  // uint64 elements
 constexpr int CLAIM_WORDS = 3;       // 64-ray words per claim
 
 // Persistent form of k_visibility_grid: each workgroup copies the quantised
-// macro table (at most MACRO_LDS_MAX blocks) into LDS once; then each
+// macro table (at most lds_table(WPC) blocks) into LDS once; then each
 // wavefront repeatedly claims a 64-ray word of output (one atomic on one of
 // the 8 counters next_task[0..7]), so waves never wait for one another and
 // the load balances whatever number of workgroups is resident.  Dead-block
-// tests read LDS.
-template <bool ANY, bool SINK_MAJOR, int SEM, bool REV = false>
-__global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
+// tests read LDS.  WPC: workgroups per CU the registers are bounded for.
+template <bool ANY, bool SINK_MAJOR, int SEM, bool REV = false, int WPC = LDS_WG_PER_CU>
+__global__ __launch_bounds__(GB, WPC) void k_visibility_grid_lds(
     const float *__restrict__ src, int64_t S, const float *__restrict__ snk, int64_t K, GridDev G,
     const int32_t *__restrict__ cell_start, const float4 *__restrict__ recs,
     const float4 *__restrict__ box, const uint2 *__restrict__ mq, int nmacro,
@@ -440,8 +446,8 @@ __global__ __launch_bounds__(GB, LDS_WG_PER_CU) void k_visibility_grid_lds(
     unsigned long long *__restrict__ par_count, uint2 *__restrict__ par_queue, int64_t par_cap,
     int perm, const int32_t *__restrict__ outer_order)
 {
-    __shared__ uint32_t smqa[MACRO_LDS_MAX];
-    __shared__ uint16_t smqb[MACRO_LDS_MAX];
+    __shared__ uint32_t smqa[lds_table(WPC)];
+    __shared__ uint16_t smqb[lds_table(WPC)];
     __shared__ float4 saxis[3];
     for (int i = threadIdx.x; i < nmacro; i += GB) {
         const uint2 q = mq[i];
@@ -1196,8 +1202,8 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
     }
     if (info->nmacro <= MACRO_LDS_MAX) {
         // persistent launch: as many workgroups as are resident at once.  The
-        // registers are bounded for LDS_WG_PER_CU waves per SIMD (launch
-        // bounds), so the LDS per CU sets the count.
+        // registers are bounded for wpc waves per SIMD (launch bounds), so the
+        // LDS per CU sets the count.
         int dev = 0, cus = 0, lds = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -1206,8 +1212,10 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
             set_error("lem_visibility_grid: device query failed");
             return E_HIP;
         }
-        const int per_cu = std::max(1, std::min(LDS_WG_PER_CU,
-                                                lds / (int)(MACRO_LDS_MAX * (sizeof(uint32_t) + sizeof(uint16_t)))));
+        const bool w7 = any && semantics != LEM_SEM_NATIVE && info->nmacro <= MACRO_LDS_MAX7;
+        const int wpc = w7 ? 7 : LDS_WG_PER_CU;
+        const int per_cu = std::max(
+            1, std::min(wpc, lds / (int)(lds_table(wpc) * (sizeof(uint32_t) + sizeof(uint16_t)))));
         const int64_t words = NO * ceil_div(NI, 64);
         const int64_t nwg =
             std::max<int64_t>(1, std::min<int64_t>(ceil_div(words, 4), (int64_t)cus * per_cu));
@@ -1218,9 +1226,9 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
         unsigned long long *pcount = reinterpret_cast<unsigned long long *>(qb);
         uint2 *pq = reinterpret_cast<uint2 *>(qb + 256);
         hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(GB), 0, st, pcount, 1);
-#define LEM_VGL(A, M, SM, R)                                                                  \
+#define LEM_VGL(A, M, SM, R, WP)                                                              \
     do {                                                                                      \
-        hipLaunchKernelGGL((k_visibility_grid_lds<A, M, SM, R>), dim3((unsigned)nwg), dim3(GB), 0, \
+        hipLaunchKernelGGL((k_visibility_grid_lds<A, M, SM, R, WP>), dim3((unsigned)nwg), dim3(GB), 0, \
                            st, src, S, snk, K, G, cell_start, rc4, bx4, mq, info->nmacro, order, \
                            hb, dist, L.counter, skip_self, pcount, pq, PAR_QUEUE_CAP,          \
                            perm ? 1 : 0, outer_order);                                         \
@@ -1228,25 +1236,29 @@ extern "C" int lem_visibility_grid(const float *src, int64_t S, const float *snk
                            st, src, S, snk, K, G, cell_start, rc4, bx4, mq, info->nmacro, order, \
                            hb, dist, skip_self, pcount, pq, PAR_QUEUE_CAP, perm ? 1 : 0);      \
     } while (0)
-#define LEM_VGL2(A, SM, R)                                                                    \
+#define LEM_VGL2(A, SM, R, WP)                                                                \
     do {                                                                                      \
         if (sink_major)                                                                       \
-            LEM_VGL(A, true, SM, R);                                                          \
+            LEM_VGL(A, true, SM, R, WP);                                                      \
         else                                                                                  \
-            LEM_VGL(A, false, SM, R);                                                         \
+            LEM_VGL(A, false, SM, R, WP);                                                     \
     } while (0)
         if (semantics == LEM_SEM_NATIVE) {
             if (rev)
-                LEM_VGL2(true, 1, true);
+                LEM_VGL2(true, 1, true, LDS_WG_PER_CU);
             else
-                LEM_VGL2(true, 1, false);
+                LEM_VGL2(true, 1, false, LDS_WG_PER_CU);
         } else if (any) {
-            if (rev)
-                LEM_VGL2(true, 0, true);
+            if (rev && w7)
+                LEM_VGL2(true, 0, true, 7);
+            else if (rev)
+                LEM_VGL2(true, 0, true, LDS_WG_PER_CU);
+            else if (w7)
+                LEM_VGL2(true, 0, false, 7);
             else
-                LEM_VGL2(true, 0, false);
+                LEM_VGL2(true, 0, false, LDS_WG_PER_CU);
         } else {
-            LEM_VGL2(false, 0, false);
+            LEM_VGL2(false, 0, false, LDS_WG_PER_CU);
         }
 #undef LEM_VGL2
 #undef LEM_VGL
