@@ -428,7 +428,11 @@ constexpr int64_t PAR_QUEUE_BYTES = 256 + 8 * PAR_QUEUE_CAP;
 // The 8 task counters sit 4 KB apart: device-scope atomics on one cache line
 // serialise where the line lives, which capped claiming near 82 M per second.
 constexpr int COUNTER_STRIDE = 512;  // uint64 elements
-constexpr int CLAIM_WORDS = 3;       // 64-ray words per claim
+// 64-ray words per claim.  With the rows claimed in a spatial order of the
+// outer points, longer claims keep a wave on neighbouring rays longer:
+// 3 / 8 / 12 / 16 / 32 words gave C2 (1M-triangle terrain) 1,105 / 1,084 /
+// 1,083 / 1,089 / 1,093 ms and C3 (one bounce) 758 / 738 / 736 / 733 / 732 ms.
+constexpr int CLAIM_WORDS = 12;
 
 // Persistent form of k_visibility_grid: each workgroup copies the quantised
 // macro table (at most lds_table(WPC) blocks) into LDS once; then each
