@@ -483,7 +483,35 @@ __global__ __launch_bounds__(GB, WPC) void k_visibility_grid_lds(
         // scalar and the walk keeps its vector registers
         t = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned int)(t >> 32)) << 32) |
             (unsigned long long)__builtin_amdgcn_readfirstlane((unsigned int)t);
-        const int64_t oc = (int64_t)(t / (unsigned long long)UPR);
+        int64_t oc = (int64_t)(t / (unsigned long long)UPR);
+        int64_t u = (int64_t)(t % (unsigned long long)UPR);
+        if constexpr (!ANY) {
+            // Closest hit: when an XCD's share of a row is long (C2: 171
+            // claims), walking it whole for one row sweeps more triangles
+            // through the XCD's L2 than it holds; claims then interleave 64
+            // spatially neighbouring rows (the index runs over the group's
+            // rows first, then along their words), so the waves in flight
+            // share a compact patch of lane-side points (C2 +2 %).  Each t
+            // below UPR * NO maps to one (row, claim), larger t to oc = NO.
+            // The any-hit walks keep one row at a time: their shares are
+            // short on the one-bounce legs (1-11 claims), where interleaving
+            // measured 0.6 % slower and its divisions cost them registers.
+            constexpr int64_t GROUP = 64;
+            if (UPR >= GROUP) {
+                const int64_t full = (NO / GROUP) * GROUP;
+                if (t < (unsigned long long)(UPR * full)) {
+                    const int64_t r = (int64_t)(t % (unsigned long long)(UPR * GROUP));
+                    u = r / GROUP;
+                    oc = (int64_t)(t / (unsigned long long)(UPR * GROUP)) * GROUP + r % GROUP;
+                } else {
+                    // the last NO - full rows, interleaved the same way
+                    const int64_t t2 = (int64_t)(t - (unsigned long long)(UPR * full));
+                    const int64_t G2 = NO - full;
+                    u = G2 > 0 ? t2 / G2 : 0;
+                    oc = (G2 > 0 && u < UPR) ? full + t2 % G2 : NO;
+                }
+            }
+        }
         if (oc >= NO) {
             if (++tried == 8)
                 break;
@@ -493,7 +521,7 @@ __global__ __launch_bounds__(GB, WPC) void k_visibility_grid_lds(
         // claims walk the outer points in a spatial order (consecutive claims
         // share cells in L2); rows are written at the point's own index
         const int64_t outer = outer_order ? (int64_t)outer_order[oc] : oc;
-        const int64_t w0 = owner * CH + (int64_t)(t % (unsigned long long)UPR) * CLAIM_WORDS;
+        const int64_t w0 = owner * CH + u * CLAIM_WORDS;
         const int64_t end = (int64_t)(owner + 1) * CH < W ? (int64_t)(owner + 1) * CH : W;
         const int64_t w1 = w0 + CLAIM_WORDS < end ? w0 + CLAIM_WORDS : end;
         for (int64_t word = w0; word < w1; ++word) {

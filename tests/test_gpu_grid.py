@@ -127,6 +127,26 @@ def test_grid_mask_rows_at_the_lds_limit(native, S):
             assert np.array_equal(d.cpu().numpy().T, ref_d.cpu().numpy())
 
 
+def test_grid_claims_interleave_rows(native):
+    """Closest-hit walks over rows long enough (400,000 lane-side points: 66
+    claims per XCD share) claim them in interleaved groups of 64 rows; 100
+    sinks give one full group and a last group of 36 (any-hit walks take one
+    row at a time).  Masks and distances equal the brute-force kernel."""
+    S, K = 400_000, 100
+    src, snk, tris = _small_tri_scene(21, n_tris=500, S=S, K=K)
+    g = native.Grid(tris)
+    print(f"interleaved claims: {S} x {K}", flush=True)
+    ref, ref_d = native.visibility_dense(src, snk, tris, any_hit=False, want_dist=True)
+    mb = native.unpack_bits(ref, K)
+    assert 0.05 < mb.mean() < 0.95
+    for any_hit in (True, False):
+        b, d = g.visibility(src, snk, any_hit=any_hit, sink_major=True, want_dist=not any_hit)
+        m = native.unpack_bits(b, S).T
+        assert np.array_equal(m, mb), f"any_hit={any_hit}: {(m != mb).sum()} mismatches"
+        if not any_hit:
+            assert np.array_equal(d.cpu().numpy().T, ref_d.cpu().numpy())
+
+
 def test_grid_sink_major_layout(native):
     src, snk, tris = _small_tri_scene(3, S=130, K=70)
     g = native.Grid(tris)
